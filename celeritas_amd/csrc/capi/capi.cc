@@ -1,0 +1,499 @@
+//----------------------------------*-C++-*----------------------------------//
+// celeritas_amd: MI355X-native per-step transport loop
+//---------------------------------------------------------------------------//
+//! \file capi/capi.cc
+//! C ABI over the host C++ classes (include/celeritas_amd.h).
+//---------------------------------------------------------------------------//
+#include "celeritas_amd.h"
+
+#include <cstring>
+#include <string>
+
+#include "global/CoreParams.hh"
+#include "global/CoreState.hh"
+#include "global/Stepper.hh"
+#include "track/TrackActions.hh"
+#include "track/TrackKernels.hh"
+
+using namespace celeritas_amd;
+
+static_assert(sizeof(CelerPrimary) == sizeof(Primary), "Primary layout");
+static_assert(sizeof(CelerCounters) == sizeof(CoreCounters), "Counters layout");
+static_assert(sizeof(CelerTrackInitializer) == sizeof(TrackInitializer),
+              "TrackInitializer layout");
+static_assert(sizeof(CelerSecondary) == sizeof(Secondary), "Secondary layout");
+
+struct CelerParams
+{
+    std::shared_ptr<CoreParams> params;
+};
+
+struct CelerState
+{
+    std::shared_ptr<CoreParams> params;
+    std::shared_ptr<CoreState> state;
+};
+
+struct CelerStepper
+{
+    std::shared_ptr<CoreParams> params;
+    std::shared_ptr<Stepper> stepper;
+    CelerState state_handle;
+};
+
+namespace
+{
+thread_local std::string g_last_error;
+
+template<class F>
+int guarded(F&& f)
+{
+    try
+    {
+        g_last_error.clear();
+        f();
+        return 0;
+    }
+    catch (std::exception const& e)
+    {
+        g_last_error = e.what();
+    }
+    catch (...)
+    {
+        g_last_error = "unknown C++ exception";
+    }
+    return 1;
+}
+
+template<class T, class F>
+T* guarded_new(F&& f)
+{
+    T* result = nullptr;
+    guarded([&] { result = f(); });
+    return result;
+}
+
+//! Device pointer and byte size of a state field
+std::pair<void*, std::size_t> field_ptr(CoreStateRef const& s,
+                                        CoreParams const& p,
+                                        CelerStateField f,
+                                        size_type num_detectors)
+{
+    std::size_t const n = s.size;
+    switch (f)
+    {
+        case CELER_FIELD_TRACK_ID: return {s.sim.track_id, 4 * n};
+        case CELER_FIELD_PARENT_ID: return {s.sim.parent_id, 4 * n};
+        case CELER_FIELD_EVENT_ID: return {s.sim.event_id, 4 * n};
+        case CELER_FIELD_NUM_STEPS: return {s.sim.num_steps, 4 * n};
+        case CELER_FIELD_TIME: return {s.sim.time, 8 * n};
+        case CELER_FIELD_STATUS: return {s.sim.status, n};
+        case CELER_FIELD_STEP_LENGTH: return {s.sim.step_length, 8 * n};
+        case CELER_FIELD_POST_STEP_ACTION: return {s.sim.post_step_action, n};
+        case CELER_FIELD_ALONG_STEP_ACTION: return {s.sim.along_step_action, n};
+        case CELER_FIELD_PARTICLE_ID: return {s.particle.particle_id, n};
+        case CELER_FIELD_ENERGY: return {s.particle.energy, 8 * n};
+        case CELER_FIELD_POS_X: return {s.geo.pos[0], 8 * n};
+        case CELER_FIELD_POS_Y: return {s.geo.pos[1], 8 * n};
+        case CELER_FIELD_POS_Z: return {s.geo.pos[2], 8 * n};
+        case CELER_FIELD_DIR_X: return {s.geo.dir[0], 8 * n};
+        case CELER_FIELD_DIR_Y: return {s.geo.dir[1], 8 * n};
+        case CELER_FIELD_DIR_Z: return {s.geo.dir[2], 8 * n};
+        case CELER_FIELD_VOLUME: return {s.geo.volume, 4 * n};
+        case CELER_FIELD_SURFACE: return {s.geo.surface, 4 * n};
+        case CELER_FIELD_INTERACTION_MFP: return {s.phys.interaction_mfp, 8 * n};
+        case CELER_FIELD_SEC_START: return {s.phys.sec_start, 4 * n};
+        case CELER_FIELD_SEC_COUNT: return {s.phys.sec_count, 4 * n};
+        case CELER_FIELD_ENERGY_DEPOSITION:
+            return {s.phys.energy_deposition, 8 * n};
+        case CELER_FIELD_RNG_XS0:
+        case CELER_FIELD_RNG_XS1:
+        case CELER_FIELD_RNG_XS2:
+        case CELER_FIELD_RNG_XS3:
+        case CELER_FIELD_RNG_XS4:
+            return {s.rng.xs[f - CELER_FIELD_RNG_XS0], 4 * n};
+        case CELER_FIELD_RNG_WEYL: return {s.rng.weyl, 4 * n};
+        case CELER_FIELD_VACANCIES: return {s.init.vacancies, 4 * n};
+        case CELER_FIELD_SECONDARY_COUNTS:
+            return {s.init.secondary_counts, 4 * (n + 1)};
+        case CELER_FIELD_TRACK_COUNTERS:
+            return {s.init.track_counters, 4 * std::size_t(s.init.max_events)};
+        case CELER_FIELD_INITIALIZERS:
+            return {s.init.initializers,
+                    sizeof(TrackInitializer) * std::size_t(s.init.capacity)};
+        case CELER_FIELD_SECONDARIES:
+            return {s.secondaries.storage,
+                    sizeof(Secondary) * std::size_t(s.secondaries.capacity)};
+        case CELER_FIELD_CALO_EDEP:
+            return {s.calo_edep, 8 * std::size_t(num_detectors)};
+        case CELER_FIELD_MACRO_XS: return {s.phys.macro_xs, 8 * n};
+        case CELER_FIELD_DEDX_RANGE: return {s.phys.dedx_range, 8 * n};
+        case CELER_FIELD_TRACK_SLOTS: return {s.track_slots, 4 * n};
+        case CELER_FIELD_ACTION_OFFSETS:
+            return {s.action_offsets,
+                    4 * std::size_t(p.action_reg()->num_actions() + 2)};
+        case CELER_FIELD_NEXT_STEP: return {s.geo.next_step, 8 * n};
+        case CELER_FIELD_GEO_FAILED: return {s.geo.failed, n};
+        default: break;
+    }
+    throw RuntimeError("invalid state field " + std::to_string(int(f)));
+}
+
+std::shared_ptr<ExtendFromPrimariesAction const> primaries_action(CoreParams const& p)
+{
+    auto id = p.action_reg()->find_action(ExtendFromPrimariesAction::label_str());
+    CELER_VALIDATE(id != invalid_id, << "no primary action");
+    return std::dynamic_pointer_cast<ExtendFromPrimariesAction const>(
+        p.action_reg()->action(id));
+}
+}  // namespace
+
+extern "C" {
+
+//---------------------------------------------------------------------------//
+const char* celer_amd_version(void)
+{
+    return "0.1.0";
+}
+
+int celer_amd_device_available(void)
+{
+    return device_available() ? 1 : 0;
+}
+
+const char* celer_amd_last_error(void)
+{
+    return g_last_error.c_str();
+}
+
+//---------------------------------------------------------------------------//
+CelerParams* celer_params_create(const char* json)
+{
+    return guarded_new<CelerParams>([&] {
+        require_device();
+        return new CelerParams{CoreParams::from_json(json)};
+    });
+}
+
+void celer_params_destroy(CelerParams* params)
+{
+    delete params;
+}
+
+uint32_t celer_params_num_actions(const CelerParams* params)
+{
+    return params->params->action_reg()->num_actions();
+}
+
+int celer_params_action_label(const CelerParams* params,
+                              uint32_t action_id,
+                              char* buf,
+                              size_t len)
+{
+    return guarded([&] {
+        std::string const& s = params->params->action_reg()->id_to_label(action_id);
+        std::snprintf(buf, len, "%s", s.c_str());
+    });
+}
+
+uint32_t celer_params_find_action(const CelerParams* params, const char* label)
+{
+    return params->params->action_reg()->find_action(label);
+}
+
+int celer_params_action_order(const CelerParams* params, uint32_t action_id)
+{
+    auto const& a = params->params->action_reg()->action(action_id);
+    if (auto sa = std::dynamic_pointer_cast<StepActionInterface const>(a))
+        return static_cast<int>(sa->order());
+    return -1;
+}
+
+uint32_t celer_params_find_particle(const CelerParams* params, int pdg)
+{
+    return params->params->physics().find_particle(pdg);
+}
+
+uint32_t celer_params_num_volumes(const CelerParams* params)
+{
+    return params->params->geometry().num_volumes();
+}
+
+const void* celer_params_device_ref(const CelerParams* params)
+{
+    return &params->params->device_ref();
+}
+
+//---------------------------------------------------------------------------//
+CelerState* celer_state_create(CelerParams* params, uint32_t stream_id, uint32_t num_slots)
+{
+    return guarded_new<CelerState>([&] {
+        auto st = std::make_shared<CoreState>(*params->params, stream_id, num_slots);
+        return new CelerState{params->params, std::move(st)};
+    });
+}
+
+void celer_state_destroy(CelerState* state)
+{
+    delete state;
+}
+
+uint32_t celer_state_size(const CelerState* state)
+{
+    return state->state->size();
+}
+
+const void* celer_state_device_ref(const CelerState* state)
+{
+    return &state->state->ref();
+}
+
+void* celer_state_stream(const CelerState* state)
+{
+    return state->state->stream();
+}
+
+int celer_state_insert_primaries(CelerState* state,
+                                 const CelerPrimary* primaries,
+                                 uint32_t count)
+{
+    return guarded([&] {
+        primaries_action(*state->params)
+            ->insert(*state->params, *state->state,
+                     reinterpret_cast<Primary const*>(primaries), count);
+    });
+}
+
+int celer_state_execute_action(CelerState* state, uint32_t action_id)
+{
+    return guarded([&] {
+        auto const& a = state->params->action_reg()->action(action_id);
+        auto sa = std::dynamic_pointer_cast<StepActionInterface const>(a);
+        CELER_VALIDATE(sa, << "action " << action_id << " is not a step action");
+        sa->step(*state->params, *state->state);
+    });
+}
+
+int celer_state_counters(CelerState* state, CelerCounters* out)
+{
+    return guarded([&] {
+        CoreCounters const& c = state->state->sync_counters();
+        std::memcpy(out, &c, sizeof(CoreCounters));
+    });
+}
+
+int celer_state_set_counters(CelerState* state, const CelerCounters* in)
+{
+    return guarded([&] {
+        CELER_HIP_CALL(hipMemcpyAsync(state->state->ref().counters, in,
+                                      sizeof(CoreCounters), hipMemcpyHostToDevice,
+                                      state->state->stream()));
+        state->state->sync_counters();
+    });
+}
+
+int celer_state_copy_field(CelerState* state,
+                           CelerStateField field,
+                           int direction,
+                           void* host,
+                           size_t bytes)
+{
+    return guarded([&] {
+        auto [ptr, size] = field_ptr(state->state->ref(), *state->params, field,
+                                     state->state->num_detectors());
+        CELER_VALIDATE(bytes <= size, << "requested " << bytes
+                                      << " bytes from a field of " << size);
+        hipStream_t s = state->state->stream();
+        if (direction == 0)
+            CELER_HIP_CALL(hipMemcpyAsync(host, ptr, bytes, hipMemcpyDeviceToHost, s));
+        else
+            CELER_HIP_CALL(hipMemcpyAsync(ptr, host, bytes, hipMemcpyHostToDevice, s));
+        CELER_HIP_CALL(hipStreamSynchronize(s));
+    });
+}
+
+int celer_state_reseed(CelerState* state, uint64_t event_id)
+{
+    return guarded([&] {
+        auto const& p = *state->params;
+        CELER_HIP_CALL(launch_reseed_rng(state->state->ref().rng,
+                                         p.device_ref().rng_jump, p.rng().seed(),
+                                         event_id, state->state->stream()));
+        CELER_HIP_CALL(hipStreamSynchronize(state->state->stream()));
+    });
+}
+
+int celer_state_rng_initialize(CelerState* state,
+                               const uint64_t* subsequence,
+                               const uint64_t* offset)
+{
+    return guarded([&] {
+        auto const& p = *state->params;
+        std::size_t n = state->state->size();
+        DeviceVector<ull_int> sub(n), off(n);
+        sub.copy_from_host(reinterpret_cast<ull_int const*>(subsequence), n);
+        off.copy_from_host(reinterpret_cast<ull_int const*>(offset), n);
+        CELER_HIP_CALL(launch_rng_initialize(state->state->ref().rng,
+                                             p.device_ref().rng_jump, p.rng().seed(),
+                                             sub.data(), off.data(),
+                                             state->state->stream()));
+        CELER_HIP_CALL(hipStreamSynchronize(state->state->stream()));
+    });
+}
+
+int celer_state_rng_draw(CelerState* state, uint32_t draws, int kind, void* host_out)
+{
+    return guarded([&] {
+        CELER_VALIDATE(kind >= 0 && kind <= 2, << "bad draw kind");
+        std::size_t n = std::size_t(state->state->size()) * draws;
+        std::size_t bytes = n * (kind == 1 ? 8 : 4);
+        DeviceVector<char> buf(bytes);
+        CELER_HIP_CALL(launch_rng_draw(state->state->ref().rng, draws, kind,
+                                       buf.data(), state->state->stream()));
+        CELER_HIP_CALL(hipMemcpyAsync(host_out, buf.data(), bytes,
+                                      hipMemcpyDeviceToHost, state->state->stream()));
+        CELER_HIP_CALL(hipStreamSynchronize(state->state->stream()));
+    });
+}
+
+int celer_state_reset(CelerState* state)
+{
+    return guarded([&] { state->state->reset(); });
+}
+
+int celer_state_synchronize(CelerState* state)
+{
+    return guarded([&] { CELER_HIP_CALL(hipStreamSynchronize(state->state->stream())); });
+}
+
+//---------------------------------------------------------------------------//
+CelerStepper* celer_stepper_create(CelerParams* params,
+                                   uint32_t stream_id,
+                                   uint32_t num_slots,
+                                   int action_times)
+{
+    return guarded_new<CelerStepper>([&] {
+        StepperInput inp;
+        inp.params = params->params;
+        inp.stream_id = stream_id;
+        inp.num_track_slots = num_slots;
+        inp.action_times = action_times != 0;
+        auto* result = new CelerStepper;
+        result->params = params->params;
+        result->stepper = std::make_shared<Stepper>(inp);
+        result->state_handle.params = params->params;
+        result->state_handle.state = result->stepper->sp_state();
+        return result;
+    });
+}
+
+void celer_stepper_destroy(CelerStepper* stepper)
+{
+    delete stepper;
+}
+
+int celer_stepper_step(CelerStepper* stepper,
+                       const CelerPrimary* primaries,
+                       uint32_t count,
+                       CelerStepperResult* result)
+{
+    return guarded([&] {
+        StepperResult r;
+        if (primaries && count > 0)
+            r = (*stepper->stepper)(reinterpret_cast<Primary const*>(primaries), count);
+        else
+            r = (*stepper->stepper)();
+        if (result)
+        {
+            result->generated = r.generated;
+            result->queued = r.queued;
+            result->active = r.active;
+            result->alive = r.alive;
+        }
+    });
+}
+
+int celer_stepper_warm_up(CelerStepper* stepper)
+{
+    return guarded([&] { stepper->stepper->warm_up(); });
+}
+
+int celer_stepper_kill_active(CelerStepper* stepper)
+{
+    return guarded([&] { stepper->stepper->kill_active(); });
+}
+
+int celer_stepper_reseed(CelerStepper* stepper, uint64_t event_id)
+{
+    return guarded([&] { stepper->stepper->reseed(event_id); });
+}
+
+CelerState* celer_stepper_state(CelerStepper* stepper)
+{
+    return &stepper->state_handle;
+}
+
+int celer_stepper_action_sequence(const CelerStepper* stepper, char* buf, size_t len)
+{
+    return guarded([&] {
+        std::string s;
+        for (auto const& a : stepper->stepper->actions())
+        {
+            s += std::string(a->label());
+            s += '\n';
+        }
+        std::snprintf(buf, len, "%s", s.c_str());
+    });
+}
+
+//---------------------------------------------------------------------------//
+int celer_hip_extend_from_primaries(const void* state_ref,
+                                    const CelerPrimary* device_primaries,
+                                    uint32_t count,
+                                    void* stream)
+{
+    return launch_extend_from_primaries(
+        *static_cast<CoreStateRef const*>(state_ref),
+        reinterpret_cast<Primary const*>(device_primaries), count,
+        static_cast<hipStream_t>(stream));
+}
+
+int celer_hip_initialize_tracks(const void* params_ref, const void* state_ref, void* stream)
+{
+    return launch_initialize_tracks(*static_cast<CoreParamsRef const*>(params_ref),
+                                    *static_cast<CoreStateRef const*>(state_ref),
+                                    static_cast<hipStream_t>(stream));
+}
+
+int celer_hip_extend_from_secondaries(const void* params_ref,
+                                      const void* state_ref,
+                                      void* stream)
+{
+    return launch_extend_from_secondaries(
+        *static_cast<CoreParamsRef const*>(params_ref),
+        *static_cast<CoreStateRef const*>(state_ref), static_cast<hipStream_t>(stream));
+}
+
+int celer_hip_reseed(const void* params_ref,
+                     const void* state_ref,
+                     uint64_t event_id,
+                     void* stream)
+{
+    auto const& p = *static_cast<CoreParamsRef const*>(params_ref);
+    auto const& s = *static_cast<CoreStateRef const*>(state_ref);
+    return launch_reseed_rng(s.rng, p.rng_jump, p.rng_seed, event_id,
+                             static_cast<hipStream_t>(stream));
+}
+
+int celer_params_export_json(const CelerParams*, char* buf, size_t len, size_t* needed)
+{
+    return guarded([&] {
+        std::string s = "{}";
+        if (needed)
+            *needed = s.size() + 1;
+        if (buf && len > 0)
+            std::snprintf(buf, len, "%s", s.c_str());
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,319 @@
+//----------------------------------*-C++-*----------------------------------//
+// celeritas_amd: MI355X-native per-step transport loop
+//---------------------------------------------------------------------------//
+//! \file geo/GeoParams.cc
+//---------------------------------------------------------------------------//
+#include "GeoParams.hh"
+
+#include <algorithm>
+#include <cmath>
+#include <sstream>
+
+namespace celeritas_amd
+{
+namespace
+{
+//---------------------------------------------------------------------------//
+SurfaceType surface_type_from_string(std::string const& s)
+{
+    static char const* const names[] = {"px", "py", "pz", "cxc", "cyc", "czc", "sc", "p"};
+    for (int i = 0; i < static_cast<int>(SurfaceType::size_); ++i)
+    {
+        if (s == names[i])
+            return static_cast<SurfaceType>(i);
+    }
+    throw RuntimeError("unsupported ORANGE surface type '" + s + "'");
+}
+
+//! Axis-aligned box used for interval propagation through the logic
+struct Box
+{
+    double lo[3];
+    double hi[3];
+
+    static Box infinite()
+    {
+        return {{-HUGE_VAL, -HUGE_VAL, -HUGE_VAL}, {HUGE_VAL, HUGE_VAL, HUGE_VAL}};
+    }
+    static Box empty()
+    {
+        return {{HUGE_VAL, HUGE_VAL, HUGE_VAL}, {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL}};
+    }
+};
+Box intersect(Box const& a, Box const& b)
+{
+    Box r;
+    for (int k = 0; k < 3; ++k)
+    {
+        r.lo[k] = std::max(a.lo[k], b.lo[k]);
+        r.hi[k] = std::min(a.hi[k], b.hi[k]);
+    }
+    return r;
+}
+Box unite(Box const& a, Box const& b)
+{
+    Box r;
+    for (int k = 0; k < 3; ++k)
+    {
+        r.lo[k] = std::min(a.lo[k], b.lo[k]);
+        r.hi[k] = std::max(a.hi[k], b.hi[k]);
+    }
+    return r;
+}
+
+//! Bounding boxes of the (negative, positive) sides of a surface
+void halfspace_boxes(SurfaceType t, std::vector<double> const& d, Box* neg, Box* pos)
+{
+    *neg = Box::infinite();
+    *pos = Box::infinite();
+    if (t <= SurfaceType::pz)
+    {
+        int ax = static_cast<int>(t);
+        neg->hi[ax] = d[0];
+        pos->lo[ax] = d[0];
+    }
+    else if (t >= SurfaceType::cxc && t <= SurfaceType::czc)
+    {
+        int ax = static_cast<int>(t) - static_cast<int>(SurfaceType::cxc);
+        double r = std::sqrt(d[0]);
+        for (int k : {(ax + 1) % 3, (ax + 2) % 3})
+        {
+            neg->lo[k] = -r;
+            neg->hi[k] = r;
+        }
+    }
+    else if (t == SurfaceType::sc)
+    {
+        double r = std::sqrt(d[0]);
+        for (int k = 0; k < 3; ++k)
+        {
+            neg->lo[k] = -r;
+            neg->hi[k] = r;
+        }
+    }
+}
+//---------------------------------------------------------------------------//
+}  // namespace
+
+//---------------------------------------------------------------------------//
+std::vector<std::int32_t> parse_logic_string(std::string const& s)
+{
+    std::vector<std::int32_t> result;
+    std::istringstream is(s);
+    std::string tok;
+    while (is >> tok)
+    {
+        if (tok == "~")
+            result.push_back(logic::lnot);
+        else if (tok == "&")
+            result.push_back(logic::land);
+        else if (tok == "|")
+            result.push_back(logic::lor);
+        else if (tok == "*")
+            result.push_back(logic::ltrue);
+        else
+            result.push_back(static_cast<std::int32_t>(std::stol(tok)));
+    }
+    return result;
+}
+
+//---------------------------------------------------------------------------//
+GeoInput geo_input_from_orange_json(JsonValue const& doc)
+{
+    JsonValue const& univ = doc.has("universes") ? doc.at("universes")[0] : doc;
+    CELER_VALIDATE(univ.get_str("_type", "simple unit") == "simple unit",
+                   << "only a single simple unit is supported");
+    GeoInput inp;
+    JsonValue const& surfs = univ.at("surfaces");
+    auto const& types = surfs.at("types");
+    auto const data = surfs.at("data").as_doubles();
+    auto const sizes = surfs.at("sizes").as_ints();
+    std::size_t off = 0;
+    for (std::size_t i = 0; i < types.size(); ++i)
+    {
+        SurfaceType t = surface_type_from_string(types[i].as_string());
+        std::size_t n = static_cast<std::size_t>(sizes[i]);
+        CELER_VALIDATE(static_cast<int>(n) == surface_data_size(t),
+                       << "bad data size for surface " << i);
+        inp.surf_types.push_back(static_cast<std::uint8_t>(t));
+        inp.surf_data.emplace_back(data.begin() + off, data.begin() + off + n);
+        off += n;
+    }
+    if (univ.has("surface_names"))
+    {
+        for (auto const& s : univ.at("surface_names").arr)
+            inp.surface_labels.push_back(s.as_string());
+    }
+    auto const& cells = univ.at("cells");
+    for (std::size_t i = 0; i < cells.size(); ++i)
+    {
+        JsonValue const& c = cells[i];
+        GeoInput::Volume v;
+        for (auto f : c.at("faces").as_ints())
+            v.faces.push_back(static_cast<std::uint32_t>(f));
+        v.logic = parse_logic_string(c.at("logic").as_string());
+        v.flags = static_cast<std::uint8_t>(c.get_int("flags", 0)
+                                            & (internal_surfaces | implicit_vol));
+        v.background = (c.get_int("zorder", 2) == 1);
+        if (univ.has("cell_names"))
+            v.label = univ.at("cell_names")[i].as_string();
+        inp.volumes.push_back(std::move(v));
+    }
+    return inp;
+}
+
+//---------------------------------------------------------------------------//
+GeoParams::GeoParams(GeoInput const& inp)
+{
+    num_surfaces_ = static_cast<size_type>(inp.surf_types.size());
+    num_volumes_ = static_cast<size_type>(inp.volumes.size());
+    CELER_VALIDATE(num_volumes_ > 0, << "geometry has no volumes");
+
+    // Surfaces
+    for (size_type s = 0; s < num_surfaces_; ++s)
+    {
+        h_surf_type.push_back(inp.surf_types[s]);
+        h_surf_offset.push_back(static_cast<std::uint32_t>(h_surf_data.size()));
+        for (double d : inp.surf_data[s])
+            h_surf_data.push_back(d);
+    }
+
+    // Volumes
+    std::vector<std::vector<std::uint32_t>> conn(num_surfaces_);
+    for (size_type v = 0; v < num_volumes_; ++v)
+    {
+        auto const& vol = inp.volumes[v];
+        CELER_VALIDATE(vol.faces.size() <= static_cast<std::size_t>(max_faces),
+                       << "volume " << v << " has " << vol.faces.size()
+                       << " faces (max " << max_faces << ")");
+        h_face_begin.push_back(static_cast<std::uint32_t>(h_faces.size()));
+        h_num_faces.push_back(static_cast<std::uint8_t>(vol.faces.size()));
+        int max_isect = 0;
+        bool simple = true;
+        for (auto s : vol.faces)
+        {
+            CELER_VALIDATE(s < num_surfaces_, << "bad face in volume " << v);
+            h_faces.push_back(s);
+            conn[s].push_back(v);
+            auto t = static_cast<SurfaceType>(inp.surf_types[s]);
+            max_isect += (t <= SurfaceType::pz || t == SurfaceType::p) ? 1 : 2;
+            simple = simple && (t != SurfaceType::p || true);
+        }
+        h_max_isect.push_back(static_cast<std::uint8_t>(max_isect));
+        h_logic_begin.push_back(static_cast<std::uint32_t>(h_logic.size()));
+        h_logic_size.push_back(static_cast<std::uint16_t>(vol.logic.size()));
+        for (auto t : vol.logic)
+        {
+            CELER_VALIDATE(t < 0 || t < static_cast<std::int32_t>(vol.faces.size()),
+                           << "bad logic token in volume " << v);
+            h_logic.push_back(t);
+        }
+        std::uint8_t flags = vol.flags;
+        if (simple && !(flags & implicit_vol))
+            flags |= simple_safety;
+        h_flags.push_back(flags);
+        if (vol.background)
+            background = v;
+        labels_.push_back(vol.label);
+
+        // Bounding box by interval propagation of the postfix logic
+        std::vector<std::pair<Box, Box>> stack;  // (true region, false region)
+        for (auto t : vol.logic)
+        {
+            if (t >= 0)
+            {
+                std::uint32_t s = vol.faces[t];
+                Box neg, pos;
+                halfspace_boxes(static_cast<SurfaceType>(inp.surf_types[s]),
+                                inp.surf_data[s], &neg, &pos);
+                stack.push_back({pos, neg});
+            }
+            else if (t == logic::lnot)
+            {
+                std::swap(stack.back().first, stack.back().second);
+            }
+            else if (t == logic::ltrue)
+            {
+                stack.push_back({Box::infinite(), Box::empty()});
+            }
+            else
+            {
+                CELER_VALIDATE(stack.size() >= 2, << "bad logic in volume " << v);
+                auto b = stack.back();
+                stack.pop_back();
+                auto a = stack.back();
+                stack.pop_back();
+                if (t == logic::land)
+                    stack.push_back({intersect(a.first, b.first),
+                                     unite(a.second, b.second)});
+                else
+                    stack.push_back({unite(a.first, b.first),
+                                     intersect(a.second, b.second)});
+            }
+        }
+        CELER_VALIDATE(stack.size() == 1, << "unbalanced logic in volume " << v);
+        Box const& bb = stack.back().first;
+        for (int k = 0; k < 3; ++k)
+            h_bbox.push_back(bb.lo[k]);
+        for (int k = 0; k < 3; ++k)
+            h_bbox.push_back(bb.hi[k]);
+    }
+    for (size_type s = 0; s < num_surfaces_; ++s)
+    {
+        h_conn_begin.push_back(static_cast<std::uint32_t>(h_conn_vols.size()));
+        h_conn_size.push_back(static_cast<std::uint32_t>(conn[s].size()));
+        for (auto v : conn[s])
+            h_conn_vols.push_back(v);
+    }
+
+    // Upload
+    d_surf_type_ = to_device(h_surf_type);
+    d_surf_offset_ = to_device(h_surf_offset);
+    d_surf_data_ = to_device(h_surf_data);
+    d_face_begin_ = to_device(h_face_begin);
+    d_num_faces_ = to_device(h_num_faces);
+    d_max_isect_ = to_device(h_max_isect);
+    d_faces_ = to_device(h_faces);
+    d_logic_begin_ = to_device(h_logic_begin);
+    d_logic_size_ = to_device(h_logic_size);
+    d_logic_ = to_device(h_logic);
+    d_flags_ = to_device(h_flags);
+    d_bbox_ = to_device(h_bbox);
+    d_conn_begin_ = to_device(h_conn_begin);
+    d_conn_size_ = to_device(h_conn_size);
+    d_conn_vols_ = to_device(h_conn_vols);
+
+    ref_.num_surfaces = num_surfaces_;
+    ref_.num_volumes = num_volumes_;
+    ref_.surf_type = d_surf_type_.data();
+    ref_.surf_offset = d_surf_offset_.data();
+    ref_.surf_data = d_surf_data_.data();
+    ref_.vol_face_begin = d_face_begin_.data();
+    ref_.vol_num_faces = d_num_faces_.data();
+    ref_.vol_max_isect = d_max_isect_.data();
+    ref_.faces = d_faces_.data();
+    ref_.vol_logic_begin = d_logic_begin_.data();
+    ref_.vol_logic_size = d_logic_size_.data();
+    ref_.logic = d_logic_.data();
+    ref_.vol_flags = d_flags_.data();
+    ref_.vol_bbox = d_bbox_.data();
+    ref_.conn_begin = d_conn_begin_.data();
+    ref_.conn_size = d_conn_size_.data();
+    ref_.conn_vols = d_conn_vols_.data();
+    ref_.background = background;
+    ref_.exterior = 0;
+}
+
+//---------------------------------------------------------------------------//
+std::uint32_t GeoParams::find_volume(std::string const& label) const
+{
+    for (size_type v = 0; v < num_volumes_; ++v)
+    {
+        if (labels_[v] == label)
+            return v;
+    }
+    return invalid_id;
+}
+
+//---------------------------------------------------------------------------//
+}  // namespace celeritas_amd

@@ -1,0 +1,107 @@
+//----------------------------------*-C++-*----------------------------------//
+// celeritas_amd: MI355X-native per-step transport loop
+//---------------------------------------------------------------------------//
+//! \file geo/GeoParams.hh
+//---------------------------------------------------------------------------//
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "corecel/DeviceRuntime.hh"
+#include "corecel/Json.hh"
+#include "GeoData.hh"
+
+namespace celeritas_amd
+{
+//---------------------------------------------------------------------------//
+/*!
+ * Host description of one simple unit (the ORANGE JSON "simple unit").
+ */
+struct GeoInput
+{
+    std::vector<std::uint8_t> surf_types;
+    std::vector<std::vector<double>> surf_data;
+    std::vector<std::string> surface_labels;
+
+    struct Volume
+    {
+        std::vector<std::uint32_t> faces;
+        std::vector<std::int32_t> logic;
+        std::uint8_t flags{0};
+        std::string label;
+        bool background{false};
+    };
+    std::vector<Volume> volumes;
+};
+
+//! Parse the first universe of an ORANGE ".org.json" document
+GeoInput geo_input_from_orange_json(JsonValue const& doc);
+
+//! Convert an ORANGE postfix logic string ("0 1 ~ & 2 &") to tokens
+std::vector<std::int32_t> parse_logic_string(std::string const& s);
+
+//---------------------------------------------------------------------------//
+/*!
+ * Geometry params: validated host data plus device arrays.
+ *
+ * Replaces OrangeParams (orange/OrangeParams.hh) for one simple unit; the
+ * BIH used by the reference for point location is replaced by per-volume
+ * bounding boxes derived from the CSG logic (interval propagation through
+ * the postfix expression), tested before the sense evaluation.
+ */
+class GeoParams
+{
+  public:
+    explicit GeoParams(GeoInput const& inp);
+
+    GeoParamsRef const& device_ref() const { return ref_; }
+    size_type num_volumes() const { return num_volumes_; }
+    size_type num_surfaces() const { return num_surfaces_; }
+    std::string const& volume_label(size_type v) const { return labels_[v]; }
+    //! Find a volume by label (invalid_id if absent)
+    std::uint32_t find_volume(std::string const& label) const;
+
+    // Host copies (also exported for the test harness)
+    std::vector<std::uint8_t> h_surf_type;
+    std::vector<std::uint32_t> h_surf_offset;
+    std::vector<double> h_surf_data;
+    std::vector<std::uint32_t> h_face_begin;
+    std::vector<std::uint8_t> h_num_faces;
+    std::vector<std::uint8_t> h_max_isect;
+    std::vector<std::uint32_t> h_faces;
+    std::vector<std::uint32_t> h_logic_begin;
+    std::vector<std::uint16_t> h_logic_size;
+    std::vector<std::int32_t> h_logic;
+    std::vector<std::uint8_t> h_flags;
+    std::vector<double> h_bbox;
+    std::vector<std::uint32_t> h_conn_begin;
+    std::vector<std::uint32_t> h_conn_size;
+    std::vector<std::uint32_t> h_conn_vols;
+    std::uint32_t background{invalid_id};
+
+  private:
+    size_type num_volumes_{0};
+    size_type num_surfaces_{0};
+    std::vector<std::string> labels_;
+
+    DeviceVector<std::uint8_t> d_surf_type_;
+    DeviceVector<std::uint32_t> d_surf_offset_;
+    DeviceVector<double> d_surf_data_;
+    DeviceVector<std::uint32_t> d_face_begin_;
+    DeviceVector<std::uint8_t> d_num_faces_;
+    DeviceVector<std::uint8_t> d_max_isect_;
+    DeviceVector<std::uint32_t> d_faces_;
+    DeviceVector<std::uint32_t> d_logic_begin_;
+    DeviceVector<std::uint16_t> d_logic_size_;
+    DeviceVector<std::int32_t> d_logic_;
+    DeviceVector<std::uint8_t> d_flags_;
+    DeviceVector<double> d_bbox_;
+    DeviceVector<std::uint32_t> d_conn_begin_;
+    DeviceVector<std::uint32_t> d_conn_size_;
+    DeviceVector<std::uint32_t> d_conn_vols_;
+    GeoParamsRef ref_;
+};
+
+//---------------------------------------------------------------------------//
+}  // namespace celeritas_amd

@@ -1,0 +1,63 @@
+//----------------------------------*-C++-*----------------------------------//
+// celeritas_amd: MI355X-native per-step transport loop
+//---------------------------------------------------------------------------//
+//! \file global/CoreActions.cc
+//---------------------------------------------------------------------------//
+#include "CoreActions.hh"
+
+#include "CoreParams.hh"
+#include "CoreState.hh"
+
+namespace celeritas_amd
+{
+//---------------------------------------------------------------------------//
+void AlongStepNeutralAction::step(CoreParams const& params, CoreState& state) const
+{
+    CELER_HIP_CALL(launch_along_step_neutral(
+        params.device_ref(), state.ref(),
+        static_cast<std::uint8_t>(this->action_id()), state.stream()));
+}
+
+void AlongStepChargedAction::step(CoreParams const& params, CoreState& state) const
+{
+    CELER_HIP_CALL(launch_along_step_charged(
+        params.device_ref(), state.ref(),
+        static_cast<std::uint8_t>(this->action_id()), state.stream()));
+}
+
+void BoundaryAction::step(CoreParams const& params, CoreState& state) const
+{
+    CELER_HIP_CALL(launch_boundary(params.device_ref(), state.ref(), state.stream()));
+}
+
+void TrackingCutAction::step(CoreParams const& params, CoreState& state) const
+{
+    CELER_HIP_CALL(
+        launch_tracking_cut(params.device_ref(), state.ref(), state.stream()));
+}
+
+//---------------------------------------------------------------------------//
+std::string make_along_step_action(std::string const& kind,
+                                   PhysicsParams const&,
+                                   JsonValue const&,
+                                   ActionRegistry* reg)
+{
+    std::string label;
+    if (kind == "neutral")
+    {
+        label = "along-step-neutral";
+        reg->insert(std::make_shared<AlongStepNeutralAction>(reg->next_id()));
+        return label;
+    }
+    if (kind == "general_linear")
+        label = "along-step-general-linear";
+    else if (kind == "uniform_field")
+        label = "along-step-uniform-msc";
+    else
+        CELER_VALIDATE(false, << "unknown along-step kind '" << kind << "'");
+    reg->insert(std::make_shared<AlongStepChargedAction>(reg->next_id(), label));
+    return label;
+}
+
+//---------------------------------------------------------------------------//
+}  // namespace celeritas_amd

@@ -1,0 +1,185 @@
+//----------------------------------*-C++-*----------------------------------//
+// celeritas_amd: MI355X-native per-step transport loop
+//---------------------------------------------------------------------------//
+//! \file global/CoreData.hh
+//! Plain-old-data views of the core params and per-slot state.
+//!
+//! Counterparts of CoreParamsData / CoreStateData
+//! (celeritas/global/CoreTrackData.hh:66-160). Every per-slot field is its
+//! own 256-byte-aligned array so that a wave64 access is coalesced; kernels
+//! receive these structs by value (kernel argument segment), so there is no
+//! device-side indirection to reach a field.
+//---------------------------------------------------------------------------//
+#pragma once
+
+#include "corecel/Types.hh"
+#include "geo/GeoData.hh"
+#include "phys/PhysicsData.hh"
+#include "random/XorwowEngine.hh"
+
+namespace celeritas_amd
+{
+//---------------------------------------------------------------------------//
+//! Track initializer (celeritas/track/TrackInitData.hh:66-80)
+struct TrackInitializer
+{
+    std::uint32_t track_id;
+    std::uint32_t parent_id;
+    std::uint32_t event_id;
+    std::uint32_t particle_id;
+    double time;
+    double pos[3];
+    double dir[3];
+    double energy;
+};
+
+//! Secondary particle (celeritas/phys/Secondary.hh)
+struct Secondary
+{
+    std::uint32_t particle_id;  //!< invalid_id: rejected by cutoff
+    std::uint32_t pad_;
+    double energy;
+    double dir[3];
+};
+
+//! Primary particle (celeritas/phys/Primary.hh:23-31)
+struct Primary
+{
+    std::uint32_t particle_id;
+    std::uint32_t event_id;
+    double energy;
+    double pos[3];
+    double dir[3];
+    double time;
+};
+
+//---------------------------------------------------------------------------//
+/*!
+ * Device-resident counters (celeritas/track/CoreStateCounters.hh:24-47).
+ *
+ * The host reads them asynchronously; kernels read and update them so that
+ * no step requires a host round trip.
+ */
+struct CoreCounters
+{
+    std::uint32_t num_initializers;
+    std::uint32_t num_vacancies;
+    std::uint32_t num_active;
+    std::uint32_t num_secondaries;
+    std::uint32_t num_alive;
+    std::uint32_t num_generated;
+    // Launch parameters computed on device for the next kernel
+    std::uint32_t num_new_tracks;  //!< this step's min(vacancies, inits)
+    std::uint32_t init_base;  //!< initializer count before the launch
+    std::uint32_t vacancy_base;  //!< vacancy count before the launch
+    std::uint32_t prev_secondaries;  //!< secondaries at the back of inits
+    std::uint32_t secondary_stack_size;  //!< allocated secondaries
+    std::uint32_t error_flags;  //!< bit 0: initializer overflow, bit 1: stack
+    std::uint32_t steps_taken;  //!< number of step iterations on device
+    std::uint32_t pad_[3];
+};
+
+enum CoreErrorFlags : std::uint32_t
+{
+    err_initializer_overflow = 1u,
+    err_secondary_overflow = 2u,
+    err_event_overflow = 4u,
+};
+
+//---------------------------------------------------------------------------//
+//! Simulation state (celeritas/track/SimData.hh:119-175)
+struct SimStateRef
+{
+    std::uint32_t* track_id{nullptr};
+    std::uint32_t* parent_id{nullptr};
+    std::uint32_t* event_id{nullptr};
+    std::uint32_t* num_steps{nullptr};
+    double* time{nullptr};
+    std::uint8_t* status{nullptr};
+    double* step_length{nullptr};
+    std::uint8_t* post_step_action{nullptr};
+    std::uint8_t* along_step_action{nullptr};
+};
+
+//! Particle state (celeritas/phys/ParticleData.hh:119-135)
+struct ParticleStateRef
+{
+    std::uint8_t* particle_id{nullptr};
+    double* energy{nullptr};
+};
+
+//! Track initialization state (celeritas/track/TrackInitData.hh:100-140)
+struct InitStateRef
+{
+    std::uint32_t* vacancies{nullptr};
+    std::uint32_t* secondary_counts{nullptr};  //!< size+1
+    std::uint32_t* parents{nullptr};
+    std::uint32_t* indices{nullptr};
+    std::uint32_t* track_counters{nullptr};  //!< per event
+    TrackInitializer* initializers{nullptr};
+    std::uint32_t* block_scratch{nullptr};  //!< compaction partials
+    size_type capacity{0};
+    size_type max_events{0};
+};
+
+//! Secondary storage (corecel/data/StackAllocator.hh)
+struct SecondaryStackRef
+{
+    Secondary* storage{nullptr};
+    size_type capacity{0};
+};
+
+//---------------------------------------------------------------------------//
+/*!
+ * All per-slot mutable state.
+ */
+struct CoreStateRef
+{
+    size_type size{0};
+    std::uint32_t stream_id{0};
+    SimStateRef sim;
+    ParticleStateRef particle;
+    GeoStateRef geo;
+    PhysicsStateRef phys;
+    RngStateRef rng;
+    InitStateRef init;
+    SecondaryStackRef secondaries;
+    CoreCounters* counters{nullptr};
+    std::uint32_t* track_slots{nullptr};  //!< thread -> slot indirection
+    std::uint32_t* action_offsets{nullptr};  //!< sorted ranges per action
+    double* calo_edep{nullptr};  //!< per-volume energy deposition tally
+};
+
+//---------------------------------------------------------------------------//
+//! Action IDs of the core actions, known to the kernels
+struct CoreScalars
+{
+    std::uint8_t boundary_action{invalid_id8};
+    std::uint8_t propagation_limit_action{invalid_id8};
+    std::uint8_t tracking_cut_action{invalid_id8};
+    std::uint8_t along_step_user_action{invalid_id8};
+    std::uint8_t along_step_neutral_action{invalid_id8};
+    std::uint8_t initialize_tracks_action{invalid_id8};
+    std::uint8_t pad_[2];
+    std::uint32_t max_steps{invalid_id};
+    std::uint32_t num_actions{0};
+    TrackOrder track_order{TrackOrder::none};
+    PrecisionMode precision{PrecisionMode::fp64};
+};
+
+/*!
+ * Immutable shared data.
+ */
+struct CoreParamsRef
+{
+    CoreScalars scalars;
+    GeoParamsRef geo;
+    std::uint32_t const* volume_to_material{nullptr};
+    std::uint32_t const* volume_to_detector{nullptr};  //!< calo tally map
+    PhysicsParamsRef phys;
+    XorwowJumpTables const* rng_jump{nullptr};
+    std::uint32_t rng_seed{0};
+};
+
+//---------------------------------------------------------------------------//
+}  // namespace celeritas_amd

@@ -1,0 +1,156 @@
+//----------------------------------*-C++-*----------------------------------//
+// celeritas_amd: MI355X-native per-step transport loop
+//---------------------------------------------------------------------------//
+//! \file global/CoreParams.cc
+//---------------------------------------------------------------------------//
+#include "CoreParams.hh"
+
+#include "track/TrackActions.hh"
+#include "CoreActions.hh"
+
+namespace celeritas_amd
+{
+//---------------------------------------------------------------------------//
+CoreParams::CoreParams(Input inp)
+    : geo_(std::move(inp.geometry))
+    , phys_(std::move(inp.physics))
+    , rng_(std::move(inp.rng))
+    , reg_(std::move(inp.action_reg))
+    , init_capacity_(inp.init_capacity)
+    , max_events_(inp.max_events)
+{
+    CELER_VALIDATE(geo_ && phys_ && rng_ && reg_, << "incomplete core params input");
+    CELER_VALIDATE(init_capacity_ > 0 && max_events_ > 0,
+                   << "track initializer capacity and max events must be positive");
+    vol_to_mat_ = inp.volume_to_material;
+    vol_to_mat_.resize(geo_->num_volumes(), invalid_id);
+    d_vol_to_mat_ = to_device(vol_to_mat_);
+
+    auto det = inp.volume_to_detector;
+    det.resize(geo_->num_volumes(), invalid_id);
+    for (auto d : det)
+    {
+        if (d != invalid_id)
+            num_detectors_ = std::max<size_type>(num_detectors_, d + 1);
+    }
+    d_vol_to_det_ = to_device(det);
+
+    std::vector<XorwowJumpTables> jt{rng_->tables()};
+    d_jump_ = to_device(jt);
+
+    // Always-present actions (CoreParams.cc:139-215 order)
+    CoreScalars& sc = ref_.scalars;
+    sc.initialize_tracks_action = static_cast<std::uint8_t>(reg_->next_id());
+    reg_->insert(std::make_shared<InitializeTracksAction>(reg_->next_id()));
+
+    ActionId user_along = inp.along_step_label.empty()
+                              ? invalid_id
+                              : reg_->find_action(inp.along_step_label);
+    ActionId neutral = reg_->find_action("along-step-neutral");
+    if (neutral == invalid_id)
+    {
+        neutral = reg_->next_id();
+        reg_->insert(std::make_shared<AlongStepNeutralAction>(neutral));
+    }
+    sc.along_step_neutral_action = static_cast<std::uint8_t>(neutral);
+    sc.along_step_user_action = static_cast<std::uint8_t>(
+        user_along == invalid_id ? neutral : user_along);
+
+    sc.propagation_limit_action = static_cast<std::uint8_t>(reg_->next_id());
+    reg_->insert(std::make_shared<ImplicitAction>(
+        reg_->next_id(), "geo-propagation-limit", "pause due to propagation misses"));
+    sc.boundary_action = static_cast<std::uint8_t>(reg_->next_id());
+    reg_->insert(std::make_shared<BoundaryAction>(reg_->next_id()));
+    sc.tracking_cut_action = static_cast<std::uint8_t>(reg_->next_id());
+    reg_->insert(std::make_shared<TrackingCutAction>(reg_->next_id()));
+    reg_->insert(std::make_shared<ExtendFromSecondariesAction>(reg_->next_id()));
+    if (reg_->find_action(ExtendFromPrimariesAction::label_str()) == invalid_id)
+    {
+        reg_->insert(std::make_shared<ExtendFromPrimariesAction>(reg_->next_id()));
+    }
+    sc.num_actions = reg_->num_actions();
+    sc.track_order = inp.track_order;
+    sc.precision = inp.precision;
+
+    ref_.geo = geo_->device_ref();
+    ref_.volume_to_material = d_vol_to_mat_.data();
+    ref_.volume_to_detector = d_vol_to_det_.data();
+    ref_.phys = phys_->device_ref();
+    ref_.rng_jump = d_jump_.data();
+    ref_.rng_seed = rng_->seed();
+}
+
+//---------------------------------------------------------------------------//
+std::shared_ptr<CoreParams> CoreParams::from_json(std::string const& text)
+{
+    JsonValue const doc = parse_json(text);
+    Input inp;
+    inp.action_reg = std::make_shared<ActionRegistry>();
+
+    JsonValue const& geo_doc = doc.at("geometry");
+    inp.geometry = std::make_shared<GeoParams>(geo_input_from_orange_json(geo_doc));
+
+    // Physics first: its actions take the lowest IDs as in the reference
+    JsonValue empty;
+    empty.type = JsonValue::Type::object;
+    JsonValue const& phys_doc = doc.has("physics") ? doc.at("physics") : empty;
+    inp.physics = std::make_shared<PhysicsParams>(phys_doc, inp.action_reg.get());
+
+    // Along-step action for charged particles
+    std::string along = doc.get_str("along_step", "");
+    if (!along.empty())
+    {
+        inp.along_step_label = make_along_step_action(along, *inp.physics, doc,
+                                                      inp.action_reg.get());
+    }
+
+    // Volume -> material map: explicit, else from the ORANGE document
+    if (doc.has("volume_to_material"))
+    {
+        for (auto v : doc.at("volume_to_material").as_ints())
+            inp.volume_to_material.push_back(
+                v < 0 ? invalid_id : static_cast<std::uint32_t>(v));
+    }
+    else if (geo_doc.has("materials"))
+    {
+        for (auto v : geo_doc.at("materials").at("cell_to_mat").as_ints())
+            inp.volume_to_material.push_back(
+                v < 0 ? invalid_id : static_cast<std::uint32_t>(v));
+    }
+    if (doc.has("volume_to_detector"))
+    {
+        for (auto v : doc.at("volume_to_detector").as_ints())
+            inp.volume_to_detector.push_back(
+                v < 0 ? invalid_id : static_cast<std::uint32_t>(v));
+    }
+
+    inp.rng = std::make_shared<XorwowParams>(
+        static_cast<std::uint32_t>(doc.get_int("seed", 12345)));
+    if (doc.has("track_init"))
+    {
+        auto const& ti = doc.at("track_init");
+        inp.init_capacity = static_cast<size_type>(ti.get_int("capacity", 4096));
+        inp.max_events = static_cast<size_type>(ti.get_int("max_events", 4096));
+        std::string order = ti.get_str("track_order", "none");
+        if (order == "none")
+            inp.track_order = TrackOrder::none;
+        else if (order == "init_charge")
+            inp.track_order = TrackOrder::init_charge;
+        else if (order == "reindex_status")
+            inp.track_order = TrackOrder::reindex_status;
+        else if (order == "reindex_particle_type")
+            inp.track_order = TrackOrder::reindex_particle_type;
+        else if (order == "reindex_step_limit_action")
+            inp.track_order = TrackOrder::reindex_step_limit_action;
+        else
+            CELER_VALIDATE(false, << "unsupported track order '" << order << "'");
+    }
+    std::string prec = doc.get_str("precision", "fp64");
+    CELER_VALIDATE(prec == "fp64" || prec == "mixed",
+                   << "precision must be 'fp64' or 'mixed'");
+    inp.precision = prec == "fp64" ? PrecisionMode::fp64 : PrecisionMode::mixed;
+    return std::make_shared<CoreParams>(std::move(inp));
+}
+
+//---------------------------------------------------------------------------//
+}  // namespace celeritas_amd

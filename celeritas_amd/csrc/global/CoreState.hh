@@ -1,0 +1,74 @@
+//----------------------------------*-C++-*----------------------------------//
+// celeritas_amd: MI355X-native per-step transport loop
+//---------------------------------------------------------------------------//
+//! \file global/CoreState.hh
+//---------------------------------------------------------------------------//
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <memory>
+#include <vector>
+
+#include "corecel/DeviceRuntime.hh"
+#include "CoreData.hh"
+
+namespace celeritas_amd
+{
+class CoreParams;
+
+//---------------------------------------------------------------------------//
+/*!
+ * Per-stream mutable state: every per-slot field in one HBM arena.
+ *
+ * Mirrors CoreState<device> (celeritas/global/CoreState.cc:28-71):
+ * constructed from params, a stream ID and a number of track slots; slots
+ * start inactive with all of them vacant, RNG states are filled from the
+ * seeded Mersenne twister stream, and the counters live on the device with
+ * a pinned host mirror updated by sync_counters().
+ */
+class CoreState
+{
+  public:
+    CoreState(CoreParams const& params, std::uint32_t stream_id, size_type num_slots);
+    ~CoreState();
+    CoreState(CoreState const&) = delete;
+    CoreState& operator=(CoreState const&) = delete;
+
+    CoreStateRef const& ref() const { return ref_; }
+    size_type size() const { return ref_.size; }
+    std::uint32_t stream_id() const { return ref_.stream_id; }
+    hipStream_t stream() const { return stream_; }
+
+    //! Copy device counters to the host mirror (synchronizes the stream)
+    CoreCounters const& sync_counters();
+    //! Last synchronized counters
+    CoreCounters const& counters() const { return *host_counters_; }
+    //! Enqueue an async copy of the counters to the pinned mirror
+    void enqueue_counter_readback();
+
+    //! Reset to an empty state (all slots vacant, counters cleared)
+    void reset();
+
+    //! Scratch buffer for primaries (grown on demand)
+    Primary* primary_buffer(size_type count);
+
+    bool warming_up() const { return warming_up_; }
+    void warming_up(bool w) { warming_up_ = w; }
+
+    std::size_t memory_bytes() const { return arena_.size(); }
+    size_type num_detectors() const { return num_detectors_; }
+
+  private:
+    CoreStateRef ref_;
+    DeviceArena arena_;
+    hipStream_t stream_{nullptr};
+    CoreCounters* host_counters_{nullptr};
+    DeviceVector<Primary> primaries_;
+    size_type num_detectors_{0};
+    bool warming_up_{false};
+    std::vector<std::uint32_t> initial_rng_;
+};
+
+//---------------------------------------------------------------------------//
+}  // namespace celeritas_amd

@@ -1,0 +1,148 @@
+//----------------------------------*-C++-*----------------------------------//
+// celeritas_amd: MI355X-native per-step transport loop
+//---------------------------------------------------------------------------//
+//! \file global/Stepper.cc
+//---------------------------------------------------------------------------//
+#include "Stepper.hh"
+
+#include <algorithm>
+#include <chrono>
+
+#include "track/TrackActions.hh"
+#include "track/TrackKernels.hh"
+#include "CoreParams.hh"
+#include "CoreState.hh"
+
+namespace celeritas_amd
+{
+hipError_t launch_kill_active(CoreParamsRef const&, CoreStateRef const&, hipStream_t);
+
+//---------------------------------------------------------------------------//
+Stepper::Stepper(StepperInput input)
+    : params_(std::move(input.params)), action_times_(input.action_times)
+{
+    CELER_VALIDATE(params_, << "stepper requires params");
+    CELER_VALIDATE(input.num_track_slots > 0, << "stepper requires track slots");
+    auto const& reg = *params_->action_reg();
+    for (ActionId id = 0; id < reg.num_actions(); ++id)
+    {
+        auto sa = std::dynamic_pointer_cast<StepActionInterface const>(reg.action(id));
+        if (sa)
+            actions_.push_back(std::move(sa));
+    }
+    std::stable_sort(actions_.begin(), actions_.end(), [](auto const& a, auto const& b) {
+        return std::make_pair(a->order(), a->action_id())
+               < std::make_pair(b->order(), b->action_id());
+    });
+    times_.assign(actions_.size(), 0.0);
+
+    ActionId pid = reg.find_action(ExtendFromPrimariesAction::label_str());
+    CELER_VALIDATE(pid != invalid_id,
+                   << "primary generator was not added to the stepping loop");
+    primaries_action_
+        = std::dynamic_pointer_cast<ExtendFromPrimariesAction const>(reg.action(pid));
+
+    state_ = std::make_shared<CoreState>(*params_, input.stream_id, input.num_track_slots);
+    for (auto const& m : reg.mutable_actions())
+    {
+        if (auto br = std::dynamic_pointer_cast<BeginRunActionInterface>(m))
+            br->begin_run(*params_, *state_);
+    }
+}
+
+Stepper::~Stepper() = default;
+
+//---------------------------------------------------------------------------//
+void Stepper::enqueue_step()
+{
+    if (action_times_ && !state_->warming_up())
+    {
+        for (std::size_t i = 0; i < actions_.size(); ++i)
+        {
+            auto start = std::chrono::steady_clock::now();
+            actions_[i]->step(*params_, *state_);
+            CELER_HIP_CALL(hipStreamSynchronize(state_->stream()));
+            times_[i] += std::chrono::duration<double>(
+                             std::chrono::steady_clock::now() - start)
+                             .count();
+        }
+    }
+    else
+    {
+        for (auto const& a : actions_)
+            a->step(*params_, *state_);
+    }
+}
+
+StepperResult Stepper::make_result()
+{
+    CoreCounters const& c = state_->sync_counters();
+    CELER_VALIDATE(!(c.error_flags & err_initializer_overflow),
+                   << "insufficient capacity (" << params_->init_capacity()
+                   << ") for track initializers (created " << c.num_secondaries
+                   << " new secondaries for a total capacity requirement of "
+                   << c.num_initializers << ")");
+    CELER_VALIDATE(!(c.error_flags & err_secondary_overflow),
+                   << "secondary stack capacity exceeded");
+    CELER_VALIDATE(!(c.error_flags & err_event_overflow),
+                   << "event number exceeds max_events=" << params_->max_events());
+    StepperResult r;
+    r.generated = c.num_generated;
+    r.active = c.num_active;
+    r.alive = c.num_alive;
+    r.queued = c.num_initializers;
+    return r;
+}
+
+StepperResult Stepper::operator()()
+{
+    // Generated count is per step (Stepper.cc:125-139)
+    CELER_HIP_CALL(hipMemsetAsync(&state_->ref().counters->num_generated, 0,
+                                  sizeof(std::uint32_t), state_->stream()));
+    this->enqueue_step();
+    return this->make_result();
+}
+
+StepperResult Stepper::operator()(Primary const* primaries, size_type count)
+{
+    CELER_VALIDATE(count > 0, << "no primaries given");
+    std::uint32_t max_event = 0;
+    for (size_type i = 0; i < count; ++i)
+        max_event = std::max(max_event, primaries[i].event_id);
+    CELER_VALIDATE(max_event < params_->max_events(),
+                   << "event number " << max_event
+                   << " exceeds max_events=" << params_->max_events());
+    primaries_action_->insert(*params_, *state_, primaries, count);
+    return (*this)();
+}
+
+//---------------------------------------------------------------------------//
+void Stepper::warm_up()
+{
+    CELER_VALIDATE(state_->counters().num_active == 0,
+                   << "cannot warm up when state has active tracks");
+    state_->warming_up(true);
+    this->enqueue_step();
+    state_->warming_up(false);
+    state_->sync_counters();
+}
+
+void Stepper::kill_active()
+{
+    CELER_HIP_CALL(launch_kill_active(params_->device_ref(), state_->ref(),
+                                      state_->stream()));
+}
+
+void Stepper::reseed(std::uint64_t event_id)
+{
+    CELER_HIP_CALL(launch_reseed_rng(state_->ref().rng,
+                                     params_->device_ref().rng_jump,
+                                     params_->rng().seed(),
+                                     event_id,
+                                     state_->stream()));
+    CELER_HIP_CALL(launch_fill_u32(state_->ref().init.track_counters,
+                                   params_->max_events(), 0, state_->stream()));
+}
+
+//---------------------------------------------------------------------------//
+}  // namespace celeritas_amd

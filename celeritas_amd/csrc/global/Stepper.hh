@@ -1,0 +1,94 @@
+//----------------------------------*-C++-*----------------------------------//
+// celeritas_amd: MI355X-native per-step transport loop
+//---------------------------------------------------------------------------//
+//! \file global/Stepper.hh
+//---------------------------------------------------------------------------//
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "ActionInterface.hh"
+#include "CoreData.hh"
+
+namespace celeritas_amd
+{
+class CoreParams;
+class CoreState;
+class ExtendFromPrimariesAction;
+
+//---------------------------------------------------------------------------//
+//! Stepper construction (celeritas/global/Stepper.hh:43-55)
+struct StepperInput
+{
+    std::shared_ptr<CoreParams const> params;
+    std::uint32_t stream_id{0};
+    size_type num_track_slots{0};
+    bool action_times{false};
+};
+
+//! Counters after one step iteration (celeritas/global/Stepper.hh:61-70)
+struct StepperResult
+{
+    size_type generated{0};
+    size_type queued{0};
+    size_type active{0};
+    size_type alive{0};
+
+    explicit operator bool() const { return queued > 0 || alive > 0; }
+};
+
+//---------------------------------------------------------------------------//
+/*!
+ * Runs step iterations over one state (celeritas/global/Stepper.cc:66-201).
+ *
+ * The sequence is the registered step actions sorted by (order, ID)
+ * (corecel/sys/ActionGroups.t.hh:23-54). Each action enqueues its kernels on
+ * the state's stream; the only host synchronization per step is the counter
+ * readback that produces the StepperResult.
+ */
+class Stepper
+{
+  public:
+    using SpanConstPrimary = std::pair<Primary const*, size_type>;
+
+    explicit Stepper(StepperInput input);
+    ~Stepper();
+
+    //! Run all actions with no active tracks (allocations, first launches)
+    void warm_up();
+    //! Transport existing tracks one step
+    StepperResult operator()();
+    //! Insert primaries then take a step
+    StepperResult operator()(Primary const* primaries, size_type count);
+    //! Kill all active tracks (next step applies the tracking cut)
+    void kill_active();
+    //! Reseed RNGs and track counters at the start of an event
+    void reseed(std::uint64_t event_id);
+
+    //! Enqueue one step without reading counters back
+    void enqueue_step();
+
+    std::vector<std::shared_ptr<StepActionInterface const>> const& actions() const
+    {
+        return actions_;
+    }
+    CoreState& state() { return *state_; }
+    std::shared_ptr<CoreState> const& sp_state() const { return state_; }
+    CoreParams const& params() const { return *params_; }
+    //! Accumulated per-action times [s] when action_times is on
+    std::vector<double> const& action_times() const { return times_; }
+
+  private:
+    std::shared_ptr<CoreParams const> params_;
+    std::shared_ptr<CoreState> state_;
+    std::vector<std::shared_ptr<StepActionInterface const>> actions_;
+    std::shared_ptr<ExtendFromPrimariesAction const> primaries_action_;
+    bool action_times_{false};
+    std::vector<double> times_;
+
+    StepperResult make_result();
+};
+
+//---------------------------------------------------------------------------//
+}  // namespace celeritas_amd

@@ -1,0 +1,147 @@
+//----------------------------------*-C++-*----------------------------------//
+// celeritas_amd: MI355X-native per-step transport loop
+//---------------------------------------------------------------------------//
+//! \file phys/PhysicsData.hh
+//! Physics tables and per-slot physics state.
+//!
+//! Counterparts of PhysicsParamsData/PhysicsStateData
+//! (celeritas/phys/PhysicsData.hh:403-503), MaterialParamsData
+//! (celeritas/mat/MaterialData.hh), ParticleParamsData and CutoffParamsData.
+//! Tables are flattened into a few device arrays; each grid is addressed by
+//! a small descriptor so a lookup is one descriptor load plus two values.
+//---------------------------------------------------------------------------//
+#pragma once
+
+#include "corecel/Types.hh"
+
+namespace celeritas_amd
+{
+//---------------------------------------------------------------------------//
+inline constexpr int max_processes = 4;  //!< per particle type
+inline constexpr int max_models = 3;  //!< per process
+inline constexpr int max_elements = 4;  //!< per material
+
+//! Log-uniform grid with values (celeritas/grid/XsGridData.hh:28-50)
+struct XsGrid
+{
+    double loge_front;  //!< log(E_min / MeV)
+    double loge_back;  //!< log(E_max / MeV)
+    double loge_delta;
+    std::uint32_t size;  //!< number of points (0: no grid)
+    std::uint32_t prime_index;  //!< values at i >= prime are scaled by E
+    std::uint32_t offset;  //!< into the reals array
+    std::uint32_t pad_;
+};
+
+//! One model of a process with its lower energy bound
+struct ModelRange
+{
+    double energy_low;  //!< [MeV]
+    std::uint32_t model;  //!< ModelClass
+    std::uint32_t action;  //!< step action ID of the model
+};
+
+//! One particle-process: tables per material
+struct ProcessRecord
+{
+    std::uint32_t process_class;  //!< ProcessClass
+    std::uint32_t xs_grid;  //!< first XsGrid index (one per material)
+    std::uint32_t integral;  //!< 1 if the integral approach is used
+    std::uint32_t energy_max_xs;  //!< reals offset (one per material)
+    std::uint32_t num_models;
+    std::uint32_t elem_grid;  //!< element CDF grids [model][mat][elem]
+    ModelRange models[max_models];
+};
+
+//! Per-particle physics
+struct ParticlePhysics
+{
+    std::uint32_t num_processes;
+    std::int32_t eloss_process;  //!< index of the process with dE/dx, or -1
+    std::uint32_t dedx_grid;  //!< first XsGrid (one per material)
+    std::uint32_t range_grid;  //!< first XsGrid (one per material)
+    std::uint32_t msc_grid;  //!< first XsGrid (one per material) or invalid
+    std::uint32_t pad_;
+    ProcessRecord process[max_processes];
+};
+
+//---------------------------------------------------------------------------//
+//! Tracking and step-limit scalars (celeritas/phys/PhysicsData.hh:271-330)
+struct PhysicsScalars
+{
+    double min_range;  //!< rho [cm]
+    double max_step_over_range;  //!< alpha
+    double min_eprime_over_e;  //!< xi
+    double linear_loss_limit;
+    double lowest_electron_energy;  //!< [MeV]
+    double fixed_step_limiter;  //!< [cm] (0 = off)
+    double secondary_stack_factor;
+    std::uint32_t num_models;
+    std::uint8_t discrete_action;
+    std::uint8_t range_action;
+    std::uint8_t integral_rejection_action;
+    std::uint8_t failure_action;
+    std::uint8_t msc_action;
+    std::uint8_t fixed_step_action;
+    std::uint8_t pad_[2];
+};
+
+//---------------------------------------------------------------------------//
+/*!
+ * Immutable physics data (device pointers).
+ */
+struct PhysicsParamsRef
+{
+    PhysicsScalars scalars;
+    std::uint32_t num_particles{0};
+    std::uint32_t num_materials{0};
+    ParticlePhysics const* particle{nullptr};
+    XsGrid const* grids{nullptr};
+    double const* reals{nullptr};
+    // Particle properties
+    double const* mass{nullptr};  //!< [MeV/c^2]
+    double const* charge{nullptr};  //!< [e]
+    std::uint32_t gamma_id{invalid_id};
+    std::uint32_t electron_id{invalid_id};
+    std::uint32_t positron_id{invalid_id};
+    // Cutoffs [particle][material]: energy [MeV] and range [cm]
+    double const* cutoff_energy{nullptr};
+    double const* cutoff_range{nullptr};
+    // Material properties
+    void const* materials{nullptr};  //!< MaterialRecord array
+    void const* elements{nullptr};  //!< ElementRecord array
+    void const* models{nullptr};  //!< model-specific data block
+};
+
+//---------------------------------------------------------------------------//
+/*!
+ * Per-slot physics state (SoA).
+ *
+ * Step-local fields (macro_xs, per_process_xs, energy deposition, MSC step)
+ * are only written by the unfused per-action kernels; the fused step kernel
+ * keeps them in registers.
+ */
+struct PhysicsStateRef
+{
+    double* interaction_mfp{nullptr};  //!< remaining MFPs (<=0: interact)
+    double* macro_xs{nullptr};
+    double* per_process_xs{nullptr};  //!< [max_processes][size]
+    double* dedx_range{nullptr};
+    double* energy_deposition{nullptr};
+    std::uint32_t* sec_start{nullptr};  //!< first secondary on the stack
+    std::uint32_t* sec_count{nullptr};
+    std::uint8_t* element{nullptr};  //!< sampled element component
+    // Urban MSC persistent range data (celeritas/em/data/UrbanMscData.hh)
+    double* msc_range_init{nullptr};
+    double* msc_range_factor{nullptr};
+    double* msc_limit_min{nullptr};
+    // MSC step data
+    double* msc_true_path{nullptr};
+    double* msc_geom_path{nullptr};
+    double* msc_alpha{nullptr};
+    std::uint8_t* msc_is_displaced{nullptr};
+    std::uint8_t* msc_enabled{nullptr};
+};
+
+//---------------------------------------------------------------------------//
+}  // namespace celeritas_amd

@@ -1,0 +1,87 @@
+//----------------------------------*-C++-*----------------------------------//
+// celeritas_amd: MI355X-native per-step transport loop
+//---------------------------------------------------------------------------//
+//! \file track/TrackActions.cc
+//---------------------------------------------------------------------------//
+#include "TrackActions.hh"
+
+#include <algorithm>
+
+#include "global/CoreParams.hh"
+#include "global/CoreState.hh"
+#include "TrackKernels.hh"
+
+namespace celeritas_amd
+{
+//---------------------------------------------------------------------------//
+ExtendFromPrimariesAction::ExtendFromPrimariesAction(ActionId id)
+    : ConcreteAction(id, label_str(), "create track initializers from primaries")
+{
+}
+
+void ExtendFromPrimariesAction::insert(CoreParams const& params,
+                                       CoreState& state,
+                                       Primary const* host_primaries,
+                                       size_type count) const
+{
+    Primary* buf = state.primary_buffer(count);
+    CELER_HIP_CALL(hipMemcpyAsync(buf,
+                                  host_primaries,
+                                  count * sizeof(Primary),
+                                  hipMemcpyHostToDevice,
+                                  state.stream()));
+    this->insert_device(params, state, buf, count);
+}
+
+void ExtendFromPrimariesAction::insert_device(CoreParams const& params,
+                                              CoreState& state,
+                                              Primary const* device_primaries,
+                                              size_type count) const
+{
+    for (auto const& p : pending_)
+    {
+        CELER_VALIDATE(p.state != &state,
+                       << "multiple consecutive primary insertions are not "
+                          "implemented");
+    }
+    size_type const num_init = state.counters().num_initializers;
+    CELER_VALIDATE(count + num_init <= params.init_capacity(),
+                   << "insufficient initializer capacity ("
+                   << params.init_capacity() << ") with size (" << num_init
+                   << ") for primaries (" << count << ")");
+    pending_.push_back({&state, device_primaries, count});
+}
+
+void ExtendFromPrimariesAction::step(CoreParams const&, CoreState& state) const
+{
+    Primary const* data = nullptr;
+    size_type count = 0;
+    auto iter = std::find_if(pending_.begin(), pending_.end(), [&](Pending const& p) {
+        return p.state == &state;
+    });
+    if (iter != pending_.end())
+    {
+        data = iter->data;
+        count = iter->count;
+        pending_.erase(iter);
+    }
+    CELER_HIP_CALL(launch_extend_from_primaries(state.ref(), data, count, state.stream()));
+}
+
+//---------------------------------------------------------------------------//
+void InitializeTracksAction::step(CoreParams const& params, CoreState& state) const
+{
+    CELER_HIP_CALL(
+        launch_initialize_tracks(params.device_ref(), state.ref(), state.stream()));
+}
+
+//---------------------------------------------------------------------------//
+void ExtendFromSecondariesAction::step(CoreParams const& params,
+                                       CoreState& state) const
+{
+    CELER_HIP_CALL(launch_extend_from_secondaries(
+        params.device_ref(), state.ref(), state.stream()));
+}
+
+//---------------------------------------------------------------------------//
+}  // namespace celeritas_amd

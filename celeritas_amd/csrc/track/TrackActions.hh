@@ -1,0 +1,84 @@
+//----------------------------------*-C++-*----------------------------------//
+// celeritas_amd: MI355X-native per-step transport loop
+//---------------------------------------------------------------------------//
+//! \file track/TrackActions.hh
+//! Track-management step actions (celeritas/track/*Action.hh).
+//---------------------------------------------------------------------------//
+#pragma once
+
+#include <vector>
+
+#include "global/ActionInterface.hh"
+#include "global/CoreData.hh"
+
+namespace celeritas_amd
+{
+//---------------------------------------------------------------------------//
+/*!
+ * Primaries -> initializers (ExtendFromPrimariesAction.cc:103-200).
+ *
+ * insert() copies host primaries to the state's device buffer; step()
+ * appends them to the initializer queue. Inserting twice before a step is
+ * rejected like the reference (CELER_NOT_IMPLEMENTED there).
+ */
+class ExtendFromPrimariesAction final : public StepActionInterface,
+                                        public ConcreteAction
+{
+  public:
+    explicit ExtendFromPrimariesAction(ActionId id);
+    static char const* label_str() { return "extend-from-primaries"; }
+
+    void insert(CoreParams const& params,
+                CoreState& state,
+                Primary const* host_primaries,
+                size_type count) const;
+    //! Insert primaries already resident in device memory
+    void insert_device(CoreParams const& params,
+                       CoreState& state,
+                       Primary const* device_primaries,
+                       size_type count) const;
+    void step(CoreParams const&, CoreState&) const final;
+    StepActionOrder order() const final { return StepActionOrder::generate; }
+
+  private:
+    struct Pending
+    {
+        CoreState const* state;
+        Primary const* data;
+        size_type count;
+    };
+    mutable std::vector<Pending> pending_;
+};
+
+//---------------------------------------------------------------------------//
+//! Fill vacancies from initializers (InitializeTracksAction.cc:56-97)
+class InitializeTracksAction final : public StepActionInterface,
+                                     public ConcreteAction
+{
+  public:
+    explicit InitializeTracksAction(ActionId id)
+        : ConcreteAction(id, "initialize-tracks", "initialize track states")
+    {
+    }
+    void step(CoreParams const&, CoreState&) const final;
+    StepActionOrder order() const final { return StepActionOrder::start; }
+};
+
+//---------------------------------------------------------------------------//
+//! Compaction and secondaries (ExtendFromSecondariesAction.cc:56-100)
+class ExtendFromSecondariesAction final : public StepActionInterface,
+                                          public ConcreteAction
+{
+  public:
+    explicit ExtendFromSecondariesAction(ActionId id)
+        : ConcreteAction(id,
+                         "extend-from-secondaries",
+                         "create track initializers from secondaries")
+    {
+    }
+    void step(CoreParams const&, CoreState&) const final;
+    StepActionOrder order() const final { return StepActionOrder::end; }
+};
+
+//---------------------------------------------------------------------------//
+}  // namespace celeritas_amd

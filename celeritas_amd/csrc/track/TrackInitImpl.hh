@@ -1,0 +1,85 @@
+//----------------------------------*-C++-*----------------------------------//
+// celeritas_amd: MI355X-native per-step transport loop
+//---------------------------------------------------------------------------//
+//! \file track/TrackInitImpl.hh
+//! Per-track initialization shared by the init and secondary kernels.
+//---------------------------------------------------------------------------//
+#pragma once
+
+#include "geo/OrangeLiteTracker.hh"
+#include "global/CoreData.hh"
+
+namespace celeritas_amd
+{
+//---------------------------------------------------------------------------//
+//! Write the sim part of an initializer (celeritas/track/SimTrackView.hh:148)
+CELER_FORCEINLINE_FUNCTION void
+init_sim(SimStateRef const& sim, size_type slot, TrackInitializer const& ti)
+{
+    sim.track_id[slot] = ti.track_id;
+    sim.parent_id[slot] = ti.parent_id;
+    sim.event_id[slot] = ti.event_id;
+    sim.num_steps[slot] = 0;
+    sim.time[slot] = ti.time;
+    sim.status[slot] = static_cast<std::uint8_t>(TrackStatus::initializing);
+    sim.step_length[slot] = 0;
+    sim.post_step_action[slot] = invalid_id8;
+    sim.along_step_action[slot] = invalid_id8;
+}
+
+//! Reset persistent physics state (PhysicsTrackView::operator=, :207-212)
+CELER_FORCEINLINE_FUNCTION void
+reset_physics(PhysicsStateRef const& phys, size_type slot)
+{
+    phys.interaction_mfp[slot] = 0;
+    phys.msc_range_init[slot] = 0;
+    phys.msc_range_factor[slot] = 0;
+    phys.msc_limit_min[slot] = 0;
+}
+
+//! Mark a track as failed (celeritas/global/CoreTrackView.hh:340-347)
+CELER_FORCEINLINE_FUNCTION void apply_errored(CoreParamsRef const& params,
+                                              CoreStateRef const& state,
+                                              size_type slot)
+{
+    state.sim.status[slot] = static_cast<std::uint8_t>(TrackStatus::errored);
+    state.sim.along_step_action[slot] = invalid_id8;
+    state.sim.post_step_action[slot] = params.scalars.tracking_cut_action;
+}
+
+/*!
+ * Initialize a track in a vacant slot from an initializer.
+ *
+ * Follows InitTracksExecutor (celeritas/track/detail/InitTracksExecutor.hh:
+ * 71-175) for the case where no parent geometry is available (the reference
+ * clears the parent slots at every generate step, ExtendFromPrimariesAction
+ * .cc:184-200, so geometry is always rebuilt from the position).
+ */
+CELER_FUNCTION void init_track(CoreParamsRef const& params,
+                               CoreStateRef const& state,
+                               size_type slot,
+                               TrackInitializer const& ti)
+{
+    init_sim(state.sim, slot, ti);
+    state.particle.particle_id[slot] = static_cast<std::uint8_t>(ti.particle_id);
+    state.particle.energy[slot] = ti.energy;
+
+    geo::GeoTrack g;
+    geo::initialize(params.geo, g, ti.pos, ti.dir);
+    g.store(state.geo, slot);
+    if (CELER_UNLIKELY(g.failed || geo::is_outside(params.geo, g)))
+    {
+        apply_errored(params, state, slot);
+        return;
+    }
+    std::uint32_t const mat = params.volume_to_material[g.volume];
+    if (CELER_UNLIKELY(mat == invalid_id))
+    {
+        apply_errored(params, state, slot);
+        return;
+    }
+    reset_physics(state.phys, slot);
+}
+
+//---------------------------------------------------------------------------//
+}  // namespace celeritas_amd

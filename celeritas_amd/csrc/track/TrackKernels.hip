@@ -1,0 +1,410 @@
+//----------------------------------*-C++-*----------------------------------//
+// celeritas_amd: MI355X-native per-step transport loop
+//---------------------------------------------------------------------------//
+//! \file track/TrackKernels.hip
+//! Primaries -> initializers, track initialization and the fused
+//! locate-alive / vacancy compaction / secondary scan / initializer creation.
+//!
+//! All counters live in device memory (CoreCounters). Launch extents that
+//! depend on counters are fixed upper bounds; a one-lane "prepare" kernel
+//! computes the real extent and the base offsets before the worker kernel,
+//! so a whole step can be enqueued (or captured in a graph) with no host
+//! round trip. The reference instead copies counts back to the host after
+//! thrust::remove_if and exclusive_scan (celeritas/track/detail/
+//! TrackInitAlgorithms.cu:34-105).
+//---------------------------------------------------------------------------//
+#include "TrackKernels.hh"
+
+#include "corecel/WaveOps.hh"
+#include "TrackInitImpl.hh"
+
+namespace celeritas_amd
+{
+namespace
+{
+//---------------------------------------------------------------------------//
+constexpr int block_size = 256;
+constexpr int num_waves = block_size / wave::size;
+
+inline unsigned int num_blocks(size_type n)
+{
+    return (n + block_size - 1) / block_size;
+}
+
+//---------------------------------------------------------------------------//
+// PRIMARIES
+//---------------------------------------------------------------------------//
+__global__ void primaries_prepare_kernel(CoreCounters* c, size_type count)
+{
+    c->init_base = c->num_initializers;
+    c->num_initializers += count;
+    c->num_generated += count;
+    // Parent slots are cleared at every generate step in the reference, so
+    // no initializer can copy its geometry from a parent slot.
+    c->prev_secondaries = 0;
+}
+
+/*!
+ * Create initializers from primaries (ProcessPrimariesExecutor.hh:56-76).
+ *
+ * Track IDs come from an atomic per-event counter, as in the reference; the
+ * hardware returns lane-ordered results within a wave.
+ */
+__global__ void __launch_bounds__(block_size)
+    process_primaries_kernel(CoreStateRef const state,
+                             Primary const* __restrict__ primaries,
+                             size_type count)
+{
+    size_type const tid = blockIdx.x * block_size + threadIdx.x;
+    if (tid >= count)
+        return;
+    Primary const p = primaries[tid];
+    size_type const idx = state.counters->init_base + tid;
+    TrackInitializer ti;
+    if (p.event_id < state.init.max_events)
+    {
+        ti.track_id = atomicAdd(&state.init.track_counters[p.event_id], 1u);
+    }
+    else
+    {
+        ti.track_id = invalid_id;
+        atomicOr(&state.counters->error_flags, err_event_overflow);
+    }
+    ti.parent_id = invalid_id;
+    ti.event_id = p.event_id;
+    ti.particle_id = p.particle_id;
+    ti.time = p.time;
+    for (int k = 0; k < 3; ++k)
+    {
+        ti.pos[k] = p.pos[k];
+        ti.dir[k] = p.dir[k];
+    }
+    ti.energy = p.energy;
+    if (idx < state.init.capacity)
+        state.init.initializers[idx] = ti;
+}
+
+//---------------------------------------------------------------------------//
+// TRACK INITIALIZATION
+//---------------------------------------------------------------------------//
+__global__ void init_prepare_kernel(CoreCounters* c, size_type num_slots)
+{
+    size_type const n = min(c->num_vacancies, c->num_initializers);
+    c->num_new_tracks = n;
+    c->init_base = c->num_initializers;
+    c->vacancy_base = c->num_vacancies;
+    c->num_initializers -= n;
+    c->num_vacancies -= n;
+    c->num_active = num_slots - c->num_vacancies;
+}
+
+/*!
+ * Pop initializers from the back into vacancies from the back
+ * (InitTracksExecutor.hh:71-175 with TrackOrder::none).
+ */
+__global__ void __launch_bounds__(block_size)
+    init_tracks_kernel(CoreParamsRef const params, CoreStateRef const state)
+{
+    size_type const tid = blockIdx.x * block_size + threadIdx.x;
+    CoreCounters const& c = *state.counters;
+    if (tid >= c.num_new_tracks)
+        return;
+    TrackInitializer const ti = state.init.initializers[c.init_base - tid - 1];
+    size_type const slot = state.init.vacancies[c.vacancy_base - tid - 1];
+    init_track(params, state, slot, ti);
+}
+
+//---------------------------------------------------------------------------//
+// EXTEND FROM SECONDARIES
+//---------------------------------------------------------------------------//
+//! Number of live secondaries of a slot, counting empty (cut) entries out
+__device__ __forceinline__ size_type count_secondaries(CoreStateRef const& s,
+                                                       size_type slot)
+{
+    size_type const start = s.phys.sec_start[slot];
+    size_type const n = s.phys.sec_count[slot];
+    size_type result = 0;
+    for (size_type i = 0; i < n; ++i)
+    {
+        if (s.secondaries.storage[start + i].particle_id != invalid_id)
+            ++result;
+    }
+    return result;
+}
+
+constexpr std::uint32_t vacancy_bit = 0x80000000u;
+
+/*!
+ * Locate alive tracks and count surviving secondaries
+ * (LocateAliveExecutor.hh:60-106), then reduce per workgroup.
+ *
+ * secondary_counts[slot] holds the count with the vacancy flag in bit 31.
+ */
+__global__ void __launch_bounds__(block_size)
+    locate_alive_kernel(CoreStateRef const state, std::uint8_t in_place)
+{
+    __shared__ unsigned int lds[2 * num_waves];
+    size_type const slot = blockIdx.x * block_size + threadIdx.x;
+    unsigned int vac = 0, nsec = 0;
+    if (slot < state.size)
+    {
+        auto const status = static_cast<TrackStatus>(state.sim.status[slot]);
+        if (status != TrackStatus::inactive)
+            nsec = count_secondaries(state, slot);
+        if (status == TrackStatus::alive)
+        {
+            vac = 0;
+        }
+        else if (nsec > 0 && in_place)
+        {
+            --nsec;
+            vac = 0;
+        }
+        else
+        {
+            vac = 1;
+        }
+        state.init.secondary_counts[slot] = nsec | (vac ? vacancy_bit : 0u);
+    }
+    wave::BlockScan2<num_waves> scan(vac, nsec, lds);
+    if (threadIdx.x == 0)
+    {
+        state.init.block_scratch[2 * blockIdx.x] = scan.total_a;
+        state.init.block_scratch[2 * blockIdx.x + 1] = scan.total_b;
+    }
+}
+
+/*!
+ * Exclusive scan over workgroup partials (one workgroup) and update of the
+ * counters (ExtendFromSecondariesAction.cc:56-100).
+ */
+__global__ void __launch_bounds__(1024)
+    scan_partials_kernel(CoreStateRef const state, unsigned int nblocks)
+{
+    constexpr int nw = 1024 / wave::size;
+    __shared__ unsigned int lds[2 * nw];
+    __shared__ unsigned int carry[2];
+    if (threadIdx.x == 0)
+    {
+        carry[0] = carry[1] = 0;
+    }
+    __syncthreads();
+    unsigned int* part = state.init.block_scratch;
+    for (unsigned int base = 0; base < nblocks; base += 1024)
+    {
+        unsigned int const i = base + threadIdx.x;
+        unsigned int a = 0, b = 0;
+        if (i < nblocks)
+        {
+            a = part[2 * i];
+            b = part[2 * i + 1];
+        }
+        wave::BlockScan2<nw> scan(a, b, lds);
+        unsigned int const ca = carry[0];
+        unsigned int const cb = carry[1];
+        if (i < nblocks)
+        {
+            part[2 * i] = ca + scan.excl_a;
+            part[2 * i + 1] = cb + scan.excl_b;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+        {
+            carry[0] = ca + scan.total_a;
+            carry[1] = cb + scan.total_b;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        CoreCounters* c = state.counters;
+        unsigned int const nvac = carry[0];
+        unsigned int const nsec = carry[1];
+        c->num_vacancies = nvac;
+        c->num_secondaries = nsec;
+        c->init_base = c->num_initializers;
+        c->num_initializers += nsec;
+        if (c->num_initializers > state.init.capacity)
+            c->error_flags |= err_initializer_overflow;
+        c->num_alive = state.size - nvac;
+        c->prev_secondaries = nsec;
+    }
+}
+
+/*!
+ * Compact vacancies (stable, like std::remove_if) and create initializers
+ * from secondaries in slot order (ProcessSecondariesExecutor.hh:70-185).
+ */
+__global__ void __launch_bounds__(block_size)
+    process_secondaries_kernel(CoreStateRef const state, std::uint8_t in_place)
+{
+    __shared__ unsigned int lds[2 * num_waves];
+    size_type const slot = blockIdx.x * block_size + threadIdx.x;
+    unsigned int vac = 0, nsec = 0;
+    if (slot < state.size)
+    {
+        unsigned int const packed = state.init.secondary_counts[slot];
+        vac = (packed & vacancy_bit) ? 1 : 0;
+        nsec = packed & ~vacancy_bit;
+    }
+    wave::BlockScan2<num_waves> scan(vac, nsec, lds);
+    if (slot >= state.size)
+        return;
+
+    unsigned int const vac_idx = state.init.block_scratch[2 * blockIdx.x]
+                                 + scan.excl_a;
+    unsigned int const sec_prefix
+        = state.init.block_scratch[2 * blockIdx.x + 1] + scan.excl_b;
+    if (vac)
+        state.init.vacancies[vac_idx] = slot;
+    // Expose the scanned offsets like the reference's exclusive_scan output
+    state.init.secondary_counts[slot] = sec_prefix;
+
+    auto status = static_cast<TrackStatus>(state.sim.status[slot]);
+    if (status == TrackStatus::inactive)
+        return;
+
+    CoreCounters const& c = *state.counters;
+    unsigned int next_idx = c.init_base + sec_prefix;
+    bool initialized = false;
+    size_type const start = state.phys.sec_start[slot];
+    size_type const count = state.phys.sec_count[slot];
+    if (count > 0)
+    {
+        std::uint32_t const parent_id = state.sim.track_id[slot];
+        std::uint32_t const event_id = state.sim.event_id[slot];
+        double const time = state.sim.time[slot];
+        double pos[3];
+        for (int k = 0; k < 3; ++k)
+            pos[k] = state.geo.pos[k][slot];
+        for (size_type i = 0; i < count; ++i)
+        {
+            Secondary const sec = state.secondaries.storage[start + i];
+            if (sec.particle_id == invalid_id)
+                continue;
+            TrackInitializer ti;
+            ti.track_id = atomicAdd(&state.init.track_counters[event_id], 1u);
+            ti.parent_id = parent_id;
+            ti.event_id = event_id;
+            ti.particle_id = sec.particle_id;
+            ti.time = time;
+            ti.energy = sec.energy;
+            for (int k = 0; k < 3; ++k)
+            {
+                ti.pos[k] = pos[k];
+                ti.dir[k] = sec.dir[k];
+            }
+            if (!initialized && status != TrackStatus::alive && in_place)
+            {
+                // Refill the killed parent's slot in place: keep geometry,
+                // take the secondary's direction, reset physics
+                init_sim(state.sim, slot, ti);
+                state.particle.particle_id[slot]
+                    = static_cast<std::uint8_t>(ti.particle_id);
+                state.particle.energy[slot] = ti.energy;
+                for (int k = 0; k < 3; ++k)
+                    state.geo.dir[k][slot] = ti.dir[k];
+                state.geo.failed[slot] = 0;
+                state.geo.next_step[slot] = 0;
+                state.geo.next_surface[slot] = invalid_id;
+                reset_physics(state.phys, slot);
+                initialized = true;
+            }
+            else
+            {
+                if (next_idx < state.init.capacity)
+                    state.init.initializers[next_idx] = ti;
+                ++next_idx;
+            }
+        }
+    }
+    if (!initialized && status == TrackStatus::killed)
+    {
+        state.sim.status[slot] = static_cast<std::uint8_t>(TrackStatus::inactive);
+    }
+}
+
+//---------------------------------------------------------------------------//
+// UTILITIES
+//---------------------------------------------------------------------------//
+__global__ void fill_u32_kernel(std::uint32_t* data, size_type n, std::uint32_t v)
+{
+    size_type const i = blockIdx.x * block_size + threadIdx.x;
+    if (i < n)
+        data[i] = v;
+}
+
+__global__ void iota_u32_kernel(std::uint32_t* data, size_type n)
+{
+    size_type const i = blockIdx.x * block_size + threadIdx.x;
+    if (i < n)
+        data[i] = i;
+}
+
+//---------------------------------------------------------------------------//
+}  // namespace
+
+//---------------------------------------------------------------------------//
+// LAUNCHERS
+//---------------------------------------------------------------------------//
+hipError_t launch_extend_from_primaries(CoreStateRef const& state,
+                                        Primary const* primaries,
+                                        size_type count,
+                                        hipStream_t stream)
+{
+    primaries_prepare_kernel<<<1, 1, 0, stream>>>(state.counters, count);
+    if (count > 0)
+    {
+        process_primaries_kernel<<<num_blocks(count), block_size, 0, stream>>>(
+            state, primaries, count);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_initialize_tracks(CoreParamsRef const& params,
+                                    CoreStateRef const& state,
+                                    hipStream_t stream)
+{
+    init_prepare_kernel<<<1, 1, 0, stream>>>(state.counters, state.size);
+    init_tracks_kernel<<<num_blocks(state.size), block_size, 0, stream>>>(
+        params, state);
+    return hipGetLastError();
+}
+
+hipError_t launch_extend_from_secondaries(CoreParamsRef const& params,
+                                          CoreStateRef const& state,
+                                          hipStream_t stream)
+{
+    unsigned int const nb = num_blocks(state.size);
+    std::uint8_t const in_place = params.scalars.track_order
+                                  != TrackOrder::init_charge;
+    locate_alive_kernel<<<nb, block_size, 0, stream>>>(state, in_place);
+    scan_partials_kernel<<<1, 1024, 0, stream>>>(state, nb);
+    process_secondaries_kernel<<<nb, block_size, 0, stream>>>(state, in_place);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(std::uint32_t* data,
+                           size_type n,
+                           std::uint32_t value,
+                           hipStream_t stream)
+{
+    if (n > 0)
+        fill_u32_kernel<<<num_blocks(n), block_size, 0, stream>>>(data, n, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_iota_u32(std::uint32_t* data, size_type n, hipStream_t stream)
+{
+    if (n > 0)
+        iota_u32_kernel<<<num_blocks(n), block_size, 0, stream>>>(data, n);
+    return hipGetLastError();
+}
+
+size_type extend_block_scratch_size(size_type num_slots)
+{
+    return 2 * num_blocks(num_slots);
+}
+
+//---------------------------------------------------------------------------//
+}  // namespace celeritas_amd

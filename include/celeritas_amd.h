@@ -1,0 +1,270 @@
+/*----------------------------------*-C-*------------------------------------*
+ * celeritas_amd: MI355X-native per-step transport loop — C ABI
+ *
+ * Plain C entry points (no C++ or torch types) for binding from any
+ * language (ctypes, cgo, JNI, N-API). Every function returns 0 on success
+ * and a nonzero code on failure; constructors return NULL on failure.
+ * celer_amd_last_error() returns the message of the last failure on the
+ * calling thread. Handles are opaque.
+ *
+ * Reference interfaces each group replaces (paths relative to the
+ * celeritas source tree):
+ *   params    celeritas/global/CoreParams.hh:68-172 (CoreParams(Input))
+ *   state     celeritas/global/CoreState.hh:74-182 (CoreState<device>)
+ *   actions   corecel/sys/ActionInterface.hh:176-186 (step(params, state))
+ *             corecel/sys/ActionRegistry.hh:42-105 (find_action, labels)
+ *   stepper   celeritas/global/Stepper.hh:43-201 (Stepper<device>)
+ *   offload   accel/LocalTransporter.hh:52-109 (Push/Flush/InitializeEvent)
+ *   launch    celeritas/track/*Action.cu, celeritas/random/RngReseed.cu:59
+ *---------------------------------------------------------------------------*/
+#ifndef CELERITAS_AMD_H
+#define CELERITAS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/*---------------------------------------------------------------------------*
+ * PLAIN DATA (layouts match the device structs)
+ *---------------------------------------------------------------------------*/
+/* Primary particle: celeritas/phys/Primary.hh:23-31 */
+typedef struct CelerPrimary
+{
+    uint32_t particle_id;
+    uint32_t event_id;
+    double energy; /* MeV */
+    double pos[3]; /* cm */
+    double dir[3];
+    double time; /* s */
+} CelerPrimary;
+
+/* Step counters: celeritas/global/Stepper.hh:61-70 (StepperResult) */
+typedef struct CelerStepperResult
+{
+    uint32_t generated;
+    uint32_t queued;
+    uint32_t active;
+    uint32_t alive;
+} CelerStepperResult;
+
+/* Device counters: celeritas/track/CoreStateCounters.hh:24-47 */
+typedef struct CelerCounters
+{
+    uint32_t num_initializers;
+    uint32_t num_vacancies;
+    uint32_t num_active;
+    uint32_t num_secondaries;
+    uint32_t num_alive;
+    uint32_t num_generated;
+    uint32_t num_new_tracks;
+    uint32_t init_base;
+    uint32_t vacancy_base;
+    uint32_t prev_secondaries;
+    uint32_t secondary_stack_size;
+    uint32_t error_flags;
+    uint32_t steps_taken;
+    uint32_t pad_[3];
+} CelerCounters;
+
+/* Track initializer: celeritas/track/TrackInitData.hh:66-80 */
+typedef struct CelerTrackInitializer
+{
+    uint32_t track_id;
+    uint32_t parent_id;
+    uint32_t event_id;
+    uint32_t particle_id;
+    double time;
+    double pos[3];
+    double dir[3];
+    double energy;
+} CelerTrackInitializer;
+
+/* Secondary: celeritas/phys/Secondary.hh (particle_id UINT32_MAX = cut) */
+typedef struct CelerSecondary
+{
+    uint32_t particle_id;
+    uint32_t pad_;
+    double energy;
+    double dir[3];
+} CelerSecondary;
+
+/* Per-slot state fields accessible with celer_state_copy_field */
+typedef enum CelerStateField
+{
+    CELER_FIELD_TRACK_ID = 0,        /* u32 */
+    CELER_FIELD_PARENT_ID,           /* u32 */
+    CELER_FIELD_EVENT_ID,            /* u32 */
+    CELER_FIELD_NUM_STEPS,           /* u32 */
+    CELER_FIELD_TIME,                /* f64 */
+    CELER_FIELD_STATUS,              /* u8  (TrackStatus) */
+    CELER_FIELD_STEP_LENGTH,         /* f64 */
+    CELER_FIELD_POST_STEP_ACTION,    /* u8  */
+    CELER_FIELD_ALONG_STEP_ACTION,   /* u8  */
+    CELER_FIELD_PARTICLE_ID,         /* u8  */
+    CELER_FIELD_ENERGY,              /* f64 */
+    CELER_FIELD_POS_X,               /* f64 */
+    CELER_FIELD_POS_Y,
+    CELER_FIELD_POS_Z,
+    CELER_FIELD_DIR_X,               /* f64 */
+    CELER_FIELD_DIR_Y,
+    CELER_FIELD_DIR_Z,
+    CELER_FIELD_VOLUME,              /* u32 */
+    CELER_FIELD_SURFACE,             /* u32 */
+    CELER_FIELD_INTERACTION_MFP,     /* f64 */
+    CELER_FIELD_SEC_START,           /* u32 */
+    CELER_FIELD_SEC_COUNT,           /* u32 */
+    CELER_FIELD_ENERGY_DEPOSITION,   /* f64 */
+    CELER_FIELD_RNG_XS0,             /* u32 x 6 words, SoA order */
+    CELER_FIELD_RNG_XS1,
+    CELER_FIELD_RNG_XS2,
+    CELER_FIELD_RNG_XS3,
+    CELER_FIELD_RNG_XS4,
+    CELER_FIELD_RNG_WEYL,
+    CELER_FIELD_VACANCIES,           /* u32 [num_slots] */
+    CELER_FIELD_SECONDARY_COUNTS,    /* u32 [num_slots + 1] */
+    CELER_FIELD_TRACK_COUNTERS,      /* u32 [max_events] */
+    CELER_FIELD_INITIALIZERS,        /* CelerTrackInitializer [capacity] */
+    CELER_FIELD_SECONDARIES,         /* CelerSecondary [stack capacity] */
+    CELER_FIELD_CALO_EDEP,           /* f64 [num_detectors] */
+    CELER_FIELD_MACRO_XS,            /* f64 */
+    CELER_FIELD_DEDX_RANGE,          /* f64 */
+    CELER_FIELD_TRACK_SLOTS,         /* u32 */
+    CELER_FIELD_ACTION_OFFSETS,      /* u32 [num_actions + 2] */
+    CELER_FIELD_NEXT_STEP,           /* f64 */
+    CELER_FIELD_GEO_FAILED,          /* u8 */
+    CELER_FIELD_SIZE_
+} CelerStateField;
+
+typedef struct CelerParams CelerParams;
+typedef struct CelerState CelerState;
+typedef struct CelerStepper CelerStepper;
+typedef struct CelerTransporter CelerTransporter;
+
+/*---------------------------------------------------------------------------*
+ * LIBRARY
+ *---------------------------------------------------------------------------*/
+const char* celer_amd_version(void);
+/* 1 if a HIP device is usable, else 0 (initializes nothing on the GPU) */
+int celer_amd_device_available(void);
+/* Message of the last failure on this thread ("" if none) */
+const char* celer_amd_last_error(void);
+
+/*---------------------------------------------------------------------------*
+ * PARAMS: CoreParams from a JSON problem description
+ * (geometry in ORANGE .org.json form, physics, rng seed, track init)
+ *---------------------------------------------------------------------------*/
+CelerParams* celer_params_create(const char* json);
+void celer_params_destroy(CelerParams* params);
+uint32_t celer_params_num_actions(const CelerParams* params);
+/* Copy an action label into buf (NUL-terminated, truncated to len) */
+int celer_params_action_label(const CelerParams* params,
+                              uint32_t action_id,
+                              char* buf,
+                              size_t len);
+/* UINT32_MAX if no action has this label */
+uint32_t celer_params_find_action(const CelerParams* params, const char* label);
+/* Step-action order (StepActionOrder) or -1 for implicit actions */
+int celer_params_action_order(const CelerParams* params, uint32_t action_id);
+uint32_t celer_params_find_particle(const CelerParams* params, int pdg);
+uint32_t celer_params_num_volumes(const CelerParams* params);
+/* Serialize host-side tables (physics, geometry) as JSON for checkers */
+int celer_params_export_json(const CelerParams* params, char* buf, size_t len,
+                             size_t* needed);
+
+/*---------------------------------------------------------------------------*
+ * STATE: one CoreState<device> with its own HIP stream
+ *---------------------------------------------------------------------------*/
+CelerState* celer_state_create(CelerParams* params,
+                               uint32_t stream_id,
+                               uint32_t num_slots);
+void celer_state_destroy(CelerState* state);
+uint32_t celer_state_size(const CelerState* state);
+/* Buffer primaries for the next extend-from-primaries action */
+int celer_state_insert_primaries(CelerState* state,
+                                 const CelerPrimary* primaries,
+                                 uint32_t count);
+/* Run one step action by ID (like StepActionInterface::step) */
+int celer_state_execute_action(CelerState* state, uint32_t action_id);
+/* Synchronize and copy the device counters */
+int celer_state_counters(CelerState* state, CelerCounters* out);
+/* Copy a field: direction 0 = device->host, 1 = host->device */
+int celer_state_copy_field(CelerState* state,
+                           CelerStateField field,
+                           int direction,
+                           void* host,
+                           size_t bytes);
+/* Overwrite the device counters (test harness: mock interactions) */
+int celer_state_set_counters(CelerState* state, const CelerCounters* in);
+/* Per-event reseed: SplitMix64(seed) + subsequence event*N + slot */
+int celer_state_reseed(CelerState* state, uint64_t event_id);
+/* Seed each slot with its own subsequence and offset */
+int celer_state_rng_initialize(CelerState* state,
+                               const uint64_t* subsequence,
+                               const uint64_t* offset);
+/* Draw per slot: kind 0 = u32, 1 = f64 canonical, 2 = f32 canonical */
+int celer_state_rng_draw(CelerState* state,
+                         uint32_t draws_per_slot,
+                         int kind,
+                         void* host_out);
+int celer_state_reset(CelerState* state);
+int celer_state_synchronize(CelerState* state);
+
+/*---------------------------------------------------------------------------*
+ * STEPPER: Stepper<device>
+ *---------------------------------------------------------------------------*/
+CelerStepper* celer_stepper_create(CelerParams* params,
+                                   uint32_t stream_id,
+                                   uint32_t num_slots,
+                                   int action_times);
+void celer_stepper_destroy(CelerStepper* stepper);
+/* Insert primaries (may be NULL/0) then take one step */
+int celer_stepper_step(CelerStepper* stepper,
+                       const CelerPrimary* primaries,
+                       uint32_t count,
+                       CelerStepperResult* result);
+int celer_stepper_warm_up(CelerStepper* stepper);
+int celer_stepper_kill_active(CelerStepper* stepper);
+int celer_stepper_reseed(CelerStepper* stepper, uint64_t event_id);
+/* The stepper's state (owned by the stepper) */
+CelerState* celer_stepper_state(CelerStepper* stepper);
+/* Labels of step actions in execution order, newline-separated */
+int celer_stepper_action_sequence(const CelerStepper* stepper,
+                                  char* buf,
+                                  size_t len);
+
+/*---------------------------------------------------------------------------*
+ * LOW-LEVEL LAUNCH ABI: enqueue kernels on a caller stream with the POD
+ * device views obtained from celer_*_device_ref. hipStream_t is passed as
+ * void*. Returns a hipError_t value.
+ *---------------------------------------------------------------------------*/
+const void* celer_params_device_ref(const CelerParams* params);
+const void* celer_state_device_ref(const CelerState* state);
+void* celer_state_stream(const CelerState* state);
+
+/* track/ExtendFromPrimariesAction.cu:23-38 */
+int celer_hip_extend_from_primaries(const void* state_ref,
+                                    const CelerPrimary* device_primaries,
+                                    uint32_t count,
+                                    void* stream);
+/* track/InitializeTracksAction.cu:23-36 */
+int celer_hip_initialize_tracks(const void* params_ref,
+                                const void* state_ref,
+                                void* stream);
+/* track/ExtendFromSecondariesAction.cu:45-69 + TrackInitAlgorithms.cu */
+int celer_hip_extend_from_secondaries(const void* params_ref,
+                                      const void* state_ref,
+                                      void* stream);
+/* random/RngReseed.cu:59-72 */
+int celer_hip_reseed(const void* params_ref,
+                     const void* state_ref,
+                     uint64_t event_id,
+                     void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CELERITAS_AMD_H */

@@ -1,0 +1,406 @@
+//----------------------------------*-C++-*----------------------------------//
+// ORACLE — CPU restatement used only as a checker by tests/, smoke() and the
+// cpu_baseline leg of bench.py. Never linked into the product library.
+//---------------------------------------------------------------------------//
+//! \file oracle/capi.cc
+//! C entry points of liboracle.so for the Python test harness.
+//---------------------------------------------------------------------------//
+#include <cstring>
+#include <string>
+
+#include "core.hh"
+#include "physics.hh"
+#include "problem.hh"
+#include "rng.hh"
+#include "loop.hh"
+
+using namespace orc;
+
+namespace
+{
+thread_local std::string g_err;
+
+template<class F>
+int guard(F&& f)
+{
+    try
+    {
+        g_err.clear();
+        f();
+        return 0;
+    }
+    catch (std::exception const& e)
+    {
+        g_err = e.what();
+    }
+    return 1;
+}
+
+struct OrcProblem
+{
+    std::shared_ptr<Problem> p;
+};
+struct OrcState
+{
+    std::unique_ptr<State> s;
+};
+
+void load(XorwowState& s, std::uint32_t const* w)
+{
+    for (int i = 0; i < 5; ++i)
+        s.x[i] = w[i];
+    s.d = w[5];
+}
+void save(XorwowState const& s, std::uint32_t* w)
+{
+    for (int i = 0; i < 5; ++i)
+        w[i] = s.x[i];
+    w[5] = s.d;
+}
+}  // namespace
+
+extern "C" {
+
+const char* orc_last_error()
+{
+    return g_err.c_str();
+}
+
+//---------------------------------------------------------------------------//
+// RNG
+void orc_rng_seeded_states(std::uint32_t seed, std::uint32_t stream,
+                           std::uint32_t n, std::uint32_t* out)
+{
+    auto st = seeded_states(seed, stream, n);
+    for (std::uint32_t i = 0; i < n; ++i)
+        save(st[i], out + 6 * i);
+}
+
+void orc_rng_initialize(std::uint32_t seed, unsigned long long subsequence,
+                        unsigned long long offset, std::uint32_t* out6)
+{
+    XorwowState s;
+    xorwow_initialize(s, seed, subsequence, offset);
+    save(s, out6);
+}
+
+void orc_rng_discard(std::uint32_t* state6, unsigned long long count)
+{
+    XorwowState s;
+    load(s, state6);
+    xorwow_discard(s, count);
+    save(s, state6);
+}
+
+//! kind 0 = u32, 1 = double, 2 = float
+void orc_rng_draw(std::uint32_t* state6, std::uint32_t n, int kind, void* out)
+{
+    XorwowState s;
+    load(s, state6);
+    for (std::uint32_t i = 0; i < n; ++i)
+    {
+        if (kind == 0)
+            static_cast<std::uint32_t*>(out)[i] = xorwow_next(s);
+        else if (kind == 1)
+            static_cast<double*>(out)[i] = canonical_double(s);
+        else
+            static_cast<float*>(out)[i] = canonical_float(s);
+    }
+    save(s, state6);
+}
+
+//---------------------------------------------------------------------------//
+// PROBLEM AND STATE
+void* orc_problem_create(const char* json)
+{
+    OrcProblem* r = nullptr;
+    guard([&] { r = new OrcProblem{Problem::from_json(json)}; });
+    return r;
+}
+
+void orc_problem_destroy(void* p)
+{
+    delete static_cast<OrcProblem*>(p);
+}
+
+int orc_problem_action_label(void* p, std::uint32_t id, char* buf, std::size_t len)
+{
+    return guard([&] {
+        auto const& labels = static_cast<OrcProblem*>(p)->p->actions.labels;
+        std::snprintf(buf, len, "%s", labels.at(id).c_str());
+    });
+}
+
+std::uint32_t orc_problem_num_actions(void* p)
+{
+    return static_cast<std::uint32_t>(static_cast<OrcProblem*>(p)->p->actions.labels.size());
+}
+
+void* orc_state_create(void* problem, std::uint32_t num_slots, std::uint32_t stream)
+{
+    OrcState* r = nullptr;
+    guard([&] { r = new OrcState{make_state(static_cast<OrcProblem*>(problem)->p,
+                                            num_slots, stream)}; });
+    return r;
+}
+
+void orc_state_destroy(void* s)
+{
+    delete static_cast<OrcState*>(s);
+}
+
+//! Primaries in the C ABI layout (particle, event, energy, pos, dir, time)
+struct CPrimary
+{
+    std::uint32_t particle;
+    std::uint32_t event;
+    double energy;
+    double pos[3];
+    double dir[3];
+    double time;
+};
+
+int orc_state_insert_primaries(void* s, CPrimary const* prim, std::uint32_t n)
+{
+    return guard([&] {
+        std::vector<PrimaryP> v;
+        for (std::uint32_t i = 0; i < n; ++i)
+        {
+            PrimaryP p;
+            p.particle = prim[i].particle;
+            p.event = prim[i].event;
+            p.energy = prim[i].energy;
+            for (int k = 0; k < 3; ++k)
+            {
+                p.pos[k] = prim[i].pos[k];
+                p.dir[k] = prim[i].dir[k];
+            }
+            p.time = prim[i].time;
+            v.push_back(p);
+        }
+        insert_primaries(*static_cast<OrcState*>(s)->s, v);
+    });
+}
+
+//! Run an action by label
+int orc_state_action(void* s, const char* label)
+{
+    return guard([&] { run_action(*static_cast<OrcState*>(s)->s, label); });
+}
+
+int orc_state_mock_interact(void* s, std::uint32_t const* nsec, std::uint8_t const* alive)
+{
+    return guard([&] {
+        State& st = *static_cast<OrcState*>(s)->s;
+        std::size_t n = st.tracks.size();
+        mock_interact(st, std::vector<std::uint32_t>(nsec, nsec + n),
+                      std::vector<char>(alive, alive + n));
+    });
+}
+
+//! Counters: initializers, vacancies, active, secondaries, alive, generated
+void orc_state_counters(void* s, std::uint32_t* out)
+{
+    Counters const& c = static_cast<OrcState*>(s)->s->counters;
+    out[0] = c.num_initializers;
+    out[1] = c.num_vacancies;
+    out[2] = c.num_active;
+    out[3] = c.num_secondaries;
+    out[4] = c.num_alive;
+    out[5] = c.num_generated;
+}
+
+//! Integer track fields by name; returns number written
+std::uint32_t orc_state_get_u32(void* s, const char* name, std::uint32_t* out,
+                                std::uint32_t max)
+{
+    State const& st = *static_cast<OrcState*>(s)->s;
+    std::string f(name);
+    std::uint32_t n = 0;
+    auto put = [&](std::uint32_t v) {
+        if (n < max)
+            out[n] = v;
+        ++n;
+    };
+    if (f == "vacancies")
+        for (std::uint32_t i = 0; i < st.counters.num_vacancies; ++i)
+            put(st.vacancies[i]);
+    else if (f == "init_ids")
+        for (std::uint32_t i = 0; i < st.counters.num_initializers; ++i)
+            put(st.initializers[i].track_id);
+    else
+        for (auto const& t : st.tracks)
+        {
+            if (f == "track_id") put(t.track_id);
+            else if (f == "parent_id") put(t.parent_id);
+            else if (f == "event_id") put(t.event_id);
+            else if (f == "status") put(t.status);
+            else if (f == "particle") put(t.particle);
+            else if (f == "volume") put(t.geo.vol);
+            else if (f == "post_action") put(t.post_action);
+            else if (f == "along_action") put(t.along_action);
+            else if (f == "num_steps") put(t.num_steps);
+            else if (f == "rng")
+                for (int k = 0; k < 6; ++k)
+                    put(k < 5 ? t.rng.x[k] : t.rng.d);
+        }
+    return n;
+}
+
+//! Real track fields by name; returns number written
+std::uint32_t orc_state_get_f64(void* s, const char* name, double* out,
+                                std::uint32_t max)
+{
+    State const& st = *static_cast<OrcState*>(s)->s;
+    std::string f(name);
+    std::uint32_t n = 0;
+    auto put = [&](double v) {
+        if (n < max)
+            out[n] = v;
+        ++n;
+    };
+    if (f == "calo_edep")
+    {
+        for (double v : st.calo_edep)
+            put(v);
+        return n;
+    }
+    for (auto const& t : st.tracks)
+    {
+        if (f == "energy") put(t.energy);
+        else if (f == "pos") for (double v : t.geo.pos) put(v);
+        else if (f == "dir") for (double v : t.geo.dir) put(v);
+        else if (f == "time") put(t.time);
+        else if (f == "step_length") put(t.step_length);
+        else if (f == "interaction_mfp") put(t.interaction_mfp);
+        else if (f == "edep") put(t.edep);
+    }
+    return n;
+}
+
+//! Set RNG words of all slots (6 per slot)
+void orc_state_set_rng(void* s, std::uint32_t const* words)
+{
+    State& st = *static_cast<OrcState*>(s)->s;
+    for (std::size_t i = 0; i < st.tracks.size(); ++i)
+        load(st.tracks[i].rng, words + 6 * i);
+}
+
+//---------------------------------------------------------------------------//
+// GEOMETRY
+//! Trace a ray: writes volume IDs and distances until leaving the world
+std::uint32_t orc_geo_trace(void* problem, double const* pos, double const* dir,
+                            std::uint32_t max, std::uint32_t* vols, double* dists,
+                            double* safeties)
+{
+    Geometry const& g = static_cast<OrcProblem*>(problem)->p->geo;
+    GeoState st;
+    geo_initialize(g, st, pos, dir);
+    std::uint32_t n = 0;
+    while (n < max && !geo_is_outside(st) && !st.failed)
+    {
+        bool bnd;
+        double d = geo_find_next_step(g, st, 0, &bnd);
+        vols[n] = st.vol;
+        dists[n] = d;
+        if (safeties)
+        {
+            // Safety at the midpoint of the segment
+            GeoState mid = st;
+            geo_move_internal(mid, d / 2);
+            safeties[n] = geo_find_safety(g, mid);
+        }
+        ++n;
+        if (!bnd)
+            break;
+        geo_move_to_boundary(st);
+        geo_cross_boundary(g, st);
+    }
+    return n;
+}
+
+}  // extern "C"
+
+extern "C" {
+//! GenerateCanonical32 applied to a fixed word sequence (SequenceEngine)
+void orc_canonical_from_words(std::uint32_t const* words, std::uint32_t n_out,
+                              int kind, void* out)
+{
+    std::uint32_t pos = 0;
+    for (std::uint32_t i = 0; i < n_out; ++i)
+    {
+        if (kind == 1)
+        {
+            unsigned long long upper = words[pos++];
+            unsigned long long lower = words[pos++];
+            static_cast<double*>(out)[i]
+                = static_cast<double>((upper << 21) ^ lower) * 0x1p-53;
+        }
+        else
+        {
+            static_cast<float*>(out)[i] = 0x1p-32f * static_cast<float>(words[pos++]);
+        }
+    }
+}
+}
+
+extern "C" {
+//! Draw from many independent states (n x 6 words, updated in place)
+void orc_rng_draw_batch(std::uint32_t* states, std::uint32_t n, std::uint32_t draws,
+                        int kind, void* out)
+{
+#pragma omp parallel for schedule(static)
+    for (std::uint32_t i = 0; i < n; ++i)
+    {
+        std::size_t base = std::size_t(i) * draws;
+        if (kind == 0)
+            orc_rng_draw(states + 6 * std::size_t(i), draws, 0,
+                         static_cast<std::uint32_t*>(out) + base);
+        else if (kind == 1)
+            orc_rng_draw(states + 6 * std::size_t(i), draws, 1,
+                         static_cast<double*>(out) + base);
+        else
+            orc_rng_draw(states + 6 * std::size_t(i), draws, 2,
+                         static_cast<float*>(out) + base);
+    }
+}
+
+//! Initialize many states with (subsequence, offset) pairs
+void orc_rng_initialize_batch(std::uint32_t seed, std::uint32_t n,
+                              unsigned long long const* sub,
+                              unsigned long long const* off, std::uint32_t* out)
+{
+#pragma omp parallel for schedule(dynamic, 64)
+    for (std::uint32_t i = 0; i < n; ++i)
+        orc_rng_initialize(seed, sub[i], off[i], out + 6 * std::size_t(i));
+}
+
+//! Initializer records (same layout as the product's C ABI struct)
+std::uint32_t orc_state_initializers(void* s, void* out, std::uint32_t max)
+{
+    struct Rec
+    {
+        std::uint32_t track_id, parent_id, event_id, particle;
+        double time, pos[3], dir[3], energy;
+    };
+    State const& st = *static_cast<OrcState*>(s)->s;
+    std::uint32_t n = st.counters.num_initializers;
+    Rec* r = static_cast<Rec*>(out);
+    for (std::uint32_t i = 0; i < n && i < max; ++i)
+    {
+        Initializer const& ti = st.initializers[i];
+        r[i].track_id = ti.track_id;
+        r[i].parent_id = ti.parent_id;
+        r[i].event_id = ti.event_id;
+        r[i].particle = ti.particle;
+        r[i].time = ti.time;
+        r[i].energy = ti.energy;
+        for (int k = 0; k < 3; ++k)
+        {
+            r[i].pos[k] = ti.pos[k];
+            r[i].dir[k] = ti.dir[k];
+        }
+    }
+    return n;
+}
+}

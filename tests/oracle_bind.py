@@ -1,0 +1,221 @@
+"""ctypes binding of the oracle (oracle/_build/liboracle.so).
+
+The oracle is test infrastructure: a CPU restatement of the reference
+algorithms used as the checker for the HIP path. Only tests/, the smoke
+check and bench.py's cpu_baseline leg load it.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+
+_lib = None
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OracleError(f"oracle not built: {LIB_PATH}")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, u32, i32, u64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, ctypes.c_ulonglong
+    sigs = {
+        "orc_last_error": (ctypes.c_char_p, []),
+        "orc_rng_seeded_states": (None, [u32, u32, u32, vp]),
+        "orc_rng_initialize": (None, [u32, u64, u64, vp]),
+        "orc_rng_discard": (None, [vp, u64]),
+        "orc_rng_draw": (None, [vp, u32, i32, vp]),
+        "orc_problem_create": (vp, [ctypes.c_char_p]),
+        "orc_problem_destroy": (None, [vp]),
+        "orc_problem_action_label": (i32, [vp, u32, ctypes.c_char_p, ctypes.c_size_t]),
+        "orc_problem_num_actions": (u32, [vp]),
+        "orc_state_create": (vp, [vp, u32, u32]),
+        "orc_state_destroy": (None, [vp]),
+        "orc_state_insert_primaries": (i32, [vp, vp, u32]),
+        "orc_state_action": (i32, [vp, ctypes.c_char_p]),
+        "orc_state_mock_interact": (i32, [vp, vp, vp]),
+        "orc_state_counters": (None, [vp, vp]),
+        "orc_state_get_u32": (u32, [vp, ctypes.c_char_p, vp, u32]),
+        "orc_state_get_f64": (u32, [vp, ctypes.c_char_p, vp, u32]),
+        "orc_state_set_rng": (None, [vp, vp]),
+        "orc_geo_trace": (u32, [vp, vp, vp, u32, vp, vp, vp]),
+    }
+    for name, (res, args) in sigs.items():
+        if not hasattr(lib, name):
+            continue
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise OracleError(load().orc_last_error().decode())
+
+
+# --------------------------------------------------------------------------- #
+# RNG
+# --------------------------------------------------------------------------- #
+def rng_seeded_states(seed: int, stream: int, n: int) -> np.ndarray:
+    out = np.zeros((n, 6), dtype=np.uint32)
+    load().orc_rng_seeded_states(seed, stream, n, out.ctypes.data)
+    return out
+
+
+def rng_initialize(seed: int, subsequence: int, offset: int) -> np.ndarray:
+    out = np.zeros(6, dtype=np.uint32)
+    load().orc_rng_initialize(seed, subsequence, offset, out.ctypes.data)
+    return out
+
+
+def rng_discard(state6: np.ndarray, count: int) -> np.ndarray:
+    s = np.array(state6, dtype=np.uint32)
+    load().orc_rng_discard(s.ctypes.data, count)
+    return s
+
+
+def rng_draw(state6: np.ndarray, n: int, kind: str = "u32"):
+    """Return (values, new_state)."""
+    k = {"u32": 0, "f64": 1, "f32": 2}[kind]
+    dtype = {0: np.uint32, 1: np.float64, 2: np.float32}[k]
+    s = np.array(state6, dtype=np.uint32)
+    out = np.zeros(n, dtype=dtype)
+    load().orc_rng_draw(s.ctypes.data, n, k, out.ctypes.data)
+    return out, s
+
+
+# --------------------------------------------------------------------------- #
+# PROBLEM / STATE
+# --------------------------------------------------------------------------- #
+class Problem:
+    def __init__(self, problem):
+        text = problem if isinstance(problem, str) else json.dumps(problem)
+        self._h = load().orc_problem_create(text.encode())
+        if not self._h:
+            raise OracleError(load().orc_last_error().decode())
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            load().orc_problem_destroy(self._h)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def action_labels(self):
+        n = load().orc_problem_num_actions(self._h)
+        result = []
+        for i in range(n):
+            buf = ctypes.create_string_buffer(256)
+            _check(load().orc_problem_action_label(self._h, i, buf, 256))
+            result.append(buf.value.decode())
+        return result
+
+    def trace(self, pos, dir, max_segments=1000, safety=False):
+        p = np.asarray(pos, dtype=np.float64)
+        d = np.asarray(dir, dtype=np.float64)
+        vols = np.zeros(max_segments, dtype=np.uint32)
+        dists = np.zeros(max_segments, dtype=np.float64)
+        safe = np.zeros(max_segments, dtype=np.float64) if safety else None
+        n = load().orc_geo_trace(self._h, p.ctypes.data, d.ctypes.data, max_segments,
+                                 vols.ctypes.data, dists.ctypes.data,
+                                 safe.ctypes.data if safety else None)
+        if safety:
+            return vols[:n], dists[:n], safe[:n]
+        return vols[:n], dists[:n]
+
+
+class State:
+    def __init__(self, problem: Problem, num_slots: int, stream_id: int = 0):
+        self.problem = problem
+        self.size = num_slots
+        self._h = load().orc_state_create(problem.handle, num_slots, stream_id)
+        if not self._h:
+            raise OracleError(load().orc_last_error().decode())
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            load().orc_state_destroy(self._h)
+
+    def insert_primaries(self, primaries: np.ndarray):
+        from celeritas_amd import PRIMARY_DTYPE
+
+        p = np.ascontiguousarray(primaries, dtype=PRIMARY_DTYPE)
+        _check(load().orc_state_insert_primaries(self._h, p.ctypes.data, len(p)))
+
+    def action(self, label: str):
+        _check(load().orc_state_action(self._h, label.encode()))
+
+    def mock_interact(self, num_secondaries, alive):
+        n = np.ascontiguousarray(num_secondaries, dtype=np.uint32)
+        a = np.ascontiguousarray(alive, dtype=np.uint8)
+        _check(load().orc_state_mock_interact(self._h, n.ctypes.data, a.ctypes.data))
+
+    def counters(self) -> dict:
+        out = np.zeros(6, dtype=np.uint32)
+        load().orc_state_counters(self._h, out.ctypes.data)
+        names = ["num_initializers", "num_vacancies", "num_active", "num_secondaries",
+                 "num_alive", "num_generated"]
+        return dict(zip(names, (int(v) for v in out)))
+
+    def get_u32(self, name: str, max_count: int = 1 << 22) -> np.ndarray:
+        out = np.zeros(max_count, dtype=np.uint32)
+        n = load().orc_state_get_u32(self._h, name.encode(), out.ctypes.data, max_count)
+        return out[:n].copy()
+
+    def get_f64(self, name: str, max_count: int = 1 << 22) -> np.ndarray:
+        out = np.zeros(max_count, dtype=np.float64)
+        n = load().orc_state_get_f64(self._h, name.encode(), out.ctypes.data, max_count)
+        return out[:n].copy()
+
+    def set_rng(self, words: np.ndarray):
+        w = np.ascontiguousarray(words, dtype=np.uint32)
+        load().orc_state_set_rng(self._h, w.ctypes.data)
+
+
+def rng_draw_batch(states: np.ndarray, draws: int, kind: str = "u32"):
+    k = {"u32": 0, "f64": 1, "f32": 2}[kind]
+    dtype = {0: np.uint32, 1: np.float64, 2: np.float32}[k]
+    st = np.ascontiguousarray(states, dtype=np.uint32).copy()
+    out = np.zeros((len(st), draws), dtype=dtype)
+    f = load().orc_rng_draw_batch
+    f.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p]
+    f(st.ctypes.data, len(st), draws, k, out.ctypes.data)
+    return out, st
+
+
+def rng_initialize_batch(seed: int, subsequence, offset) -> np.ndarray:
+    sub = np.ascontiguousarray(subsequence, dtype=np.uint64)
+    off = np.ascontiguousarray(offset, dtype=np.uint64)
+    out = np.zeros((len(sub), 6), dtype=np.uint32)
+    f = load().orc_rng_initialize_batch
+    f.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
+                  ctypes.c_void_p]
+    f(seed, len(sub), sub.ctypes.data, off.ctypes.data, out.ctypes.data)
+    return out
+
+
+def state_initializers(state: "State") -> np.ndarray:
+    from celeritas_amd import INITIALIZER_DTYPE
+
+    n = state.counters()["num_initializers"]
+    out = np.zeros(max(n, 1), dtype=INITIALIZER_DTYPE)
+    f = load().orc_state_initializers
+    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]
+    f.restype = ctypes.c_uint32
+    f(state._h, out.ctypes.data, len(out))
+    return out[:n]
