@@ -454,7 +454,7 @@ int celer_hip_extend_from_primaries(const void* state_ref,
 {
     return launch_extend_from_primaries(
         *static_cast<CoreStateRef const*>(state_ref),
-        reinterpret_cast<Primary const*>(device_primaries), count,
+        reinterpret_cast<Primary const*>(device_primaries), nullptr, count,
         static_cast<hipStream_t>(stream));
 }
 

@@ -182,5 +182,12 @@ Primary* CoreState::primary_buffer(size_type count)
     return primaries_.data();
 }
 
+std::uint32_t* CoreState::primary_rank_buffer(size_type count)
+{
+    if (primary_ranks_.size() < count)
+        primary_ranks_.resize(std::max<std::size_t>(count, 2 * primary_ranks_.size()));
+    return primary_ranks_.data();
+}
+
 //---------------------------------------------------------------------------//
 }  // namespace celeritas_amd

@@ -52,6 +52,8 @@ class CoreState
 
     //! Scratch buffer for primaries (grown on demand)
     Primary* primary_buffer(size_type count);
+    //! Scratch buffer for primary event ranks (grown on demand)
+    std::uint32_t* primary_rank_buffer(size_type count);
 
     bool warming_up() const { return warming_up_; }
     void warming_up(bool w) { warming_up_ = w; }
@@ -65,6 +67,7 @@ class CoreState
     hipStream_t stream_{nullptr};
     CoreCounters* host_counters_{nullptr};
     DeviceVector<Primary> primaries_;
+    DeviceVector<std::uint32_t> primary_ranks_;
     size_type num_detectors_{0};
     bool warming_up_{false};
     std::vector<std::uint32_t> initial_rng_;

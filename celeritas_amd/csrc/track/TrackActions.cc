@@ -6,6 +6,7 @@
 #include "TrackActions.hh"
 
 #include <algorithm>
+#include <unordered_map>
 
 #include "global/CoreParams.hh"
 #include "global/CoreState.hh"
@@ -24,13 +25,34 @@ void ExtendFromPrimariesAction::insert(CoreParams const& params,
                                        Primary const* host_primaries,
                                        size_type count) const
 {
+    // Event-local ranks make track IDs reproduce the serial host order
+    std::vector<std::uint32_t> ranks(2 * std::size_t(count), 0);
+    {
+        std::unordered_map<std::uint32_t, std::uint32_t> seen;
+        std::unordered_map<std::uint32_t, size_type> last;
+        for (size_type i = 0; i < count; ++i)
+        {
+            ranks[i] = seen[host_primaries[i].event_id]++;
+            last[host_primaries[i].event_id] = i;
+        }
+        for (auto const& kv : last)
+            ranks[count + kv.second] = 1;
+    }
     Primary* buf = state.primary_buffer(count);
+    std::uint32_t* dranks = state.primary_rank_buffer(2 * count);
     CELER_HIP_CALL(hipMemcpyAsync(buf,
                                   host_primaries,
                                   count * sizeof(Primary),
                                   hipMemcpyHostToDevice,
                                   state.stream()));
+    CELER_HIP_CALL(hipMemcpyAsync(dranks,
+                                  ranks.data(),
+                                  ranks.size() * sizeof(std::uint32_t),
+                                  hipMemcpyHostToDevice,
+                                  state.stream()));
+    CELER_HIP_CALL(hipStreamSynchronize(state.stream()));
     this->insert_device(params, state, buf, count);
+    pending_.back().ranks = dranks;
 }
 
 void ExtendFromPrimariesAction::insert_device(CoreParams const& params,
@@ -49,12 +71,13 @@ void ExtendFromPrimariesAction::insert_device(CoreParams const& params,
                    << "insufficient initializer capacity ("
                    << params.init_capacity() << ") with size (" << num_init
                    << ") for primaries (" << count << ")");
-    pending_.push_back({&state, device_primaries, count});
+    pending_.push_back({&state, device_primaries, nullptr, count});
 }
 
 void ExtendFromPrimariesAction::step(CoreParams const&, CoreState& state) const
 {
     Primary const* data = nullptr;
+    std::uint32_t const* ranks = nullptr;
     size_type count = 0;
     auto iter = std::find_if(pending_.begin(), pending_.end(), [&](Pending const& p) {
         return p.state == &state;
@@ -62,10 +85,11 @@ void ExtendFromPrimariesAction::step(CoreParams const&, CoreState& state) const
     if (iter != pending_.end())
     {
         data = iter->data;
+        ranks = iter->ranks;
         count = iter->count;
         pending_.erase(iter);
     }
-    CELER_HIP_CALL(launch_extend_from_primaries(state.ref(), data, count, state.stream()));
+    CELER_HIP_CALL(launch_extend_from_primaries(state.ref(), data, ranks, count, state.stream()));
 }
 
 //---------------------------------------------------------------------------//

@@ -45,6 +45,7 @@ class ExtendFromPrimariesAction final : public StepActionInterface,
     {
         CoreState const* state;
         Primary const* data;
+        std::uint32_t const* ranks;
         size_type count;
     };
     mutable std::vector<Pending> pending_;

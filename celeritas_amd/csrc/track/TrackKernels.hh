@@ -13,8 +13,10 @@
 namespace celeritas_amd
 {
 //---------------------------------------------------------------------------//
+//! \param ranks null, or 2*count words: event-local rank, then last flag
 hipError_t launch_extend_from_primaries(CoreStateRef const& state,
                                         Primary const* primaries,
+                                        std::uint32_t const* ranks,
                                         size_type count,
                                         hipStream_t stream);
 hipError_t launch_initialize_tracks(CoreParamsRef const& params,

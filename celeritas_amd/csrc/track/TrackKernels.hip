@@ -47,12 +47,16 @@ __global__ void primaries_prepare_kernel(CoreCounters* c, size_type count)
 /*!
  * Create initializers from primaries (ProcessPrimariesExecutor.hh:56-76).
  *
- * Track IDs come from an atomic per-event counter, as in the reference; the
- * hardware returns lane-ordered results within a wave.
+ * With event-local ranks (computed by the host when it copies the batch)
+ * the track ID is the event's counter plus the rank, which reproduces the
+ * serial order of the reference's host loop; the counters are advanced by
+ * a second kernel. Without ranks the ID comes from an atomic per-event
+ * counter, as in the reference's device path.
  */
 __global__ void __launch_bounds__(block_size)
     process_primaries_kernel(CoreStateRef const state,
                              Primary const* __restrict__ primaries,
+                             std::uint32_t const* __restrict__ ranks,
                              size_type count)
 {
     size_type const tid = blockIdx.x * block_size + threadIdx.x;
@@ -63,7 +67,8 @@ __global__ void __launch_bounds__(block_size)
     TrackInitializer ti;
     if (p.event_id < state.init.max_events)
     {
-        ti.track_id = atomicAdd(&state.init.track_counters[p.event_id], 1u);
+        ti.track_id = ranks ? state.init.track_counters[p.event_id] + ranks[tid]
+                            : atomicAdd(&state.init.track_counters[p.event_id], 1u);
     }
     else
     {
@@ -82,6 +87,22 @@ __global__ void __launch_bounds__(block_size)
     ti.energy = p.energy;
     if (idx < state.init.capacity)
         state.init.initializers[idx] = ti;
+}
+
+//! Advance per-event counters after ranked ID assignment (last of each event)
+__global__ void __launch_bounds__(block_size)
+    advance_track_counters_kernel(CoreStateRef const state,
+                                  Primary const* __restrict__ primaries,
+                                  std::uint32_t const* __restrict__ ranks,
+                                  std::uint32_t const* __restrict__ is_last,
+                                  size_type count)
+{
+    size_type const tid = blockIdx.x * block_size + threadIdx.x;
+    if (tid >= count || !is_last[tid])
+        return;
+    std::uint32_t const ev = primaries[tid].event_id;
+    if (ev < state.init.max_events)
+        state.init.track_counters[ev] += ranks[tid] + 1;
 }
 
 //---------------------------------------------------------------------------//
@@ -349,6 +370,7 @@ __global__ void iota_u32_kernel(std::uint32_t* data, size_type n)
 //---------------------------------------------------------------------------//
 hipError_t launch_extend_from_primaries(CoreStateRef const& state,
                                         Primary const* primaries,
+                                        std::uint32_t const* ranks,
                                         size_type count,
                                         hipStream_t stream)
 {
@@ -356,7 +378,12 @@ hipError_t launch_extend_from_primaries(CoreStateRef const& state,
     if (count > 0)
     {
         process_primaries_kernel<<<num_blocks(count), block_size, 0, stream>>>(
-            state, primaries, count);
+            state, primaries, ranks, count);
+        if (ranks)
+        {
+            advance_track_counters_kernel<<<num_blocks(count), block_size, 0, stream>>>(
+                state, primaries, ranks, ranks + count, count);
+        }
     }
     return hipGetLastError();
 }

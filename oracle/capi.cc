@@ -13,6 +13,7 @@
 #include "problem.hh"
 #include "rng.hh"
 #include "loop.hh"
+#include <stdexcept>
 
 using namespace orc;
 
@@ -402,5 +403,34 @@ std::uint32_t orc_state_initializers(void* s, void* out, std::uint32_t max)
         }
     }
     return n;
+}
+}
+
+extern "C" {
+//! Run step iterations like StepperTestBase::run: first step with the
+//! primaries, then until empty or max_iters. Writes active/queued per step.
+std::uint32_t orc_run(void* state, void const* prim, std::uint32_t nprim,
+                      std::uint32_t max_iters, std::uint32_t* active,
+                      std::uint32_t* queued)
+{
+    std::uint32_t n = 0;
+    int rc = guard([&] {
+        State& st = *static_cast<OrcState*>(state)->s;
+        orc_state_insert_primaries(state, static_cast<CPrimary const*>(prim), nprim);
+        if (!g_err.empty())
+            throw std::runtime_error(g_err);
+        StepResult r = step_once(st);
+        active[n] = r.active;
+        queued[n] = r.queued;
+        ++n;
+        while ((r.queued > 0 || r.alive > 0) && n < max_iters)
+        {
+            r = step_once(st);
+            active[n] = r.active;
+            queued[n] = r.queued;
+            ++n;
+        }
+    });
+    return rc == 0 ? n : 0;
 }
 }

@@ -3,12 +3,200 @@
 //---------------------------------------------------------------------------//
 #include "physics.hh"
 
+#include <cmath>
 #include <limits>
+#include <stdexcept>
 
+#include "em.hh"
 #include "problem.hh"
 
 namespace orc
 {
+namespace
+{
+//---------------------------------------------------------------------------//
+constexpr double inf = std::numeric_limits<double>::infinity();
+constexpr double c_light = 29979245800.0;  // [cm/s]
+
+//! Build an XS grid from lambda (unscaled) and lambda_prim (scaled by E)
+//! vectors (grid/ValueGridBuilder.cc:85-172)
+XsGridO grid_from_vectors(Json const* lo, Json const* hi)
+{
+    XsGridO g;
+    std::vector<double> values;
+    double emin, eprime, emax;
+    if (lo && hi)
+    {
+        auto lx = (*lo)("x").nums(), ly = (*lo)("y").nums();
+        auto hx = (*hi)("x").nums(), hy = (*hi)("y").nums();
+        values.assign(ly.begin(), ly.end() - 1);
+        values.insert(values.end(), hy.begin(), hy.end());
+        emin = lx.front();
+        eprime = hx.front();
+        emax = hx.back();
+    }
+    else if (hi)
+    {
+        auto hx = (*hi)("x").nums();
+        values = (*hi)("y").nums();
+        emin = eprime = hx.front();
+        emax = hx.back();
+    }
+    else
+    {
+        auto lx = (*lo)("x").nums();
+        values = (*lo)("y").nums();
+        emin = lx.front();
+        emax = lx.back();
+        eprime = 0;
+    }
+    g.front = std::log(emin);
+    g.back = std::log(emax);
+    g.size = static_cast<std::uint32_t>(values.size());
+    g.delta = (g.back - g.front) / (g.size - 1);
+    g.values = values;
+    if (eprime > 0)
+    {
+        double le = std::log(eprime);
+        auto p = static_cast<std::uint32_t>((le - g.front) / g.delta);
+        double gp1 = g.front + g.delta * (p + 1);
+        if (std::fabs(gp1 - le) <= 1e-12 * std::max(1.0, std::fabs(le)))
+            ++p;
+        g.prime = p;
+    }
+    return g;
+}
+
+//! Grid given directly as {emin, emax, prime, values}
+XsGridO grid_from_json(Json const& j)
+{
+    XsGridO g;
+    g.values = j("values").nums();
+    g.size = static_cast<std::uint32_t>(g.values.size());
+    g.front = std::log(j.num("emin", 0));
+    g.back = std::log(j.num("emax", 0));
+    g.delta = (g.back - g.front) / (g.size - 1);
+    double p = j.num("prime", -1);
+    g.prime = p < 0 ? 0xffffffffu : static_cast<std::uint32_t>(p);
+    return g;
+}
+
+std::vector<XsGridO> grids_of(Json const& proc, char const* direct,
+                              char const* lo_key, char const* hi_key,
+                              std::size_t num_mats)
+{
+    std::vector<XsGridO> result(num_mats);
+    if (auto d = proc.find(direct))
+    {
+        for (std::size_t m = 0; m < num_mats && m < d->a.size(); ++m)
+            if (d->a[m].kind == Json::obj_k)
+                result[m] = grid_from_json(d->a[m]);
+        return result;
+    }
+    Json const* lo = lo_key ? proc.find(lo_key) : nullptr;
+    Json const* hi = hi_key ? proc.find(hi_key) : nullptr;
+    if (!lo && !hi)
+        return result;
+    for (std::size_t m = 0; m < num_mats; ++m)
+    {
+        Json const* l = lo ? &lo->a.at(m) : nullptr;
+        Json const* h = hi ? &hi->a.at(m) : nullptr;
+        result[m] = grid_from_vectors(l, h);
+    }
+    return result;
+}
+
+//---------------------------------------------------------------------------//
+// Helpers mirroring the track views
+//---------------------------------------------------------------------------//
+PhysicsData const& phys(State const& s)
+{
+    return *s.problem->physics;
+}
+
+std::uint32_t material_of(State const& s, Track const& t)
+{
+    return s.problem->volume_to_material[t.geo.vol];
+}
+
+double calc_xs(State const& s, ProcessDef const& p, std::uint32_t mat, double e)
+{
+    XsGridO const& g = p.xs[mat];
+    if (!g)
+        return hardwired_xs(s, p, mat, e);
+    return g.calc(e);
+}
+
+//! PhysicsTrackView::calc_max_xs (PhysicsTrackView.hh:446-478)
+double calc_max_xs(State const& s, ProcessDef const& p, std::uint32_t mat, double e)
+{
+    PhysicsData const& pd = phys(s);
+    double emax = p.energy_max_xs[mat];
+    double exi = e * pd.min_eprime_over_e;
+    if (emax >= exi && emax < e)
+        return calc_xs(s, p, mat, emax);
+    return std::max(calc_xs(s, p, mat, e), calc_xs(s, p, mat, exi));
+}
+
+//! PhysicsTrackView::range_to_step (PhysicsTrackView.hh:624-640)
+double range_to_step(PhysicsData const& pd, double range)
+{
+    double rho = pd.min_range;
+    if (range < rho * (1 + std::sqrt(std::numeric_limits<double>::epsilon())))
+        return range;
+    double alpha = pd.max_step_over_range;
+    return alpha * range + rho * (1 - alpha) * (2 - rho / range);
+}
+
+//---------------------------------------------------------------------------//
+}  // namespace
+
+//---------------------------------------------------------------------------//
+// XS GRID
+//---------------------------------------------------------------------------//
+double XsGridO::energy(std::uint32_t i) const
+{
+    return std::exp(front + delta * i);
+}
+
+double XsGridO::at(std::uint32_t i) const
+{
+    double r = values[i];
+    if (i >= prime)
+        r /= this->energy(i);
+    return r;
+}
+
+double XsGridO::calc(double e) const
+{
+    double loge = std::log(e);
+    auto extrapolated = [&](std::uint32_t i) {
+        double r = values[i];
+        if (i >= prime)
+            r /= e;
+        return r;
+    };
+    if (loge <= front)
+        return extrapolated(0);
+    if (loge >= back)
+        return extrapolated(size - 1);
+    auto lower = static_cast<std::uint32_t>((loge - front) / delta);
+    double upper_e = std::exp(front + delta * (lower + 1));
+    double upper_xs = values[lower + 1];
+    if (lower + 1 == prime)
+        upper_xs /= upper_e;
+    double lower_e = std::exp(front + delta * lower);
+    double lower_xs = values[lower];
+    // Interpolator<linear, linear>: fma(slope, x - x_l, y_l)
+    double slope = ((-lower_xs) + upper_xs) / ((-lower_e) + upper_e);
+    double result = std::fma(slope, (-lower_e) + e, lower_xs);
+    if (lower >= prime)
+        result /= e;
+    return result;
+}
+
+//---------------------------------------------------------------------------//
+// SETUP
 //---------------------------------------------------------------------------//
 void build_physics(Json const& inp, PhysicsData& out)
 {
@@ -24,24 +212,156 @@ void build_physics(Json const& inp, PhysicsData& out)
             out.particles.push_back(d);
         }
     }
+    for (std::uint32_t i = 0; i < out.particles.size(); ++i)
+    {
+        if (out.particles[i].pdg == 22)
+            out.gamma = i;
+        else if (out.particles[i].pdg == 11)
+            out.electron = i;
+        else if (out.particles[i].pdg == -11)
+            out.positron = i;
+    }
+    if (auto els = inp.find("elements"))
+    {
+        for (auto const& e : els->a)
+        {
+            ElementDef d;
+            d.z = static_cast<int>(e.num("z", 0));
+            d.atomic_mass = e.num("atomic_mass", 0);
+            out.elements.push_back(d);
+        }
+    }
+    if (auto mats = inp.find("materials"))
+    {
+        for (auto const& m : mats->a)
+        {
+            MaterialDef d;
+            d.label = m.str("label", "");
+            d.number_density = m.num("number_density", 0);
+            d.electron_density = m.num("electron_density", 0);
+            d.mean_exc_energy = m.num("mean_exc_energy", 0);
+            if (auto comps = m.find("elements"))
+                for (auto const& c : comps->a)
+                    d.elements.emplace_back(
+                        static_cast<std::uint32_t>(c.num("element", 0)),
+                        c.num("fraction", 1));
+            out.materials.push_back(d);
+        }
+    }
+    std::size_t const num_mats = std::max<std::size_t>(out.materials.size(), 1);
     if (auto opts = inp.find("options"))
+    {
         out.secondary_stack_factor = opts->num("secondary_stack_factor", 3);
+        out.min_range = opts->num("min_range", out.min_range);
+        out.max_step_over_range = opts->num("max_step_over_range", out.max_step_over_range);
+        out.min_eprime_over_e = opts->num("min_eprime_over_e", out.min_eprime_over_e);
+        out.linear_loss_limit = opts->num("linear_loss_limit", out.linear_loss_limit);
+        out.lowest_electron_energy
+            = opts->num("lowest_electron_energy", out.lowest_electron_energy);
+        out.fixed_step_limiter = opts->num("fixed_step_limiter", 0);
+        out.apply_post_interaction_cut = opts->flag("apply_cuts", false);
+    }
+    // Cutoffs: {"particle label": [{"energy": E}, ...per material]}
+    out.cutoff_energy.assign(out.particles.size(), std::vector<double>(num_mats, 0));
+    if (auto cuts = inp.find("cutoffs"))
+    {
+        for (auto const& kv : cuts->o)
+        {
+            for (std::uint32_t p = 0; p < out.particles.size(); ++p)
+            {
+                if (out.particles[p].label != kv.first)
+                    continue;
+                for (std::size_t m = 0; m < num_mats && m < kv.second.a.size(); ++m)
+                    out.cutoff_energy[p][m] = kv.second.a[m].num("energy", 0);
+            }
+        }
+    }
+    out.particle_processes.assign(out.particles.size(), {});
+    out.eloss_process.assign(out.particles.size(), -1);
+    if (auto procs = inp.find("processes"))
+    {
+        for (auto const& pj : procs->a)
+        {
+            ProcessDef p;
+            p.label = pj.str("label", "");
+            p.cls = pj.str("class", "");
+            std::string plabel = pj.str("particle", "");
+            for (std::uint32_t i = 0; i < out.particles.size(); ++i)
+                if (out.particles[i].label == plabel)
+                    p.particle = i;
+            if (p.particle == none)
+                throw std::runtime_error("oracle: unknown process particle " + plabel);
+            p.integral = pj.flag("integral", false);
+            p.xs = grids_of(pj, "xs", "lambda", "lambda_prim", num_mats);
+            p.dedx = grids_of(pj, "dedx_grid", "dedx", nullptr, num_mats);
+            p.range = grids_of(pj, "range_grid", "range", nullptr, num_mats);
+            for (auto const& mj : pj("models").a)
+            {
+                ModelDef m;
+                m.label = mj.str("label", "");
+                m.model = mj.str("model", "");
+                m.low = mj.num("low", 0);
+                m.high = mj.num("high", 0);
+                p.models.push_back(m);
+                out.model_labels.push_back(m.label);
+            }
+            if (p.integral)
+            {
+                p.energy_max_xs.assign(num_mats, 0);
+                for (std::size_t m = 0; m < num_mats; ++m)
+                {
+                    XsGridO const& g = p.xs[m];
+                    if (p.cls == "annihilation" || !g)
+                        continue;
+                    double xs_max = 0, e_max = 0;
+                    for (std::uint32_t i = 0; i < g.size; ++i)
+                    {
+                        double xs = g.at(i);
+                        if (xs > xs_max)
+                        {
+                            xs_max = xs;
+                            e_max = g.energy(i);
+                        }
+                    }
+                    p.energy_max_xs[m] = e_max;
+                }
+            }
+            auto idx = static_cast<std::uint32_t>(out.processes.size());
+            auto& list = out.particle_processes[p.particle];
+            if (!p.dedx.empty() && p.dedx[0])
+                out.eloss_process[p.particle] = static_cast<int>(list.size());
+            list.push_back(idx);
+            out.processes.push_back(std::move(p));
+        }
+    }
 }
 
 void PhysicsData::finalize_actions(ActionIds const& ids)
 {
-    pre_step_action = ids.pre_step;
+    discrete_action = ids.discrete_select;
+    range_action = ids.eloss_range;
+    integral_rejection_action = ids.integral_rejected;
+    failure_action = ids.failure;
+    fixed_step_action = ids.eloss_range;
+    std::size_t k = 0;
+    for (auto& p : processes)
+        for (auto& m : p.models)
+            m.action = ids.model.at(k++);
 }
 
+//---------------------------------------------------------------------------//
+// PRE-STEP (PreStepExecutor.hh:46-115, PhysicsStepUtils.hh:37-120)
 //---------------------------------------------------------------------------//
 void pre_step(State& s)
 {
     s.stack_size = 0;
+    PhysicsData const& pd = phys(s);
+    ActionIds const& ids = s.problem->actions;
     for (auto& t : s.tracks)
     {
         if (t.status == inactive)
         {
-            t.step_length = std::numeric_limits<double>::infinity();
+            t.step_length = inf;
             t.post_action = none;
             t.along_action = none;
             continue;
@@ -53,6 +373,307 @@ void pre_step(State& s)
         if (t.status == errored)
             continue;
         t.status = alive;
+        if (pd.particles.empty())
+            continue;
+
+        if (!(t.interaction_mfp > 0))
+        {
+            // ExponentialDistribution(1): -log(u)
+            t.interaction_mfp = -std::log(canonical_double(t.rng));
+        }
+        std::uint32_t const mat = material_of(s, t);
+        auto const& procs = pd.particle_processes[t.particle];
+        double total = 0;
+        for (std::size_t i = 0; i < procs.size(); ++i)
+        {
+            ProcessDef const& p = pd.processes[procs[i]];
+            double xs = p.integral ? calc_max_xs(s, p, mat, t.energy)
+                                   : calc_xs(s, p, mat, t.energy);
+            total += xs;
+            t.per_process_xs[i] = xs;
+        }
+        t.macro_xs = total;
+        double step;
+        std::uint32_t action = pd.discrete_action;
+        if (t.energy == 0)
+        {
+            step = 0;
+        }
+        else
+        {
+            step = t.interaction_mfp / total;
+            int ep = pd.eloss_process[t.particle];
+            if (ep >= 0)
+            {
+                ProcessDef const& p = pd.processes[procs[ep]];
+                double range = p.range[mat].calc(t.energy);
+                t.dedx_range = range;
+                double eloss_step = range_to_step(pd, range);
+                if (eloss_step <= step)
+                {
+                    step = eloss_step;
+                    action = pd.range_action;
+                }
+                if (pd.fixed_step_limiter > 0 && pd.fixed_step_limiter < step)
+                {
+                    step = pd.fixed_step_limiter;
+                    action = pd.fixed_step_action;
+                }
+            }
+            else if (procs.empty())
+            {
+                action = none;
+            }
+        }
+        t.step_length = step;
+        t.post_action = action;
+        double const charge = pd.particles[t.particle].charge;
+        t.along_action = (charge == 0) ? ids.along_neutral : ids.along_user;
+    }
+}
+
+//---------------------------------------------------------------------------//
+// ALONG-STEP (NEUTRAL): linear propagation, time, MFP update
+//---------------------------------------------------------------------------//
+namespace
+{
+//! PropagationApplier with a LinearPropagator (PropagationApplier.hh:95-195)
+void propagate_linear(State& s, Track& t)
+{
+    ActionIds const& ids = s.problem->actions;
+    if (t.step_length == 0)
+        return;
+    bool boundary_hit;
+    double dist = geo_find_next_step(s.problem->geo, t.geo, t.step_length, &boundary_hit);
+    if (boundary_hit)
+        geo_move_to_boundary(t.geo);
+    else
+        geo_move_internal(t.geo, t.step_length);
+    if (boundary_hit)
+    {
+        t.step_length = dist;
+        t.post_action = ids.boundary;
+    }
+    else if (dist < t.step_length)
+    {
+        t.step_length = dist;
+        t.post_action = ids.propagation_limit;
+    }
+}
+
+//! TimeUpdater (TimeUpdater.hh:28-45)
+void update_time(State& s, Track& t)
+{
+    if (t.status == errored)
+        return;
+    double const mass = phys(s).particles[t.particle].mass;
+    double inv_gamma = mass / (t.energy + mass);
+    double beta_sq = 1 - inv_gamma * inv_gamma;
+    double speed = std::sqrt(beta_sq) * c_light;
+    if (speed > 0)
+        t.time += t.step_length / speed;
+}
+
+//! TrackUpdater (TrackUpdater.hh:32-66)
+void update_track(State& s, Track& t)
+{
+    if (t.status == errored)
+        return;
+    if (t.status == alive && t.post_action != phys(s).discrete_action)
+    {
+        t.interaction_mfp -= t.step_length * t.macro_xs;
+    }
+    ++t.num_steps;
+}
+}  // namespace
+
+void along_step_neutral(State& s)
+{
+    std::uint32_t const aid = s.problem->actions.along_neutral;
+    for (auto& t : s.tracks)
+    {
+        if (t.status == inactive || t.along_action != aid)
+            continue;
+        propagate_linear(s, t);
+        update_time(s, t);
+        update_track(s, t);
+    }
+}
+
+//---------------------------------------------------------------------------//
+// DISCRETE SELECT (DiscreteSelectExecutor.hh:38-63, PhysicsStepUtils.hh:253)
+//---------------------------------------------------------------------------//
+void discrete_select(State& s)
+{
+    PhysicsData const& pd = phys(s);
+    for (auto& t : s.tracks)
+    {
+        if (t.post_action != pd.discrete_action || t.status == inactive)
+            continue;
+        t.interaction_mfp = 0;
+        std::uint32_t const mat = material_of(s, t);
+        auto const& procs = pd.particle_processes[t.particle];
+        // Selector: accumulate until positive (random/Selector.hh)
+        double accum = -t.macro_xs * canonical_double(t.rng);
+        std::size_t pp = procs.size() - 1;
+        for (std::size_t i = 0; i + 1 < procs.size(); ++i)
+        {
+            accum += t.per_process_xs[i];
+            if (accum > 0)
+            {
+                pp = i;
+                break;
+            }
+        }
+        ProcessDef const& p = pd.processes[procs[pp]];
+        if (p.integral)
+        {
+            double xs = calc_xs(s, p, mat, t.energy);
+            if (canonical_double(t.rng) * t.per_process_xs[pp] > xs)
+            {
+                t.post_action = pd.integral_rejection_action;
+                continue;
+            }
+        }
+        // Model finder: last model whose lower bound is <= energy
+        std::size_t mi = 0;
+        for (std::size_t k = 1; k < p.models.size(); ++k)
+            if (t.energy >= p.models[k].low)
+                mi = k;
+        t.element = sample_element(s, t, p, mi, mat);
+        t.post_action = p.models[mi].action;
+    }
+}
+
+//---------------------------------------------------------------------------//
+// INTERACTION (InteractionApplier.hh:106-171)
+//---------------------------------------------------------------------------//
+void interact(State& s, std::uint32_t model_action)
+{
+    PhysicsData const& pd = phys(s);
+    ModelDef const* model = nullptr;
+    for (auto const& p : pd.processes)
+        for (auto const& m : p.models)
+            if (m.action == model_action)
+                model = &m;
+    if (!model)
+        throw std::runtime_error("oracle: no model for action");
+    for (auto& t : s.tracks)
+    {
+        if (t.post_action != model_action || t.status == inactive)
+            continue;
+        Interaction r = sample_interaction(s, t, *model);
+        if (r.action == Interaction::failed)
+        {
+            // step_limit({0, failure}): only applies if limiting
+            if (0 < t.step_length)
+            {
+                t.step_length = 0;
+                t.post_action = pd.failure_action;
+            }
+            continue;
+        }
+        if (r.action == Interaction::unchanged)
+            continue;
+        t.energy = r.energy;
+        if (r.action != Interaction::absorbed)
+            geo_set_dir(s.problem->geo, t.geo, r.dir);
+        else
+            t.status = killed;
+        double deposition = r.energy_deposition;
+        std::uint32_t const mat = material_of(s, t);
+        for (std::uint32_t k = 0; k < r.num_secondaries; ++k)
+        {
+            SecondaryP& sec = s.stack[r.sec_start + k];
+            if (pd.apply_post_interaction_cut && sec.particle != none
+                && sec.energy < pd.cutoff_energy[sec.particle][mat])
+            {
+                deposition += sec.energy;
+                if (sec.particle == pd.positron)
+                    deposition += 2 * pd.particles[sec.particle].mass;
+                sec = SecondaryP{};
+            }
+        }
+        t.edep += deposition;
+        t.sec_start = r.sec_start;
+        t.sec_count = r.num_secondaries;
+    }
+}
+
+//---------------------------------------------------------------------------//
+// BOUNDARY (BoundaryExecutor.hh:42-84) AND TRACKING CUT (:49-84)
+//---------------------------------------------------------------------------//
+void boundary(State& s)
+{
+    ActionIds const& ids = s.problem->actions;
+    for (auto& t : s.tracks)
+    {
+        if (t.post_action != ids.boundary || t.status != alive)
+            continue;
+        geo_cross_boundary(s.problem->geo, t.geo);
+        if (t.geo.failed)
+        {
+            t.status = errored;
+            t.along_action = none;
+            t.post_action = ids.tracking_cut;
+        }
+        else if (!geo_is_outside(t.geo))
+        {
+            if (material_of(s, t) == none)
+            {
+                t.status = errored;
+                t.along_action = none;
+                t.post_action = ids.tracking_cut;
+            }
+        }
+        else
+        {
+            t.status = killed;
+        }
+    }
+}
+
+void tracking_cut(State& s)
+{
+    PhysicsData const& pd = phys(s);
+    ActionIds const& ids = s.problem->actions;
+    for (auto& t : s.tracks)
+    {
+        if (t.post_action != ids.tracking_cut || t.status == inactive)
+            continue;
+        double dep = t.energy;
+        if (t.particle == pd.positron)
+            dep += 2 * pd.particles[t.particle].mass;
+        t.edep += dep;
+        t.energy = 0;
+        t.status = killed;
+    }
+}
+
+//---------------------------------------------------------------------------//
+void along_step_charged(State& s)
+{
+    std::uint32_t const aid = s.problem->actions.along_user;
+    for (auto& t : s.tracks)
+    {
+        if (t.status == inactive || t.along_action != aid)
+            continue;
+        throw std::runtime_error("oracle: charged along-step not available");
+    }
+}
+
+void calo_tally(State& s)
+{
+    Problem const& prob = *s.problem;
+    if (prob.num_detectors == 0)
+        return;
+    for (auto const& t : s.tracks)
+    {
+        if (t.status == inactive || t.edep == 0)
+            continue;
+        std::uint32_t det = prob.volume_to_detector[t.geo.vol];
+        if (det != none)
+            s.calo_edep[det] += t.edep;
     }
 }
 

@@ -3,7 +3,17 @@
 // cpu_baseline leg of bench.py. Never linked into the product library.
 //---------------------------------------------------------------------------//
 //! \file oracle/physics.hh
-//! Physics data and per-step physics actions of the oracle.
+//! Physics data and the per-step physics actions of the oracle, restated
+//! from celeritas/phys/detail/{PreStep,DiscreteSelect,TrackingCut}Executor.hh,
+//! celeritas/phys/PhysicsStepUtils.hh, celeritas/phys/PhysicsTrackView.hh,
+//! celeritas/grid/XsCalculator.hh, celeritas/phys/InteractionApplier.hh,
+//! celeritas/geo/detail/BoundaryExecutor.hh and the along-step appliers in
+//! celeritas/global/alongstep/detail/*.hh.
+//!
+//! Pinned by: XsCalculator.test.cc:38-60 (closed-form grids) and the
+//! SimpleCompton whole-loop anchor (Stepper.test.cc:194-225: 919 step
+//! iterations, 53.8125 steps per primary, queue high-water mark {1, 6},
+//! emptying step 3).
 //---------------------------------------------------------------------------//
 #pragma once
 
@@ -18,6 +28,20 @@ namespace orc
 struct ActionIds;
 
 //---------------------------------------------------------------------------//
+//! Uniform-in-log(E) grid; values at i >= prime are scaled by E
+struct XsGridO
+{
+    double front = 0, back = 0, delta = 0;
+    std::uint32_t size = 0;
+    std::uint32_t prime = 0xffffffffu;
+    std::vector<double> values;
+
+    explicit operator bool() const { return size >= 2; }
+    double calc(double energy) const;  // XsCalculator::operator()
+    double at(std::uint32_t i) const;  // XsCalculator::operator[]
+    double energy(std::uint32_t i) const;
+};
+
 struct ParticleDef
 {
     std::string label;
@@ -26,20 +50,66 @@ struct ParticleDef
     double charge = 0;
 };
 
+struct ElementDef
+{
+    int z = 0;
+    double atomic_mass = 0;  // [amu]
+};
+
+struct MaterialDef
+{
+    std::string label;
+    double number_density = 0;  // [1/cm^3]
+    double electron_density = 0;  // [1/cm^3]
+    double mean_exc_energy = 0;  // [MeV]
+    std::vector<std::pair<std::uint32_t, double>> elements;  // (id, fraction)
+};
+
+struct ModelDef
+{
+    std::string label;
+    std::string model;  // klein_nishina, ...
+    double low = 0, high = 0;
+    std::uint32_t action = none;
+};
+
 struct ProcessDef
 {
-    std::uint32_t particle = none;
     std::string label;
+    std::string cls;
+    std::uint32_t particle = none;
+    bool integral = false;
+    std::vector<XsGridO> xs;  // per material
+    std::vector<double> energy_max_xs;  // per material (integral)
+    std::vector<XsGridO> dedx, range;  // per material (energy loss)
+    std::vector<ModelDef> models;
 };
 
 struct PhysicsData
 {
     std::vector<ParticleDef> particles;
+    std::vector<ElementDef> elements;
+    std::vector<MaterialDef> materials;
     std::vector<ProcessDef> processes;
+    std::vector<std::vector<std::uint32_t>> particle_processes;  // per particle
+    std::vector<int> eloss_process;  // per particle: index into its list
     std::vector<std::string> model_labels;
+    // cutoffs[particle][material]
+    std::vector<std::vector<double>> cutoff_energy;
+    // Options (PhysicsParamsOptions defaults)
     double secondary_stack_factor = 3;
-    // Action IDs resolved after registration
-    std::uint32_t pre_step_action = none;
+    double min_range = 0.1;
+    double max_step_over_range = 0.2;
+    double min_eprime_over_e = 0.8;
+    double linear_loss_limit = 0.01;
+    double lowest_electron_energy = 0.001;
+    double fixed_step_limiter = 0;
+    bool apply_post_interaction_cut = false;
+    // Resolved action IDs
+    std::uint32_t discrete_action = none, range_action = none;
+    std::uint32_t integral_rejection_action = none, failure_action = none;
+    std::uint32_t fixed_step_action = none;
+    std::uint32_t gamma = none, electron = none, positron = none;
 
     void finalize_actions(ActionIds const& ids);
 };
@@ -47,8 +117,20 @@ struct PhysicsData
 //! Parse the "physics" section (same schema as the product)
 void build_physics(Json const& inp, PhysicsData& out);
 
-//! PreStepExecutor (phys/detail/PreStepExecutor.hh:46-115)
+// Step actions (each loops over all slots, applying to matching tracks)
 void pre_step(State& s);
+void along_step_neutral(State& s);
+void discrete_select(State& s);
+void interact(State& s, std::uint32_t model_action);
+void boundary(State& s);
+void tracking_cut(State& s);
+void along_step_charged(State& s);
+//! SimpleCalo tally of per-volume energy deposition (user/detail/
+//! SimpleCaloExecutor.hh:48-66, StepGatherExecutor.hh:41-153)
+void calo_tally(State& s);
+
+//! Steps of the full sequence in execution order
+void step(State& s);
 
 //---------------------------------------------------------------------------//
 }  // namespace orc

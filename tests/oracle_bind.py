@@ -219,3 +219,33 @@ def state_initializers(state: "State") -> np.ndarray:
     f.restype = ctypes.c_uint32
     f(state._h, out.ctypes.data, len(out))
     return out[:n]
+
+
+def run(state: "State", primaries: np.ndarray, max_iters: int = 1_000_000):
+    """StepperTestBase::run on the oracle: (active, queued) per step."""
+    from celeritas_amd import PRIMARY_DTYPE
+
+    p = np.ascontiguousarray(primaries, dtype=PRIMARY_DTYPE)
+    act = np.zeros(max_iters, np.uint32)
+    q = np.zeros(max_iters, np.uint32)
+    f = load().orc_run
+    f.restype = ctypes.c_uint32
+    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
+                  ctypes.c_void_p, ctypes.c_void_p]
+    n = f(state._h, p.ctypes.data, len(p), max_iters, act.ctypes.data, q.ctypes.data)
+    if n == 0:
+        raise OracleError(load().orc_last_error().decode())
+    return act[:n].copy(), q[:n].copy()
+
+
+def emptying_step(active):
+    """StepperTestBase::RunResult::calc_emptying_step (StepperTestBase.cc:120-145)."""
+    result = 0
+    prev = int(active[0])
+    max_cap = 0
+    for i in range(1, len(active)):
+        max_cap = max(prev, max_cap)
+        if prev == max_cap and int(active[i]) < max_cap:
+            result = i
+        prev = int(active[i])
+    return result

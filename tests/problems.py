@@ -31,3 +31,51 @@ def two_boxes_track_init(stack_factor=8.0):
         "track_init": {"capacity": 4096, "max_events": 4096, "track_order": "none"},
         "seed": 20220511,
     }
+
+
+def simple_compton():
+    """SimpleComptonTest: two boxes, Compton-only gamma physics with fake
+    lambda tables (test/celeritas/SimpleTestBase.cc:32-200,
+    test/celeritas/global/Stepper.test.cc:46-68)."""
+    return {
+        "geometry": load_geometry("two-boxes.org.json"),
+        "volume_to_material": [-1, 0, 1],
+        "physics": {
+            "particles": [GAMMA, ELECTRON_HALF],
+            "elements": [{"z": 13, "atomic_mass": 27.0}],
+            "materials": [
+                {"label": "Al", "elements": [{"element": 0, "fraction": 1.0}]},
+                {"label": "hard vacuum", "elements": []},
+            ],
+            "cutoffs": {
+                "gamma": [{"energy": 0.01, "range": 0.01}, {"energy": 100, "range": 100}],
+                "electron": [{"energy": 1000, "range": 1000}, {"energy": 1000, "range": 1000}],
+            },
+            "options": {"secondary_stack_factor": 3.0},
+            "processes": [
+                {
+                    "label": "Compton scattering",
+                    "particle": "gamma",
+                    "class": "compton",
+                    "integral": False,
+                    "models": [{"label": "scat-klein-nishina", "model": "klein_nishina",
+                                "low": 1e-4, "high": 1e8}],
+                    "lambda": [{"x": [1e-4, 1.0], "y": [1e1, 1e0]},
+                               {"x": [1e-4, 1.0], "y": [1e-10, 1e-10]}],
+                    "lambda_prim": [{"x": [1.0, 1e4, 1e8], "y": [1e0, 1e-2, 1e-4]},
+                                    {"x": [1.0, 1e4, 1e8], "y": [1e-10, 1e-10, 1e-10]}],
+                }
+            ],
+        },
+        "along_step": "neutral",
+        "track_init": {"capacity": 4096, "max_events": 4096, "track_order": "none"},
+        "seed": 20220511,
+    }
+
+
+def simple_compton_primaries(n):
+    import celeritas_amd
+
+    return celeritas_amd.make_primaries(
+        {"particle_id": 0, "energy": 100.0, "pos": [-22.0, 0, 0], "dir": [1.0, 0, 0]}
+        for _ in range(n))

@@ -151,9 +151,14 @@ def test_random_compaction_matches_oracle(amd, num_slots):
         oi = ob.state_initializers(ost)
         for field in ("event_id", "particle_id", "time", "pos", "dir", "energy"):
             np.testing.assert_array_equal(gi[field], oi[field], err_msg=field)
-        # Atomic track IDs: compare as multisets (the reference's own choice)
-        assert sorted(gi["track_id"].tolist()) == sorted(oi["track_id"].tolist())
-        assert sorted(gi["parent_id"].tolist()) == sorted(oi["parent_id"].tolist())
+        # Atomic track IDs (the reference's own choice, track/detail/Utils.hh:
+        # 100-117): which ID lands in a refilled slot versus the queue varies,
+        # so compare the multiset over slots and queue together.
+        ids = sorted(gi["track_id"].tolist() + st.get("track_id").tolist())
+        oids = sorted(oi["track_id"].tolist() + ost.get_u32("track_id").tolist())
+        assert ids == oids
+        pids = sorted(gi["parent_id"].tolist() + st.get("parent_id").tolist())
+        opids = sorted(oi["parent_id"].tolist() + ost.get_u32("parent_id").tolist())
+        assert pids == opids
         np.testing.assert_array_equal(st.get("status"), ost.get_u32("status"))
         np.testing.assert_array_equal(st.get("energy"), ost.get_f64("energy"))
-        assert sorted(st.get("track_id").tolist()) == sorted(ost.get_u32("track_id").tolist())
