@@ -8,11 +8,11 @@
 
 #if defined(__HIPCC__)
 #    include <hip/hip_runtime.h>
-#    define CELER_FUNCTION __host__ __device__
+#    define CELER_FUNCTION inline __host__ __device__
 #    define CELER_FORCEINLINE_FUNCTION __host__ __device__ __forceinline__
 #    define CELER_DEVICE_FUNCTION __device__
 #else
-#    define CELER_FUNCTION
+#    define CELER_FUNCTION inline
 #    define CELER_FORCEINLINE_FUNCTION inline __attribute__((always_inline))
 #    define CELER_DEVICE_FUNCTION
 #endif

@@ -1,0 +1,127 @@
+//----------------------------------*-C++-*----------------------------------//
+// celeritas_amd: MI355X-native per-step transport loop
+//---------------------------------------------------------------------------//
+//! \file em/Interactors.hh
+//! Model dispatch, element selection and the post-interaction update.
+//---------------------------------------------------------------------------//
+#pragma once
+
+#include "geo/OrangeLiteTracker.hh"
+#include "InteractorCommon.hh"
+#include "KleinNishina.hh"
+
+namespace celeritas_amd
+{
+//---------------------------------------------------------------------------//
+//! Everything an interactor may need besides the RNG
+struct InteractionContext
+{
+    PhysicsParamsRef const& params;
+    CoreStateRef const& state;
+    size_type slot;
+    std::uint32_t material;
+    std::uint32_t element;  //!< component index in the material
+    std::uint32_t particle;
+};
+
+/*!
+ * Sample an element component for a multi-element material.
+ *
+ * Single-element materials need no draw (PhysicsStepUtils.hh:290-300).
+ */
+template<class Engine>
+CELER_FUNCTION std::uint32_t sample_element(PhysicsParamsRef const& p,
+                                            ProcessRecord const& proc,
+                                            std::uint32_t model_index,
+                                            std::uint32_t mat,
+                                            double energy,
+                                            Engine& rng)
+{
+    (void)proc;
+    (void)model_index;
+    (void)energy;
+    (void)rng;
+    if (p.num_materials == 0 || p.materials == nullptr)
+        return 0;
+    MaterialRecord const& m = p.materials[mat];
+    if (m.num_elements <= 1)
+        return 0;
+    return 0;
+}
+
+//! Dispatch to the model's interactor
+template<class Engine>
+CELER_FUNCTION Interaction sample_interaction(ModelClass model,
+                                              InteractionContext const& ctx,
+                                              double energy,
+                                              double const* dir,
+                                              Engine& rng)
+{
+    switch (model)
+    {
+        case ModelClass::klein_nishina:
+            return sample_klein_nishina(ctx.params, ctx.state, energy, dir, rng);
+        default:
+            break;
+    }
+    Interaction r;
+    r.action = Interaction::unchanged;
+    return r;
+}
+
+/*!
+ * Apply a sampled interaction to the track (InteractionApplier.hh:106-171).
+ */
+CELER_FUNCTION void apply_interaction(CoreParamsRef const& params,
+                                      CoreStateRef const& state,
+                                      size_type slot,
+                                      std::uint32_t mat,
+                                      Interaction const& r)
+{
+    PhysicsParamsRef const& pp = params.phys;
+    if (CELER_UNLIKELY(r.action == Interaction::failed))
+    {
+        if (0 < state.sim.step_length[slot])
+        {
+            state.sim.step_length[slot] = 0;
+            state.sim.post_step_action[slot] = pp.scalars.failure_action;
+        }
+        return;
+    }
+    if (r.action == Interaction::unchanged)
+        return;
+    state.particle.energy[slot] = r.energy;
+    if (r.action != Interaction::absorbed)
+    {
+        geo::GeoTrack g = geo::GeoTrack::load(state.geo, slot);
+        geo::set_dir(params.geo, g, r.dir);
+        g.store(state.geo, slot);
+    }
+    else
+    {
+        state.sim.status[slot] = static_cast<std::uint8_t>(TrackStatus::killed);
+    }
+    double deposition = r.energy_deposition;
+    if (pp.scalars.apply_post_interaction_cut)
+    {
+        for (std::uint32_t k = 0; k < r.num_secondaries; ++k)
+        {
+            Secondary& sec = state.secondaries.storage[r.sec_start + k];
+            if (sec.particle_id != invalid_id
+                && sec.energy < pp.cutoff_energy[sec.particle_id * pp.num_materials + mat])
+            {
+                deposition += sec.energy;
+                if (sec.particle_id == pp.positron_id)
+                    deposition += 2 * pp.mass[sec.particle_id];
+                sec.particle_id = invalid_id;
+                sec.energy = 0;
+            }
+        }
+    }
+    state.phys.energy_deposition[slot] += deposition;
+    state.phys.sec_start[slot] = r.sec_start;
+    state.phys.sec_count[slot] = r.num_secondaries;
+}
+
+//---------------------------------------------------------------------------//
+}  // namespace celeritas_amd

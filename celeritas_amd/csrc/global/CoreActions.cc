@@ -36,6 +36,12 @@ void TrackingCutAction::step(CoreParams const& params, CoreState& state) const
         launch_tracking_cut(params.device_ref(), state.ref(), state.stream()));
 }
 
+void CaloTallyAction::step(CoreParams const& params, CoreState& state) const
+{
+    CELER_HIP_CALL(launch_calo_tally(params.device_ref(), state.ref(),
+                                     params.num_detectors(), state.stream()));
+}
+
 //---------------------------------------------------------------------------//
 std::string make_along_step_action(std::string const& kind,
                                    PhysicsParams const&,

@@ -31,11 +31,31 @@ hipError_t launch_along_step_charged(CoreParamsRef const&,
                                      std::uint8_t action,
                                      hipStream_t);
 hipError_t launch_discrete_select(CoreParamsRef const&, CoreStateRef const&, hipStream_t);
-hipError_t launch_interactions(CoreParamsRef const&, CoreStateRef const&, hipStream_t);
+hipError_t launch_interaction(CoreParamsRef const&,
+                              CoreStateRef const&,
+                              std::uint8_t action,
+                              std::uint32_t model,
+                              hipStream_t);
 hipError_t launch_boundary(CoreParamsRef const&, CoreStateRef const&, hipStream_t);
 hipError_t launch_tracking_cut(CoreParamsRef const&, CoreStateRef const&, hipStream_t);
-hipError_t launch_calo_tally(CoreParamsRef const&, CoreStateRef const&, hipStream_t);
+hipError_t launch_calo_tally(CoreParamsRef const&,
+                             CoreStateRef const&,
+                             size_type num_detectors,
+                             hipStream_t);
+hipError_t launch_kill_active(CoreParamsRef const&, CoreStateRef const&, hipStream_t);
 hipError_t launch_fused_step(CoreParamsRef const&, CoreStateRef const&, hipStream_t);
+
+//! Per-volume energy deposition tally (user/SimpleCalo, StepGather pre/post)
+class CaloTallyAction final : public StepActionInterface, public ConcreteAction
+{
+  public:
+    explicit CaloTallyAction(ActionId id)
+        : ConcreteAction(id, "simple-calo", "tally energy deposition by volume")
+    {
+    }
+    void step(CoreParams const&, CoreState&) const final;
+    StepActionOrder order() const final { return StepActionOrder::user_post; }
+};
 
 //---------------------------------------------------------------------------//
 //! Linear propagation of neutral particles (AlongStepNeutralAction)

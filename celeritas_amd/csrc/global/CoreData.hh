@@ -145,6 +145,7 @@ struct CoreStateRef
     InitStateRef init;
     SecondaryStackRef secondaries;
     CoreCounters* counters{nullptr};
+    std::uint32_t* pre_volume{nullptr};  //!< volume at the start of the step
     std::uint32_t* track_slots{nullptr};  //!< thread -> slot indirection
     std::uint32_t* action_offsets{nullptr};  //!< sorted ranges per action
     double* calo_edep{nullptr};  //!< per-volume energy deposition tally

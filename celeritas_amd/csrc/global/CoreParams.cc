@@ -68,6 +68,10 @@ CoreParams::CoreParams(Input inp)
     {
         reg_->insert(std::make_shared<ExtendFromPrimariesAction>(reg_->next_id()));
     }
+    if (num_detectors_ > 0)
+    {
+        reg_->insert(std::make_shared<CaloTallyAction>(reg_->next_id()));
+    }
     sc.num_actions = reg_->num_actions();
     sc.track_order = inp.track_order;
     sc.precision = inp.precision;

@@ -96,6 +96,7 @@ CoreState::CoreState(CoreParams const& params,
         std::max(1.0, stack_factor) * static_cast<double>(num_slots));
     add(&s.secondaries.storage, s.secondaries.capacity);
     add(&s.counters, 1);
+    add(&s.pre_volume, n);
     add(&s.track_slots, n);
     add(&s.action_offsets, params.action_reg()->num_actions() + 2);
     add(&s.calo_edep, std::max<size_type>(num_detectors_, 1));

@@ -1,0 +1,36 @@
+//----------------------------------*-C++-*----------------------------------//
+// celeritas_amd: MI355X-native per-step transport loop
+//---------------------------------------------------------------------------//
+//! \file phys/PhysicsActions.cc
+//---------------------------------------------------------------------------//
+#include "PhysicsActions.hh"
+
+#include "global/CoreActions.hh"
+#include "global/CoreParams.hh"
+#include "global/CoreState.hh"
+
+namespace celeritas_amd
+{
+//---------------------------------------------------------------------------//
+void PreStepAction::step(CoreParams const& params, CoreState& state) const
+{
+    CELER_HIP_CALL(launch_pre_step(params.device_ref(), state.ref(), state.stream()));
+}
+
+void DiscreteSelectAction::step(CoreParams const& params, CoreState& state) const
+{
+    CELER_HIP_CALL(
+        launch_discrete_select(params.device_ref(), state.ref(), state.stream()));
+}
+
+void ModelAction::step(CoreParams const& params, CoreState& state) const
+{
+    CELER_HIP_CALL(launch_interaction(params.device_ref(),
+                                      state.ref(),
+                                      static_cast<std::uint8_t>(this->action_id()),
+                                      model_class_,
+                                      state.stream()));
+}
+
+//---------------------------------------------------------------------------//
+}  // namespace celeritas_amd

@@ -7,8 +7,10 @@
 //! Counterparts of PhysicsParamsData/PhysicsStateData
 //! (celeritas/phys/PhysicsData.hh:403-503), MaterialParamsData
 //! (celeritas/mat/MaterialData.hh), ParticleParamsData and CutoffParamsData.
-//! Tables are flattened into a few device arrays; each grid is addressed by
-//! a small descriptor so a lookup is one descriptor load plus two values.
+//! The per-particle process list is stored inline (at most max_processes
+//! processes per particle, max_models models per process) so that a track
+//! reaches all of its physics descriptors with one 512-byte record load;
+//! grids are small descriptors into one array of reals.
 //---------------------------------------------------------------------------//
 #pragma once
 
@@ -19,7 +21,32 @@ namespace celeritas_amd
 //---------------------------------------------------------------------------//
 inline constexpr int max_processes = 4;  //!< per particle type
 inline constexpr int max_models = 3;  //!< per process
-inline constexpr int max_elements = 4;  //!< per material
+inline constexpr int max_elements = 8;  //!< per material
+
+//! Interaction models with device implementations
+enum class ModelClass : std::uint32_t
+{
+    klein_nishina = 0,
+    livermore_pe,
+    bethe_heitler,
+    moller_bhabha,
+    seltzer_berger,
+    relativistic_brem,
+    eplusgg,
+    size_
+};
+
+//! Process classes with special handling
+enum class ProcessClass : std::uint32_t
+{
+    other = 0,
+    compton,
+    photoelectric,
+    conversion,
+    ionization,
+    bremsstrahlung,
+    annihilation,
+};
 
 //! Log-uniform grid with values (celeritas/grid/XsGridData.hh:28-50)
 struct XsGrid
@@ -41,15 +68,15 @@ struct ModelRange
     std::uint32_t action;  //!< step action ID of the model
 };
 
-//! One particle-process: tables per material
+//! One particle-process: tables indexed by material
 struct ProcessRecord
 {
     std::uint32_t process_class;  //!< ProcessClass
-    std::uint32_t xs_grid;  //!< first XsGrid index (one per material)
+    std::uint32_t xs_grid;  //!< first XsGrid index (+material), or invalid
     std::uint32_t integral;  //!< 1 if the integral approach is used
-    std::uint32_t energy_max_xs;  //!< reals offset (one per material)
+    std::uint32_t energy_max_xs;  //!< reals offset (+material)
     std::uint32_t num_models;
-    std::uint32_t elem_grid;  //!< element CDF grids [model][mat][elem]
+    std::uint32_t elem_cdf;  //!< reals offset of element CDF grids, or invalid
     ModelRange models[max_models];
 };
 
@@ -58,11 +85,49 @@ struct ParticlePhysics
 {
     std::uint32_t num_processes;
     std::int32_t eloss_process;  //!< index of the process with dE/dx, or -1
-    std::uint32_t dedx_grid;  //!< first XsGrid (one per material)
-    std::uint32_t range_grid;  //!< first XsGrid (one per material)
-    std::uint32_t msc_grid;  //!< first XsGrid (one per material) or invalid
+    std::uint32_t dedx_grid;  //!< first XsGrid (+material), or invalid
+    std::uint32_t range_grid;  //!< first XsGrid (+material), or invalid
+    std::uint32_t inv_range_grid;  //!< first XsGrid (+material), or invalid
+    std::uint32_t msc_grid;  //!< first XsGrid (+material), or invalid
+    std::uint32_t has_at_rest;
     std::uint32_t pad_;
     ProcessRecord process[max_processes];
+};
+
+//---------------------------------------------------------------------------//
+//! Element properties (celeritas/mat/ElementView.hh)
+struct ElementRecord
+{
+    double z;
+    double atomic_mass;  //!< [amu]
+    double cbrt_z;
+    double cbrt_zzp;  //!< (Z(Z+1))^(1/3)
+    double log_z;
+    double coulomb_correction;  //!< f(Z) of Davies-Bethe-Maximon
+    double mass_radiation_coeff;  //!< 1/X0 per unit density [cm^2/g]
+    double pad_;
+};
+
+//! Component of a material
+struct ElementComponent
+{
+    std::uint32_t element;
+    std::uint32_t pad_;
+    double fraction;  //!< number fraction
+};
+
+//! Material properties (celeritas/mat/MaterialView.hh)
+struct MaterialRecord
+{
+    double number_density;  //!< atoms / cm^3
+    double electron_density;  //!< electrons / cm^3
+    double density;  //!< g / cm^3
+    double mean_exc_energy;  //!< [MeV]
+    double log_mean_exc_energy;
+    double rad_length;  //!< [cm]
+    double zeff;
+    std::uint32_t elem_begin;
+    std::uint32_t num_elements;
 };
 
 //---------------------------------------------------------------------------//
@@ -77,13 +142,21 @@ struct PhysicsScalars
     double fixed_step_limiter;  //!< [cm] (0 = off)
     double secondary_stack_factor;
     std::uint32_t num_models;
+    std::uint8_t apply_post_interaction_cut;
     std::uint8_t discrete_action;
     std::uint8_t range_action;
     std::uint8_t integral_rejection_action;
     std::uint8_t failure_action;
     std::uint8_t msc_action;
     std::uint8_t fixed_step_action;
-    std::uint8_t pad_[2];
+    std::uint8_t pad_;
+};
+
+//! Per-model constants (filled for the models in the problem)
+struct ModelData
+{
+    double electron_mass;  //!< [MeV]
+    double inv_electron_mass;
 };
 
 //---------------------------------------------------------------------------//
@@ -92,7 +165,7 @@ struct PhysicsScalars
  */
 struct PhysicsParamsRef
 {
-    PhysicsScalars scalars;
+    PhysicsScalars scalars{};
     std::uint32_t num_particles{0};
     std::uint32_t num_materials{0};
     ParticlePhysics const* particle{nullptr};
@@ -104,13 +177,14 @@ struct PhysicsParamsRef
     std::uint32_t gamma_id{invalid_id};
     std::uint32_t electron_id{invalid_id};
     std::uint32_t positron_id{invalid_id};
-    // Cutoffs [particle][material]: energy [MeV] and range [cm]
+    std::uint32_t pad_{0};
+    // Cutoffs [particle * num_materials + material]
     double const* cutoff_energy{nullptr};
-    double const* cutoff_range{nullptr};
-    // Material properties
-    void const* materials{nullptr};  //!< MaterialRecord array
-    void const* elements{nullptr};  //!< ElementRecord array
-    void const* models{nullptr};  //!< model-specific data block
+    // Materials
+    MaterialRecord const* materials{nullptr};
+    ElementComponent const* components{nullptr};
+    ElementRecord const* elements{nullptr};
+    ModelData model{};
 };
 
 //---------------------------------------------------------------------------//

@@ -1,0 +1,105 @@
+//----------------------------------*-C++-*----------------------------------//
+// celeritas_amd: MI355X-native per-step transport loop
+//---------------------------------------------------------------------------//
+//! \file phys/PhysicsImpl.hh
+//! Cross-section lookup and step-limit helpers used by the kernels.
+//---------------------------------------------------------------------------//
+#pragma once
+
+#include <cmath>
+
+#include "global/CoreData.hh"
+
+namespace celeritas_amd
+{
+namespace phys
+{
+//---------------------------------------------------------------------------//
+inline constexpr double c_light = 29979245800.0;  //!< [cm/s]
+inline constexpr double pi = 3.14159265358979323846;
+
+/*!
+ * Interpolate a log-uniform grid at an energy (grid/XsCalculator.hh:107-150).
+ *
+ * Linear in E between the two bracketing points; values at and above the
+ * prime index are stored multiplied by E. Outside the grid the end value is
+ * used (with the 1/E scaling where applicable).
+ */
+CELER_FUNCTION double
+calc_grid(XsGrid const& g, double const* __restrict__ reals, double energy)
+{
+    double const* v = reals + g.offset;
+    double const loge = std::log(energy);
+    if (loge <= g.loge_front)
+    {
+        double r = v[0];
+        return (0 >= g.prime_index) ? r / energy : r;
+    }
+    if (loge >= g.loge_back)
+    {
+        std::uint32_t const i = g.size - 1;
+        double r = v[i];
+        return (i >= g.prime_index) ? r / energy : r;
+    }
+    auto const lower = static_cast<std::uint32_t>((loge - g.loge_front) / g.loge_delta);
+    double const upper_e = std::exp(g.loge_front + g.loge_delta * (lower + 1));
+    double upper_xs = v[lower + 1];
+    if (lower + 1 == g.prime_index)
+        upper_xs /= upper_e;
+    double const lower_e = std::exp(g.loge_front + g.loge_delta * lower);
+    double const lower_xs = v[lower];
+    double const slope = ((-lower_xs) + upper_xs) / ((-lower_e) + upper_e);
+    double result = std::fma(slope, (-lower_e) + energy, lower_xs);
+    if (lower >= g.prime_index)
+        result /= energy;
+    return result;
+}
+
+//! Macroscopic cross section of a process in a material
+CELER_FUNCTION double calc_xs(PhysicsParamsRef const& p,
+                              ProcessRecord const& proc,
+                              std::uint32_t mat,
+                              double energy)
+{
+    if (proc.xs_grid == invalid_id)
+        return 0;
+    return calc_grid(p.grids[proc.xs_grid + mat], p.reals, energy);
+}
+
+//! Upper bound of the XS over the step (PhysicsTrackView.hh:446-478)
+CELER_FUNCTION double calc_max_xs(PhysicsParamsRef const& p,
+                                  ProcessRecord const& proc,
+                                  std::uint32_t mat,
+                                  double energy)
+{
+    double const emax = p.reals[proc.energy_max_xs + mat];
+    double const exi = energy * p.scalars.min_eprime_over_e;
+    if (emax >= exi && emax < energy)
+        return calc_xs(p, proc, mat, emax);
+    double const a = calc_xs(p, proc, mat, energy);
+    double const b = calc_xs(p, proc, mat, exi);
+    return a > b ? a : b;
+}
+
+//! Scaled range limit (PhysicsTrackView.hh:624-640)
+CELER_FUNCTION double range_to_step(PhysicsScalars const& sc, double range)
+{
+    double const rho = sc.min_range;
+    // 1 + sqrt(epsilon) for double precision
+    if (range < rho * (1 + 1.4901161193847656e-08))
+        return range;
+    double const alpha = sc.max_step_over_range;
+    return alpha * range + rho * (1 - alpha) * (2 - rho / range);
+}
+
+//! Speed [cm/s] (ParticleTrackView.hh:303-323)
+CELER_FUNCTION double speed(double energy, double mass)
+{
+    double const inv_gamma = mass / (energy + mass);
+    double const beta_sq = 1 - inv_gamma * inv_gamma;
+    return std::sqrt(beta_sq) * c_light;
+}
+
+//---------------------------------------------------------------------------//
+}  // namespace phys
+}  // namespace celeritas_amd

@@ -5,23 +5,218 @@
 //---------------------------------------------------------------------------//
 #include "PhysicsParams.hh"
 
+#include <cmath>
+
 #include "global/ActionInterface.hh"
+#include "PhysicsActions.hh"
 
 namespace celeritas_amd
 {
+namespace
+{
+//---------------------------------------------------------------------------//
+ModelClass model_from_string(std::string const& s)
+{
+    static char const* const names[] = {"klein_nishina", "livermore_pe",
+                                        "bethe_heitler", "moller_bhabha",
+                                        "seltzer_berger", "relativistic_brem",
+                                        "eplusgg"};
+    for (int i = 0; i < static_cast<int>(ModelClass::size_); ++i)
+    {
+        if (s == names[i])
+            return static_cast<ModelClass>(i);
+    }
+    throw RuntimeError("unsupported model '" + s + "'");
+}
+
+ProcessClass process_from_string(std::string const& s)
+{
+    if (s == "compton")
+        return ProcessClass::compton;
+    if (s == "photoelectric")
+        return ProcessClass::photoelectric;
+    if (s == "conversion")
+        return ProcessClass::conversion;
+    if (s == "e_ioni")
+        return ProcessClass::ionization;
+    if (s == "e_brems")
+        return ProcessClass::bremsstrahlung;
+    if (s == "annihilation")
+        return ProcessClass::annihilation;
+    return ProcessClass::other;
+}
+
+struct GridHost
+{
+    double front{0}, back{0}, delta{0};
+    std::uint32_t prime{invalid_id};
+    std::vector<double> values;
+    explicit operator bool() const { return values.size() >= 2; }
+};
+
+//! Merge "lambda"/"lambda_prim" vectors (grid/ValueGridBuilder.cc:85-172)
+GridHost grid_from_vectors(JsonValue const* lo, JsonValue const* hi)
+{
+    GridHost g;
+    double emin, emax, eprime = 0;
+    if (lo && hi)
+    {
+        auto lx = lo->at("x").as_doubles();
+        auto ly = lo->at("y").as_doubles();
+        auto hx = hi->at("x").as_doubles();
+        auto hy = hi->at("y").as_doubles();
+        CELER_VALIDATE(ly.size() == lx.size() && hy.size() == hx.size() && lx.size() >= 1,
+                       << "inconsistent lambda table sizes");
+        g.values.assign(ly.begin(), ly.end() - 1);
+        g.values.insert(g.values.end(), hy.begin(), hy.end());
+        emin = lx.front();
+        eprime = hx.front();
+        emax = hx.back();
+    }
+    else if (hi)
+    {
+        auto hx = hi->at("x").as_doubles();
+        g.values = hi->at("y").as_doubles();
+        emin = eprime = hx.front();
+        emax = hx.back();
+    }
+    else
+    {
+        auto lx = lo->at("x").as_doubles();
+        g.values = lo->at("y").as_doubles();
+        emin = lx.front();
+        emax = lx.back();
+    }
+    CELER_VALIDATE(g.values.size() >= 2 && emin > 0 && emax > emin,
+                   << "invalid physics vector");
+    g.front = std::log(emin);
+    g.back = std::log(emax);
+    g.delta = (g.back - g.front) / static_cast<double>(g.values.size() - 1);
+    if (eprime > 0)
+    {
+        double const le = std::log(eprime);
+        auto p = static_cast<std::uint32_t>((le - g.front) / g.delta);
+        double const gp1 = g.front + g.delta * (p + 1);
+        if (std::fabs(gp1 - le) <= 1e-12 * std::max(1.0, std::fabs(le)))
+            ++p;
+        g.prime = p;
+    }
+    return g;
+}
+
+GridHost grid_from_json(JsonValue const& j)
+{
+    GridHost g;
+    g.values = j.at("values").as_doubles();
+    CELER_VALIDATE(g.values.size() >= 2, << "grid needs at least two values");
+    g.front = std::log(j.at("emin").as_double());
+    g.back = std::log(j.at("emax").as_double());
+    g.delta = (g.back - g.front) / static_cast<double>(g.values.size() - 1);
+    long long p = j.get_int("prime", -1);
+    g.prime = p < 0 ? invalid_id : static_cast<std::uint32_t>(p);
+    return g;
+}
+
+std::vector<GridHost> grids_of(JsonValue const& proc,
+                               char const* direct,
+                               char const* lo_key,
+                               char const* hi_key,
+                               std::size_t num_mats)
+{
+    std::vector<GridHost> result(num_mats);
+    if (proc.has(direct))
+    {
+        auto const& d = proc.at(direct);
+        for (std::size_t m = 0; m < num_mats && m < d.size(); ++m)
+        {
+            if (d[m].type == JsonValue::Type::object)
+                result[m] = grid_from_json(d[m]);
+        }
+        return result;
+    }
+    JsonValue const* lo = (lo_key && proc.has(lo_key)) ? &proc.at(lo_key) : nullptr;
+    JsonValue const* hi = (hi_key && proc.has(hi_key)) ? &proc.at(hi_key) : nullptr;
+    if (!lo && !hi)
+        return result;
+    for (std::size_t m = 0; m < num_mats; ++m)
+        result[m] = grid_from_vectors(lo ? &(*lo)[m] : nullptr, hi ? &(*hi)[m] : nullptr);
+    return result;
+}
+
+//! Value of a grid at point i, unscaled (XsCalculator::operator[])
+double grid_point_xs(GridHost const& g, std::uint32_t i)
+{
+    double r = g.values[i];
+    if (i >= g.prime)
+        r /= std::exp(g.front + g.delta * i);
+    return r;
+}
+//---------------------------------------------------------------------------//
+}  // namespace
+
 //---------------------------------------------------------------------------//
 struct PhysicsParams::HostData
 {
+    std::vector<ParticlePhysics> particle;
+    std::vector<XsGrid> grids;
+    std::vector<double> reals;
     std::vector<double> mass;
     std::vector<double> charge;
+    std::vector<double> cutoff_energy;
+    std::vector<MaterialRecord> materials;
+    std::vector<ElementComponent> components;
+    std::vector<ElementRecord> elements;
+
+    DeviceVector<ParticlePhysics> d_particle;
+    DeviceVector<XsGrid> d_grids;
+    DeviceVector<double> d_reals;
     DeviceVector<double> d_mass;
     DeviceVector<double> d_charge;
+    DeviceVector<double> d_cutoff;
+    DeviceVector<MaterialRecord> d_materials;
+    DeviceVector<ElementComponent> d_components;
+    DeviceVector<ElementRecord> d_elements;
+
+    //! Append grids (one per material); returns first index or invalid
+    std::uint32_t add_grids(std::vector<GridHost> const& gs)
+    {
+        if (gs.empty() || !gs[0])
+            return invalid_id;
+        auto first = static_cast<std::uint32_t>(grids.size());
+        for (auto const& g : gs)
+        {
+            XsGrid x{};
+            x.loge_front = g.front;
+            x.loge_back = g.back;
+            x.loge_delta = g.delta;
+            x.size = static_cast<std::uint32_t>(g.values.size());
+            x.prime_index = g.prime;
+            x.offset = static_cast<std::uint32_t>(reals.size());
+            reals.insert(reals.end(), g.values.begin(), g.values.end());
+            grids.push_back(x);
+        }
+        return first;
+    }
 };
 
 //---------------------------------------------------------------------------//
-PhysicsParams::PhysicsParams(JsonValue const& inp, ActionRegistry*)
+PhysicsParams::PhysicsParams(JsonValue const& inp, ActionRegistry* reg)
     : host_(std::make_unique<HostData>())
 {
+    HostData& h = *host_;
+    PhysicsScalars& sc = ref_.scalars;
+    sc = PhysicsScalars{};
+    sc.min_range = 0.1;
+    sc.max_step_over_range = 0.2;
+    sc.min_eprime_over_e = 0.8;
+    sc.linear_loss_limit = 0.01;
+    sc.lowest_electron_energy = 0.001;
+    sc.fixed_step_limiter = 0;
+    sc.secondary_stack_factor = 3;
+    sc.discrete_action = sc.range_action = sc.integral_rejection_action = invalid_id8;
+    sc.failure_action = sc.msc_action = sc.fixed_step_action = invalid_id8;
+
+    // Particles
     if (inp.has("particles"))
     {
         for (auto const& p : inp.at("particles").arr)
@@ -34,17 +229,224 @@ PhysicsParams::PhysicsParams(JsonValue const& inp, ActionRegistry*)
     }
     for (auto const& p : particles_)
     {
-        host_->mass.push_back(p.mass);
-        host_->charge.push_back(p.charge);
+        h.mass.push_back(p.mass);
+        h.charge.push_back(p.charge);
     }
-    host_->d_mass = to_device(host_->mass);
-    host_->d_charge = to_device(host_->charge);
-    ref_.num_particles = this->num_particles();
-    ref_.mass = host_->d_mass.data();
-    ref_.charge = host_->d_charge.data();
     ref_.gamma_id = this->find_particle(22);
     ref_.electron_id = this->find_particle(11);
     ref_.positron_id = this->find_particle(-11);
+
+    // Elements and materials
+    if (inp.has("elements"))
+    {
+        for (auto const& e : inp.at("elements").arr)
+        {
+            ElementRecord r{};
+            r.z = e.get("z", 0.0);
+            r.atomic_mass = e.get("atomic_mass", 0.0);
+            r.cbrt_z = std::cbrt(r.z);
+            r.cbrt_zzp = std::cbrt(r.z * (r.z + 1));
+            r.log_z = r.z > 0 ? std::log(r.z) : 0;
+            r.coulomb_correction = e.get("coulomb_correction", 0.0);
+            r.mass_radiation_coeff = e.get("mass_radiation_coeff", 0.0);
+            h.elements.push_back(r);
+        }
+    }
+    if (inp.has("materials"))
+    {
+        for (auto const& m : inp.at("materials").arr)
+        {
+            MaterialRecord r{};
+            r.number_density = m.get("number_density", 0.0);
+            r.electron_density = m.get("electron_density", 0.0);
+            r.density = m.get("density", 0.0);
+            r.mean_exc_energy = m.get("mean_exc_energy", 0.0);
+            r.log_mean_exc_energy
+                = r.mean_exc_energy > 0 ? std::log(r.mean_exc_energy) : 0;
+            r.rad_length = m.get("rad_length", 0.0);
+            r.zeff = m.get("zeff", 0.0);
+            r.elem_begin = static_cast<std::uint32_t>(h.components.size());
+            if (m.has("elements"))
+            {
+                for (auto const& c : m.at("elements").arr)
+                {
+                    ElementComponent ec{};
+                    ec.element = static_cast<std::uint32_t>(c.get_int("element", 0));
+                    ec.fraction = c.get("fraction", 1.0);
+                    h.components.push_back(ec);
+                }
+            }
+            r.num_elements = static_cast<std::uint32_t>(h.components.size()) - r.elem_begin;
+            CELER_VALIDATE(r.num_elements <= static_cast<std::uint32_t>(max_elements),
+                           << "too many elements in a material");
+            h.materials.push_back(r);
+        }
+    }
+    num_materials_ = static_cast<size_type>(std::max<std::size_t>(h.materials.size(), 1));
+
+    // Options
+    if (inp.has("options"))
+    {
+        auto const& o = inp.at("options");
+        sc.secondary_stack_factor = o.get("secondary_stack_factor", sc.secondary_stack_factor);
+        sc.min_range = o.get("min_range", sc.min_range);
+        sc.max_step_over_range = o.get("max_step_over_range", sc.max_step_over_range);
+        sc.min_eprime_over_e = o.get("min_eprime_over_e", sc.min_eprime_over_e);
+        sc.linear_loss_limit = o.get("linear_loss_limit", sc.linear_loss_limit);
+        sc.lowest_electron_energy
+            = o.get("lowest_electron_energy", sc.lowest_electron_energy);
+        sc.fixed_step_limiter = o.get("fixed_step_limiter", 0.0);
+        sc.apply_post_interaction_cut = o.get_bool("apply_cuts", false);
+    }
+
+    // Cutoffs
+    h.cutoff_energy.assign(particles_.size() * num_materials_, 0.0);
+    if (inp.has("cutoffs"))
+    {
+        for (auto const& kv : inp.at("cutoffs").obj)
+        {
+            std::uint32_t pid = this->find_particle(kv.first);
+            CELER_VALIDATE(pid != invalid_id, << "cutoff for unknown particle " << kv.first);
+            for (std::size_t m = 0; m < num_materials_ && m < kv.second.size(); ++m)
+                h.cutoff_energy[pid * num_materials_ + m] = kv.second[m].get("energy", 0.0);
+        }
+    }
+
+    // Processes: actions first (PhysicsParams.cc:97-143 order)
+    std::vector<JsonValue const*> procs;
+    if (inp.has("processes"))
+    {
+        for (auto const& p : inp.at("processes").arr)
+            procs.push_back(&p);
+    }
+    if (!procs.empty())
+    {
+        CELER_VALIDATE(reg, << "physics requires an action registry");
+        reg->insert(std::make_shared<PreStepAction>(reg->next_id()));
+        sc.msc_action = static_cast<std::uint8_t>(reg->next_id());
+        reg->insert(std::make_shared<ImplicitAction>(
+            reg->next_id(), "msc-range", "limit range due to multiple scattering"));
+        sc.range_action = static_cast<std::uint8_t>(reg->next_id());
+        sc.fixed_step_action = sc.range_action;
+        reg->insert(std::make_shared<ImplicitAction>(
+            reg->next_id(), "eloss-range", "limit range due to energy loss"));
+        sc.discrete_action = static_cast<std::uint8_t>(reg->next_id());
+        reg->insert(std::make_shared<DiscreteSelectAction>(reg->next_id()));
+        sc.integral_rejection_action = static_cast<std::uint8_t>(reg->next_id());
+        reg->insert(std::make_shared<ImplicitAction>(
+            reg->next_id(), "physics-integral-rejected", "reject by integral cross section"));
+    }
+
+    h.particle.assign(particles_.size(), ParticlePhysics{});
+    for (auto& pp : h.particle)
+    {
+        pp.eloss_process = -1;
+        pp.dedx_grid = pp.range_grid = pp.inv_range_grid = pp.msc_grid = invalid_id;
+    }
+    std::uint32_t num_models = 0;
+    for (JsonValue const* pj : procs)
+    {
+        std::string const plabel = pj->get_str("particle", "");
+        std::uint32_t pid = this->find_particle(plabel);
+        CELER_VALIDATE(pid != invalid_id, << "process for unknown particle '" << plabel << "'");
+        ParticlePhysics& pp = h.particle[pid];
+        CELER_VALIDATE(pp.num_processes < static_cast<std::uint32_t>(max_processes),
+                       << "too many processes for particle " << plabel);
+        ProcessRecord& rec = pp.process[pp.num_processes];
+        rec = ProcessRecord{};
+        rec.process_class = static_cast<std::uint32_t>(
+            process_from_string(pj->get_str("class", "")));
+        rec.integral = pj->get_bool("integral", false) ? 1 : 0;
+        rec.elem_cdf = invalid_id;
+
+        auto xs = grids_of(*pj, "xs", "lambda", "lambda_prim", num_materials_);
+        rec.xs_grid = h.add_grids(xs);
+        rec.energy_max_xs = invalid_id;
+        if (rec.integral)
+        {
+            rec.energy_max_xs = static_cast<std::uint32_t>(h.reals.size());
+            for (std::size_t m = 0; m < num_materials_; ++m)
+            {
+                double e_max = 0, xs_max = 0;
+                GridHost const& g = xs[m];
+                if (g && rec.process_class != static_cast<std::uint32_t>(ProcessClass::annihilation))
+                {
+                    for (std::uint32_t i = 0; i < g.values.size(); ++i)
+                    {
+                        double v = grid_point_xs(g, i);
+                        if (v > xs_max)
+                        {
+                            xs_max = v;
+                            e_max = std::exp(g.front + g.delta * i);
+                        }
+                    }
+                }
+                h.reals.push_back(e_max);
+            }
+        }
+        auto dedx = grids_of(*pj, "dedx_grid", "dedx", nullptr, num_materials_);
+        auto range = grids_of(*pj, "range_grid", "range", nullptr, num_materials_);
+        if (!dedx.empty() && dedx[0] && range[0])
+        {
+            pp.eloss_process = static_cast<std::int32_t>(pp.num_processes);
+            pp.dedx_grid = h.add_grids(dedx);
+            pp.range_grid = h.add_grids(range);
+        }
+
+        auto const& models = pj->at("models");
+        CELER_VALIDATE(models.size() >= 1 && models.size() <= static_cast<std::size_t>(max_models),
+                       << "process must have 1-" << max_models << " models");
+        rec.num_models = static_cast<std::uint32_t>(models.size());
+        for (std::size_t k = 0; k < models.size(); ++k)
+        {
+            auto const& mj = models[k];
+            ModelClass mc = model_from_string(mj.at("model").as_string());
+            rec.models[k].energy_low = mj.get("low", 0.0);
+            rec.models[k].model = static_cast<std::uint32_t>(mc);
+            rec.models[k].action = reg->next_id();
+            reg->insert(std::make_shared<ModelAction>(
+                reg->next_id(), mj.at("label").as_string(), static_cast<std::uint32_t>(mc)));
+            ++num_models;
+        }
+        process_labels_.push_back(pj->get_str("label", ""));
+        ++pp.num_processes;
+    }
+    sc.num_models = num_models;
+    if (!procs.empty())
+    {
+        sc.failure_action = static_cast<std::uint8_t>(reg->next_id());
+        reg->insert(std::make_shared<ImplicitAction>(
+            reg->next_id(), "physics-failure", "mark a track that failed to sample an interaction"));
+    }
+
+    // Model constants
+    if (ref_.electron_id != invalid_id)
+    {
+        ref_.model.electron_mass = particles_[ref_.electron_id].mass;
+        ref_.model.inv_electron_mass = 1 / ref_.model.electron_mass;
+    }
+
+    // Upload
+    h.d_particle = to_device(h.particle);
+    h.d_grids = to_device(h.grids);
+    h.d_reals = to_device(h.reals);
+    h.d_mass = to_device(h.mass);
+    h.d_charge = to_device(h.charge);
+    h.d_cutoff = to_device(h.cutoff_energy);
+    h.d_materials = to_device(h.materials);
+    h.d_components = to_device(h.components);
+    h.d_elements = to_device(h.elements);
+    ref_.num_particles = this->num_particles();
+    ref_.num_materials = num_materials_;
+    ref_.particle = h.d_particle.data();
+    ref_.grids = h.d_grids.data();
+    ref_.reals = h.d_reals.data();
+    ref_.mass = h.d_mass.data();
+    ref_.charge = h.d_charge.data();
+    ref_.cutoff_energy = h.d_cutoff.data();
+    ref_.materials = h.d_materials.data();
+    ref_.components = h.d_components.data();
+    ref_.elements = h.d_elements.data();
 }
 
 PhysicsParams::~PhysicsParams() = default;

@@ -26,6 +26,12 @@ class ActionRegistry;
  * physics actions (pre-step, msc-range, eloss-range, discrete-select,
  * integral-rejected, one per model, failure) in the reference's order and
  * builds flat cross-section tables.
+ *
+ * Input is the "physics" section of a problem description. A process gives
+ * its macroscopic cross sections per material either as Geant4-style
+ * "lambda" (unscaled, low energy) and "lambda_prim" (scaled by E) vectors
+ * on log-spaced energies, or directly as "xs" grids {emin, emax, prime,
+ * values}; energy-loss processes add "dedx"/"range" vectors or grids.
  */
 class PhysicsParams
 {
@@ -42,8 +48,8 @@ class PhysicsParams
     std::uint32_t find_particle(std::string const& label) const;
     double mass(std::uint32_t pid) const { return particles_.at(pid).mass; }
     double charge(std::uint32_t pid) const { return particles_.at(pid).charge; }
-    std::vector<std::string> process_labels() const { return process_labels_; }
-    std::uint32_t pre_step_action() const { return pre_step_action_; }
+    std::vector<std::string> const& process_labels() const { return process_labels_; }
+    bool has_processes() const { return !process_labels_.empty(); }
 
     struct Particle
     {
@@ -53,7 +59,7 @@ class PhysicsParams
         double charge;
     };
 
-    //! Host data block mirrored on device (exported for the test harness)
+    //! Host data block mirrored on device
     struct HostData;
     HostData const& host_data() const { return *host_; }
 
@@ -61,7 +67,6 @@ class PhysicsParams
     std::vector<Particle> particles_;
     size_type num_materials_{0};
     std::vector<std::string> process_labels_;
-    std::uint32_t pre_step_action_{invalid_id};
     std::unique_ptr<HostData> host_;
     PhysicsParamsRef ref_;
 };

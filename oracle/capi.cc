@@ -434,3 +434,17 @@ std::uint32_t orc_run(void* state, void const* prim, std::uint32_t nprim,
     return rc == 0 ? n : 0;
 }
 }
+
+extern "C" {
+//! One step iteration (pending primaries are consumed)
+int orc_step_once(void* state, std::uint32_t* out4)
+{
+    return guard([&] {
+        StepResult r = step_once(*static_cast<OrcState*>(state)->s);
+        out4[0] = r.generated;
+        out4[1] = r.queued;
+        out4[2] = r.active;
+        out4[3] = r.alive;
+    });
+}
+}

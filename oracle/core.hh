@@ -89,6 +89,7 @@ struct Track
     double edep = 0;
     std::uint32_t sec_start = 0, sec_count = 0;
     std::uint32_t element = none;
+    std::uint32_t pre_volume = none;  // volume at the start of the step
     // msc step
     double msc_true_path = 0, msc_geom_path = 0, msc_alpha = -1;
     bool msc_is_displaced = false, msc_enabled = false;

@@ -370,6 +370,7 @@ void pre_step(State& s)
         t.sec_start = 0;
         t.sec_count = 0;
         t.element = none;
+        t.pre_volume = t.geo.vol;
         if (t.status == errored)
             continue;
         t.status = alive;
@@ -671,7 +672,7 @@ void calo_tally(State& s)
     {
         if (t.status == inactive || t.edep == 0)
             continue;
-        std::uint32_t det = prob.volume_to_detector[t.geo.vol];
+        std::uint32_t det = prob.volume_to_detector[t.pre_volume];
         if (det != none)
             s.calo_edep[det] += t.edep;
     }

@@ -249,3 +249,12 @@ def emptying_step(active):
             result = i
         prev = int(active[i])
     return result
+
+
+def step_once(state: "State") -> dict:
+    out = np.zeros(4, np.uint32)
+    f = load().orc_step_once
+    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    f.restype = ctypes.c_int
+    _check(f(state._h, out.ctypes.data))
+    return dict(zip(("generated", "queued", "active", "alive"), (int(v) for v in out)))
