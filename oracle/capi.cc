@@ -448,3 +448,19 @@ int orc_step_once(void* state, std::uint32_t* out4)
     });
 }
 }
+
+extern "C" {
+//! Per-event track counters
+std::uint32_t orc_state_track_counters(void* s, std::uint32_t* out, std::uint32_t max)
+{
+    State const& st = *static_cast<OrcState*>(s)->s;
+    std::uint32_t n = 0;
+    for (auto v : st.track_counters)
+    {
+        if (n < max)
+            out[n] = v;
+        ++n;
+    }
+    return n;
+}
+}

@@ -258,3 +258,12 @@ def step_once(state: "State") -> dict:
     f.restype = ctypes.c_int
     _check(f(state._h, out.ctypes.data))
     return dict(zip(("generated", "queued", "active", "alive"), (int(v) for v in out)))
+
+
+def track_counters(state: "State") -> np.ndarray:
+    out = np.zeros(1 << 16, np.uint32)
+    f = load().orc_state_track_counters
+    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]
+    f.restype = ctypes.c_uint32
+    n = f(state._h, out.ctypes.data, len(out))
+    return out[:n].copy()
