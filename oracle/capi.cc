@@ -464,3 +464,146 @@ std::uint32_t orc_state_track_counters(void* s, std::uint32_t* out, std::uint32_
     return n;
 }
 }
+
+#include "em.hh"
+
+extern "C" {
+//! Sample interactions with one model. Per sample, 17 doubles: action,
+//! energy, dir[3], edep, nsec, then (particle, energy, dir[3]) x 2.
+//! rng_kind 0: default-seeded std::mt19937 through std::generate_canonical
+//! (returns the 32-bit draw count in *count); 1: replay 'uniforms' with
+//! 'stride' values per sample.
+int orc_interactor_sample(void* problem, const char* model, std::uint32_t particle,
+                          double energy, double const* dir, std::uint32_t material,
+                          std::uint32_t element, std::uint32_t n, std::uint32_t stack,
+                          int rng_kind, double const* uniforms, std::uint32_t stride,
+                          double* out, unsigned long long* count)
+{
+    return guard([&] {
+        auto prob = static_cast<OrcProblem*>(problem)->p;
+        auto st = make_state(prob, 1, 0);
+        st->stack.assign(stack, SecondaryP{});
+        InteractorInput in;
+        in.pd = prob->physics.get();
+        in.particle = particle;
+        in.energy = energy;
+        for (int i = 0; i < 3; ++i)
+            in.dir[i] = dir[i];
+        in.material = material;
+        in.element = element;
+        std::mt19937 mt;
+        MtUniform mtu{mt};
+        for (std::uint32_t k = 0; k < n; ++k)
+        {
+            Interaction r;
+            if (rng_kind == 0)
+            {
+                r = sample_model(model, in, *st, mtu);
+            }
+            else
+            {
+                ReplayUniform rep{uniforms + std::size_t(k) * stride};
+                r = sample_model(model, in, *st, rep);
+            }
+            double* o = out + 17 * std::size_t(k);
+            o[0] = r.action;
+            o[1] = r.energy;
+            for (int i = 0; i < 3; ++i)
+                o[2 + i] = r.dir[i];
+            o[5] = r.energy_deposition;
+            o[6] = r.num_secondaries;
+            for (std::uint32_t j = 0; j < 2; ++j)
+            {
+                double* so = o + 7 + 5 * j;
+                if (j < r.num_secondaries && r.action != Interaction::failed)
+                {
+                    SecondaryP const& sec = st->stack[r.sec_start + j];
+                    so[0] = sec.particle == none ? -1.0 : double(sec.particle);
+                    so[1] = sec.energy;
+                    for (int i = 0; i < 3; ++i)
+                        so[2 + i] = sec.dir[i];
+                }
+                else
+                {
+                    for (int i = 0; i < 5; ++i)
+                        so[i] = 0;
+                }
+            }
+        }
+        if (count)
+            *count = mtu.count;
+    });
+}
+
+//! Uniform doubles from a seeded std::mt19937 (std::generate_canonical)
+void orc_mt_uniforms(std::uint32_t seed, std::uint64_t n, double* out)
+{
+    std::mt19937 mt(seed);
+    MtUniform u{mt};
+    for (std::uint64_t i = 0; i < n; ++i)
+        out[i] = u();
+}
+}
+
+extern "C" {
+//! Sample a sequence of items from ONE default-seeded std::mt19937 stream.
+//! Item i: model name (NUL-separated list), particle, energy, dir[3],
+//! material, element, repeat count, stack reset capacity (0: keep stack).
+//! Writes 17 doubles per sample; returns the total 32-bit draw count.
+int orc_interactor_sequence(void* problem, const char* models, std::uint32_t nitems,
+                            std::uint32_t const* particle, double const* energy,
+                            double const* dir, std::uint32_t const* material,
+                            std::uint32_t const* element, std::uint32_t const* repeat,
+                            std::uint32_t const* stack_reset, double* out,
+                            unsigned long long* count)
+{
+    return guard([&] {
+        auto prob = static_cast<OrcProblem*>(problem)->p;
+        auto st = make_state(prob, 1, 0);
+        std::mt19937 mt;
+        MtUniform mtu{mt};
+        char const* name = models;
+        std::size_t k = 0;
+        for (std::uint32_t i = 0; i < nitems; ++i)
+        {
+            std::string model(name);
+            name += model.size() + 1;
+            if (stack_reset[i] > 0)
+            {
+                st->stack.assign(stack_reset[i], SecondaryP{});
+                st->stack_size = 0;
+            }
+            InteractorInput in;
+            in.pd = prob->physics.get();
+            in.particle = particle[i];
+            in.energy = energy[i];
+            for (int d = 0; d < 3; ++d)
+                in.dir[d] = dir[3 * i + d];
+            in.material = material[i];
+            in.element = element[i];
+            for (std::uint32_t r = 0; r < repeat[i]; ++r, ++k)
+            {
+                Interaction res = sample_model(model, in, *st, mtu);
+                double* o = out + 17 * k;
+                o[0] = res.action;
+                o[1] = res.energy;
+                for (int d = 0; d < 3; ++d)
+                    o[2 + d] = res.dir[d];
+                o[5] = res.energy_deposition;
+                o[6] = res.num_secondaries;
+                for (std::uint32_t j = 0; j < 2; ++j)
+                {
+                    double* so = o + 7 + 5 * j;
+                    bool ok = j < res.num_secondaries && res.action != Interaction::failed;
+                    SecondaryP const* sec = ok ? &st->stack[res.sec_start + j] : nullptr;
+                    so[0] = (!sec || sec->particle == none) ? -1.0 : double(sec->particle);
+                    so[1] = sec ? sec->energy : 0;
+                    for (int d = 0; d < 3; ++d)
+                        so[2 + d] = sec ? sec->dir[d] : 0;
+                }
+            }
+        }
+        *count = mtu.count;
+    });
+}
+}

@@ -13,6 +13,8 @@ namespace orc
 namespace
 {
 constexpr double pi = 3.14159265358979323846;
+constexpr double r_electron = 2.8179403262e-13;  // [cm] (CODATA 2014)
+constexpr double alpha_fine = 1 / 137.035999139;
 
 double dot3(double const* a, double const* b)
 {
@@ -22,16 +24,49 @@ double dot3(double const* a, double const* b)
     return r;
 }
 
-//---------------------------------------------------------------------------//
-/*!
- * Klein-Nishina Compton scattering (KleinNishinaInteractor.hh:104-190).
- *
- * Sampling of epsilon from the 1/eps + eps mixture with rejection
- * (Geant4 Physics Reference Manual, section "Compton scattering").
- */
-Interaction klein_nishina(State& s, Track& t)
+//! Exiting direction at polar cosine mu about dir (InteractionUtils.hh)
+template<class Engine>
+void exiting_direction(double mu, double const* dir, Engine& rng, double* out)
 {
-    PhysicsData const& pd = *s.problem->physics;
+    double phi = std::fma(2 * pi, rng(), 0.0);
+    double local[3];
+    from_spherical(mu, phi, local);
+    rotate(local, dir, out);
+}
+
+//! Momentum-conserving exiting direction (calc_exiting_direction)
+void momentum_direction(double p_in, double const* d_in, double p_out,
+                        double const* d_out, double* out)
+{
+    for (int i = 0; i < 3; ++i)
+        out[i] = d_in[i] * p_in - d_out[i] * p_out;
+    make_unit(out);
+}
+
+//! TsaiUrbanDistribution (em/distribution/TsaiUrbanDistribution.hh)
+template<class Engine>
+double tsai_urban(double energy, double mass, Engine& rng)
+{
+    double umax = 2 * (1 + energy / mass);
+    double u;
+    do
+    {
+        double a = rng();
+        double b = rng();
+        double uu = -std::log(a * b);
+        u = uu * (rng() < 0.25 ? 1.6 : 1.6 / 3);
+    } while (u > umax);
+    double r = u / umax;
+    return 1 - 2 * r * r;
+}
+
+//---------------------------------------------------------------------------//
+// KLEIN-NISHINA (KleinNishinaInteractor.hh:104-190)
+//---------------------------------------------------------------------------//
+template<class Engine>
+Interaction klein_nishina(InteractorInput const& in, State& s, Engine& rng)
+{
+    PhysicsData const& pd = *in.pd;
     Interaction r;
     std::uint32_t start;
     SecondaryP* sec = allocate_secondaries(s, 1, &start);
@@ -41,41 +76,37 @@ Interaction klein_nishina(State& s, Track& t)
         return r;
     }
     double const inv_mass = 1 / pd.particles[pd.electron].mass;
-    double const E = t.energy;
+    double const E = in.energy;
     double const k = E * inv_mass;
     double const eps0 = 1 / (1 + 2 * k);
-    // Bernoulli(-log eps0, (1 - eps0^2)/2): p = a / (a + b)
     double const a = -std::log(eps0);
     double const b = 0.5 * (1 - eps0 * eps0);
     double const p_f1 = a / (a + b);
-    double const logratio = std::log(1.0 * eps0);  // Reciprocal(eps0)
+    double const logratio = std::log(1.0 * eps0);
     double const f2_lo = eps0 * eps0;
     double const f2_delta = 1 - f2_lo;
     double eps, omc;
     while (true)
     {
         double eps_sq;
-        if (canonical_double(t.rng) < p_f1)
+        if (rng() < p_f1)
         {
-            eps = std::exp(logratio * canonical_double(t.rng));
+            eps = std::exp(logratio * rng());
             eps_sq = eps * eps;
         }
         else
         {
-            eps_sq = std::fma(f2_delta, canonical_double(t.rng), f2_lo);
+            eps_sq = std::fma(f2_delta, rng(), f2_lo);
             eps = std::sqrt(eps_sq);
         }
         omc = (1 - eps) / (eps * k);
         double sin2 = omc * (2 - omc);
         double reject = eps * sin2 / (1 + eps_sq);
-        if (!(canonical_double(t.rng) < reject))
+        if (!(rng() < reject))
             break;
     }
     r.energy = eps * E;
-    double phi = std::fma(2 * pi, canonical_double(t.rng), 0.0);
-    double local[3];
-    from_spherical(1 - omc, phi, local);
-    rotate(local, t.geo.dir, r.dir);
+    exiting_direction(1 - omc, in.dir, rng, r.dir);
     r.sec_start = start;
     r.num_secondaries = 1;
     sec->energy = E - r.energy;
@@ -86,13 +117,236 @@ Interaction klein_nishina(State& s, Track& t)
         return r;
     }
     sec->particle = pd.electron;
-    // Momentum conservation (InteractionUtils.hh calc_exiting_direction)
-    double v[3];
-    for (int i = 0; i < 3; ++i)
-        v[i] = t.geo.dir[i] * E - r.dir[i] * r.energy;
-    make_unit(v);
-    for (int i = 0; i < 3; ++i)
-        sec->dir[i] = v[i];
+    momentum_direction(E, in.dir, r.energy, r.dir, sec->dir);
+    return r;
+}
+
+//---------------------------------------------------------------------------//
+// BETHE-HEITLER (BetheHeitlerInteractor.hh:165-321), LPM off below 100 GeV
+//---------------------------------------------------------------------------//
+double bh_f1(double d)
+{
+    return d > 1.4 ? 42.038 - 8.29 * std::log(d + 0.958)
+                   : 42.184 - d * (7.444 - 1.623 * d);
+}
+double bh_f2(double d)
+{
+    return d > 1.4 ? 42.038 - 8.29 * std::log(d + 0.958)
+                   : 41.326 - d * (5.848 - 0.902 * d);
+}
+
+template<class Engine>
+Interaction bethe_heitler(InteractorInput const& in, State& s, Engine& rng)
+{
+    PhysicsData const& pd = *in.pd;
+    Interaction r;
+    std::uint32_t start;
+    SecondaryP* sec = allocate_secondaries(s, 2, &start);
+    if (!sec)
+    {
+        r.action = Interaction::failed;
+        return r;
+    }
+    MaterialDef const& mat = pd.materials[in.material];
+    ElementDef const& el = pd.elements[mat.elements[in.element].first];
+    double const me = pd.particles[pd.electron].mass;
+    double const E = in.energy;
+    double const eps0 = me / E;
+    double const half = 0.5;
+    double eps;
+    if (E < 2)
+    {
+        eps = std::fma(half - eps0, rng(), eps0);
+    }
+    else
+    {
+        double const delta_min = 4 * 136 / el.cbrt_z * eps0;
+        double f_z = 8.0 / 3.0 * el.log_z;
+        if (E > 50)
+            f_z += 8 * el.coulomb_correction;
+        double const delta_max = std::exp((42.038 - f_z) / 8.29) - 0.958;
+        double const eps1 = half - half * std::sqrt(1 - delta_min / delta_max);
+        double const eps_min = std::max(eps0, eps1);
+        double const f10 = bh_f1(delta_min) - f_z;
+        double const f20 = bh_f2(delta_min) - f_z;
+        double const st = (half - eps_min) * (half - eps_min) * f10;
+        double const sf = 1.5 * f20;
+        double const p_f1 = st / (st + sf);
+        double g;
+        do
+        {
+            if (rng() < p_f1)
+            {
+                eps = half - (half - eps_min) * std::cbrt(rng());
+                double delta = 136 / el.cbrt_z * eps0 / (eps * (1 - eps));
+                g = (bh_f1(delta) - f_z) / f10;
+            }
+            else
+            {
+                eps = eps_min + (half - eps_min) * rng();
+                double delta = 136 / el.cbrt_z * eps0 / (eps * (1 - eps));
+                g = (bh_f2(delta) - f_z) / f20;
+            }
+        } while (g < rng());
+    }
+    r.action = Interaction::absorbed;
+    r.energy = 0;
+    r.sec_start = start;
+    r.num_secondaries = 2;
+    sec[0].particle = pd.electron;
+    sec[1].particle = pd.positron;
+    sec[0].energy = (1 - eps) * E - me;
+    sec[1].energy = eps * E - me;
+    if (rng() < half)
+        std::swap(sec[0].energy, sec[1].energy);
+    double phi = std::fma(2 * pi, rng(), 0.0);
+    double local[3];
+    double cost = tsai_urban(sec[0].energy, me, rng);
+    from_spherical(cost, phi, local);
+    rotate(local, in.dir, sec[0].dir);
+    cost = tsai_urban(sec[1].energy, me, rng);
+    from_spherical(cost, phi + pi, local);
+    rotate(local, in.dir, sec[1].dir);
+    return r;
+}
+
+//---------------------------------------------------------------------------//
+// MOLLER-BHABHA (MollerBhabhaInteractor.hh:119-157 + distributions)
+//---------------------------------------------------------------------------//
+double moller_g(double gamma, double eps)
+{
+    double two_gamma_term = (2 * gamma - 1) / (gamma * gamma);
+    double comp = 1 - eps;
+    return 1 - two_gamma_term * eps
+           + eps * eps
+                 * (1 - two_gamma_term
+                    + (1 - two_gamma_term * comp) / (comp * comp));
+}
+
+double bhabha_g(double gamma, double eps_min, double eps_max)
+{
+    double y = 1 / (1 + gamma);
+    double y_sq = y * y;
+    double one_minus_2y = 1 - 2 * y;
+    double b1 = 2 - y_sq;
+    double b2 = one_minus_2y * (3 + y_sq);
+    double b4 = one_minus_2y * one_minus_2y * one_minus_2y;
+    double b3 = one_minus_2y * one_minus_2y + b4;
+    double beta_sq = 1 - 1 / (gamma * gamma);
+    double em2 = eps_max * eps_max;
+    return 1
+           + (em2 * em2 * b4 - eps_min * eps_min * eps_min * b3 + em2 * b2
+              - eps_min * b1)
+                 * beta_sq;
+}
+
+template<class Engine>
+Interaction moller_bhabha(InteractorInput const& in, State& s, Engine& rng)
+{
+    PhysicsData const& pd = *in.pd;
+    Interaction r;
+    std::uint32_t start;
+    SecondaryP* sec = allocate_secondaries(s, 1, &start);
+    if (!sec)
+    {
+        r.action = Interaction::failed;
+        return r;
+    }
+    double const me = pd.particles[pd.electron].mass;
+    double const E = in.energy;
+    double const cut = pd.cutoff_energy[pd.electron][in.material];
+    bool const is_electron = in.particle == pd.electron;
+    double const min_frac = cut / E;
+    double const gamma = 1 + E / me;
+    double eps;
+    if (is_electron)
+    {
+        double const max_frac = 0.5;
+        double const g_den = moller_g(gamma, max_frac);
+        double const a = 1 / max_frac;
+        double const delta = 1 / min_frac - a;
+        do
+        {
+            eps = 1 / std::fma(delta, rng(), a);
+        } while (g_den * rng() > moller_g(gamma, eps));
+    }
+    else
+    {
+        double const max_frac = 1;
+        double const g_den = bhabha_g(gamma, min_frac, max_frac);
+        double const a = 1 / max_frac;
+        double const delta = 1 / min_frac - a;
+        do
+        {
+            eps = 1 / std::fma(delta, rng(), a);
+        } while (g_den * rng() > bhabha_g(gamma, eps, eps));
+    }
+    double const e_sec = E * eps;
+    double const inc_momentum = std::sqrt(E * E + 2 * me * E);
+    // IoniFinalStateHelper (em/interactor/detail/IoniFinalStateHelper.hh)
+    double const momentum = std::sqrt(e_sec * (e_sec + 2 * me));
+    double const costheta = e_sec * (E + me + me) / (momentum * inc_momentum);
+    sec->energy = e_sec;
+    exiting_direction(costheta, in.dir, rng, sec->dir);
+    sec->particle = pd.electron;
+    r.energy = E - e_sec;
+    momentum_direction(inc_momentum, in.dir, momentum, sec->dir, r.dir);
+    r.sec_start = start;
+    r.num_secondaries = 1;
+    return r;
+}
+
+//---------------------------------------------------------------------------//
+// POSITRON ANNIHILATION (EPlusGGInteractor.hh:99-170)
+//---------------------------------------------------------------------------//
+template<class Engine>
+Interaction eplusgg(InteractorInput const& in, State& s, Engine& rng)
+{
+    PhysicsData const& pd = *in.pd;
+    Interaction r;
+    std::uint32_t start;
+    SecondaryP* sec = allocate_secondaries(s, 2, &start);
+    if (!sec)
+    {
+        r.action = Interaction::failed;
+        return r;
+    }
+    double const me = pd.particles[pd.electron].mass;
+    double const E = in.energy;
+    r.action = Interaction::absorbed;
+    r.energy = 0;
+    r.sec_start = start;
+    r.num_secondaries = 2;
+    sec[0].particle = sec[1].particle = pd.gamma;
+    if (E == 0)
+    {
+        sec[0].energy = sec[1].energy = me;
+        double costheta = std::fma(2.0, rng(), -1.0);
+        double phi = std::fma(2 * pi, rng(), 0.0);
+        from_spherical(costheta, phi, sec[0].dir);
+        for (int i = 0; i < 3; ++i)
+            sec[1].dir[i] = -sec[0].dir[i];
+        return r;
+    }
+    double const half = 0.5;
+    double const tau = E / me;
+    double const tau2 = tau + 2;
+    double const sqgrate = std::sqrt(tau / tau2) * half;
+    double const lo = half - sqgrate;
+    double const logratio = std::log((1 / lo) * (half + sqgrate));
+    double epsil;
+    do
+    {
+        epsil = lo * std::exp(logratio * rng());
+    } while (rng() < epsil - (2 * (tau + 1) * epsil - 1) / (epsil * tau2 * tau2));
+    double const cost = (epsil * tau2 - 1) / (epsil * std::sqrt(tau * tau2));
+    double const total = E + 2 * me;
+    double const eg = epsil * total;
+    double const eplus_moment = std::sqrt(E * total);
+    sec[0].energy = eg;
+    exiting_direction(cost, in.dir, rng, sec[0].dir);
+    sec[1].energy = total - eg;
+    momentum_direction(eplus_moment, in.dir, E, in.dir, sec[1].dir);
     return r;
 }
 //---------------------------------------------------------------------------//
@@ -152,25 +406,119 @@ SecondaryP* allocate_secondaries(State& s, std::uint32_t n, std::uint32_t* start
     return &s.stack[*start];
 }
 
-Interaction sample_interaction(State& s, Track& t, ModelDef const& model)
+template<class Engine>
+Interaction sample_model(std::string const& model, InteractorInput const& in,
+                         State& s, Engine& rng)
 {
-    if (model.model == "klein_nishina")
-        return klein_nishina(s, t);
-    throw std::runtime_error("oracle: unsupported model " + model.model);
+    if (model == "klein_nishina")
+        return klein_nishina(in, s, rng);
+    if (model == "bethe_heitler")
+        return bethe_heitler(in, s, rng);
+    if (model == "moller_bhabha")
+        return moller_bhabha(in, s, rng);
+    if (model == "eplusgg")
+        return eplusgg(in, s, rng);
+    throw std::runtime_error("oracle: unsupported model " + model);
 }
 
-double hardwired_xs(State const&, ProcessDef const& p, std::uint32_t, double)
+template Interaction sample_model<XorwowUniform>(std::string const&,
+                                                 InteractorInput const&, State&,
+                                                 XorwowUniform&);
+template Interaction sample_model<MtUniform>(std::string const&,
+                                             InteractorInput const&, State&,
+                                             MtUniform&);
+template Interaction sample_model<ReplayUniform>(std::string const&,
+                                                 InteractorInput const&, State&,
+                                                 ReplayUniform&);
+
+Interaction sample_interaction(State& s, Track& t, ModelDef const& model)
 {
+    InteractorInput in;
+    in.pd = s.problem->physics.get();
+    in.particle = t.particle;
+    in.energy = t.energy;
+    for (int i = 0; i < 3; ++i)
+        in.dir[i] = t.geo.dir[i];
+    in.material = s.problem->volume_to_material[t.geo.vol];
+    in.element = t.element == none ? 0 : t.element;
+    XorwowUniform rng{t.rng};
+    return sample_model(model.model, in, s, rng);
+}
+
+double hardwired_xs(State const& s, ProcessDef const& p, std::uint32_t mat, double e)
+{
+    PhysicsData const& pd = *s.problem->physics;
+    if (p.cls == "annihilation")
+    {
+        // EPlusGGMacroXsCalculator (em/xs/EPlusGGMacroXsCalculator.hh): Heitler
+        double const me = pd.particles[pd.electron].mass;
+        double const gamma = std::max(e, 1e-6) / me;
+        double const sqrt_gg2 = std::sqrt(gamma * (gamma + 2));
+        double const gp1 = gamma + 1;
+        double const poly = 1 + gp1 * (4 + gp1 * 1);
+        return pi * r_electron * r_electron * pd.materials[mat].electron_density
+               * (poly * std::log(gamma + 1 + sqrt_gg2) - (gamma + 4) * sqrt_gg2)
+               / (gamma * (gamma + 2) * (gamma + 2));
+    }
     throw std::runtime_error("oracle: no cross section for process " + p.label);
 }
 
-std::uint32_t sample_element(State& s, Track&, ProcessDef const&, std::size_t,
-                             std::uint32_t mat)
+std::uint32_t sample_element(State& s, Track& t, ProcessDef const& p,
+                             std::size_t model_index, std::uint32_t mat)
 {
     PhysicsData const& pd = *s.problem->physics;
-    if (mat < pd.materials.size() && pd.materials[mat].elements.size() > 1)
-        throw std::runtime_error("oracle: element selection not implemented");
-    return 0;
+    if (mat >= pd.materials.size() || pd.materials[mat].elements.size() <= 1)
+        return 0;
+    // Element selection by the per-element micro XS tables
+    auto const& cdf = p.elem_cdf.at(model_index).at(mat);
+    XorwowUniform rng{t.rng};
+    double u = rng();
+    for (std::size_t i = 0; i + 1 < cdf.size(); ++i)
+    {
+        if (u < cdf[i].calc(t.energy))
+            return static_cast<std::uint32_t>(i);
+    }
+    return static_cast<std::uint32_t>(cdf.size());
+}
+
+//---------------------------------------------------------------------------//
+double coulomb_correction(int z)
+{
+    // Davies-Bethe-Maximon: f(Z) = a^2 sum_n 1 / (n (n^2 + a^2)), written as
+    // a^2 [1/(1+a^2) + sum_k (-a^2)^k (zeta(2k+3) - 1)]
+    double const a2 = (alpha_fine * z) * (alpha_fine * z);
+    double fz = 1 / (1 + a2);
+    double apow = 1;
+    for (int k = 0; k < 16; ++k)
+    {
+        int n = 2 * k + 3;
+        double zeta_minus_one = 0;
+        for (int m = 2; m < 200; ++m)
+            zeta_minus_one += std::pow(m, -n);
+        fz += apow * zeta_minus_one;
+        apow *= -a2;
+    }
+    return fz * a2;
+}
+
+double mass_radiation_coeff(int z, double atomic_mass)
+{
+    // Tsai radiation length (PDG review "Passage of particles through matter")
+    double lrad, lrad_prime;
+    switch (z)
+    {
+        case 1: lrad = 5.31; lrad_prime = 6.144; break;
+        case 2: lrad = 4.79; lrad_prime = 5.621; break;
+        case 3: lrad = 4.74; lrad_prime = 5.805; break;
+        case 4: lrad = 4.71; lrad_prime = 5.924; break;
+        default:
+            lrad = std::log(184.15 * std::pow(z, -1.0 / 3));
+            lrad_prime = std::log(1194.0 * std::pow(z, -2.0 / 3));
+    }
+    constexpr double avogadro = 6.02214076e23;
+    double const factor = 4 * alpha_fine * r_electron * r_electron;
+    return factor * avogadro / atomic_mass
+           * (z * z * (lrad - coulomb_correction(z)) + z * lrad_prime);
 }
 
 //---------------------------------------------------------------------------//

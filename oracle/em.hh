@@ -4,8 +4,21 @@
 //---------------------------------------------------------------------------//
 //! \file oracle/em.hh
 //! EM interactors, hardwired cross sections and element selection.
+//!
+//! Interactors are templates over an engine returning uniform doubles so
+//! the same code runs with the transport XORWOW stream and with the
+//! std::mt19937 + std::generate_canonical stream of the reference's
+//! interactor unit tests (test/celeritas/phys/InteractorHostTestBase.hh),
+//! whose golden values pin them:
+//! - KleinNishina.test.cc:89-146 (energies and angles, 4 samples)
+//! - BetheHeitler.test.cc:90-150 (Cu, 100 MeV)
+//! - MollerBhabha.test.cc:108-200
+//! - EPlusGG.test.cc:82-240
 //---------------------------------------------------------------------------//
 #pragma once
+
+#include <cstdint>
+#include <random>
 
 #include "physics.hh"
 
@@ -29,8 +42,65 @@ struct Interaction
     std::uint32_t num_secondaries = 0;
 };
 
+//! Uniform doubles from a track's XORWOW state
+struct XorwowUniform
+{
+    XorwowState& s;
+    double operator()() { return canonical_double(s); }
+};
+
+//! Uniform doubles from std::mt19937 as std::generate_canonical makes them,
+//! counting the 32-bit draws (DiagnosticRngEngine)
+struct MtUniform
+{
+    std::mt19937& mt;
+    std::uint64_t count = 0;
+    struct Counting
+    {
+        using result_type = std::mt19937::result_type;
+        std::mt19937& mt;
+        std::uint64_t& count;
+        static constexpr result_type min() { return std::mt19937::min(); }
+        static constexpr result_type max() { return std::mt19937::max(); }
+        result_type operator()()
+        {
+            ++count;
+            return mt();
+        }
+    };
+    double operator()()
+    {
+        Counting c{mt, count};
+        return std::generate_canonical<double, 53>(c);
+    }
+};
+
+//! Uniform doubles replayed from a buffer (device parity tests)
+struct ReplayUniform
+{
+    double const* u;
+    std::uint32_t i = 0;
+    double operator()() { return u[i++]; }
+};
+
+//! Inputs common to all interactors
+struct InteractorInput
+{
+    PhysicsData const* pd = nullptr;
+    std::uint32_t particle = none;
+    double energy = 0;
+    double dir[3] = {0, 0, 1};
+    std::uint32_t material = 0;
+    std::uint32_t element = 0;  // component index in the material
+};
+
 //! Secondary allocation on the state's stack (StackAllocator semantics)
 SecondaryP* allocate_secondaries(State& s, std::uint32_t n, std::uint32_t* start);
+
+//! Sample one interaction of a model (by model name)
+template<class Engine>
+Interaction sample_model(std::string const& model, InteractorInput const& in,
+                         State& s, Engine& rng);
 
 //! Sample the interaction of the given model for a track
 Interaction sample_interaction(State& s, Track& t, ModelDef const& model);
@@ -41,6 +111,10 @@ double hardwired_xs(State const& s, ProcessDef const& p, std::uint32_t mat, doub
 //! Sample an element component for multi-element materials
 std::uint32_t sample_element(State& s, Track& t, ProcessDef const& p,
                              std::size_t model_index, std::uint32_t mat);
+
+//! Element constants derived from Z (mat/detail/Utils.cc equivalents)
+double coulomb_correction(int z);
+double mass_radiation_coeff(int z, double atomic_mass);
 
 //---------------------------------------------------------------------------//
 // Math helpers shared by interactors (corecel/math/ArrayUtils.hh)

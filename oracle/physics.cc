@@ -228,6 +228,10 @@ void build_physics(Json const& inp, PhysicsData& out)
             ElementDef d;
             d.z = static_cast<int>(e.num("z", 0));
             d.atomic_mass = e.num("atomic_mass", 0);
+            d.cbrt_z = std::cbrt(static_cast<double>(d.z));
+            d.log_z = std::log(static_cast<double>(d.z));
+            d.coulomb_correction = coulomb_correction(d.z);
+            d.mass_radiation_coeff = mass_radiation_coeff(d.z, d.atomic_mass);
             out.elements.push_back(d);
         }
     }

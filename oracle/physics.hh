@@ -54,6 +54,10 @@ struct ElementDef
 {
     int z = 0;
     double atomic_mass = 0;  // [amu]
+    double cbrt_z = 0;
+    double log_z = 0;
+    double coulomb_correction = 0;
+    double mass_radiation_coeff = 0;  // [cm^2/g]
 };
 
 struct MaterialDef
@@ -83,6 +87,8 @@ struct ProcessDef
     std::vector<double> energy_max_xs;  // per material (integral)
     std::vector<XsGridO> dedx, range;  // per material (energy loss)
     std::vector<ModelDef> models;
+    //! Element CDF grids [model][material][element] (multi-element only)
+    std::vector<std::vector<std::vector<XsGridO>>> elem_cdf;
 };
 
 struct PhysicsData

@@ -267,3 +267,61 @@ def track_counters(state: "State") -> np.ndarray:
     f.restype = ctypes.c_uint32
     n = f(state._h, out.ctypes.data, len(out))
     return out[:n].copy()
+
+
+INTERACTION_FIELDS = 17
+
+
+def interactor_sample(problem, model, particle, energy, direction, n, material=0,
+                      element=0, stack=None, uniforms=None, stride=0):
+    """Sample n interactions; returns (records[n, 17], mt_draw_count)."""
+    out = np.zeros((n, INTERACTION_FIELDS), np.float64)
+    d = np.asarray(direction, np.float64)
+    cnt = ctypes.c_ulonglong(0)
+    f = load().orc_interactor_sample
+    f.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_uint32, ctypes.c_double,
+                  ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                  ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint32,
+                  ctypes.c_void_p, ctypes.POINTER(ctypes.c_ulonglong)]
+    f.restype = ctypes.c_int
+    kind = 0 if uniforms is None else 1
+    u = None if uniforms is None else np.ascontiguousarray(uniforms, np.float64)
+    _check(f(problem.handle, model.encode(), particle, energy, d.ctypes.data, material,
+             element, n, stack if stack is not None else 2 * n, kind,
+             None if u is None else u.ctypes.data, stride, out.ctypes.data,
+             ctypes.byref(cnt)))
+    return out, cnt.value
+
+
+def mt_uniforms(seed: int, n: int) -> np.ndarray:
+    out = np.zeros(n, np.float64)
+    f = load().orc_mt_uniforms
+    f.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_void_p]
+    f(seed, n, out.ctypes.data)
+    return out
+
+
+def interactor_sequence(problem, items, count=False):
+    """Sample items [(model, particle, energy, dir[, repeat, stack_reset])]
+    from one default-seeded mt19937 stream; returns records (and draws)."""
+    models = b"".join(it[0].encode() + b"\0" for it in items)
+    n = len(items)
+    part = np.array([it[1] for it in items], np.uint32)
+    en = np.array([it[2] for it in items], np.float64)
+    dirs = np.array([it[3] for it in items], np.float64).reshape(n, 3)
+    mat = np.array([it[6] if len(it) > 6 else 0 for it in items], np.uint32)
+    el = np.zeros(n, np.uint32)
+    rep = np.array([it[4] if len(it) > 4 else 1 for it in items], np.uint32)
+    reset = np.array([it[5] if len(it) > 5 else 0 for it in items], np.uint32)
+    if reset[0] == 0:
+        reset[0] = max(2 * int(rep.sum()), 2)
+    out = np.zeros((int(rep.sum()), INTERACTION_FIELDS), np.float64)
+    cnt = ctypes.c_ulonglong(0)
+    f = load().orc_interactor_sequence
+    f.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_uint32] + [ctypes.c_void_p] * 7 \
+        + [ctypes.c_void_p, ctypes.POINTER(ctypes.c_ulonglong)]
+    f.restype = ctypes.c_int
+    _check(f(problem.handle, models, n, part.ctypes.data, en.ctypes.data, dirs.ctypes.data,
+             mat.ctypes.data, el.ctypes.data, rep.ctypes.data, reset.ctypes.data,
+             out.ctypes.data, ctypes.byref(cnt)))
+    return (out, cnt.value) if count else out

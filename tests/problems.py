@@ -79,3 +79,33 @@ def simple_compton_primaries(n):
     return celeritas_amd.make_primaries(
         {"particle_id": 0, "energy": 100.0, "pos": [-22.0, 0, 0], "dir": [1.0, 0, 0]}
         for _ in range(n))
+
+
+EMASS = 0.5109989461
+
+
+def interactor_host():
+    """InteractorHostTestBase particles (electron mass 0.5109989461 MeV) with
+    Cu (Z=29) and K (Z=19) and a 1 keV electron cutoff."""
+    return {
+        "geometry": load_geometry("two-boxes.org.json"),
+        "volume_to_material": [-1, 0, 0],
+        "physics": {
+            "particles": [
+                {"label": "electron", "pdg": 11, "mass": EMASS, "charge": -1.0},
+                {"label": "positron", "pdg": -11, "mass": EMASS, "charge": 1.0},
+                {"label": "gamma", "pdg": 22, "mass": 0.0, "charge": 0.0},
+            ],
+            "elements": [{"z": 29, "atomic_mass": 63.546}, {"z": 19, "atomic_mass": 39.0983}],
+            "materials": [
+                {"label": "Cu", "number_density": 6.02214076e23, "electron_density": 29 * 6.02214076e23,
+                 "elements": [{"element": 0, "fraction": 1.0}]},
+                {"label": "K", "number_density": 6.02214076e23, "electron_density": 19 * 6.02214076e23,
+                 "elements": [{"element": 1, "fraction": 1.0}]},
+            ],
+            "cutoffs": {"electron": [{"energy": 0.001}, {"energy": 0.001}]},
+        },
+    }
+
+
+ELECTRON, POSITRON, GAMMA_ID = 0, 1, 2
