@@ -422,3 +422,43 @@ class Stepper:
         buf = ctypes.create_string_buffer(8192)
         _check(_load().celer_stepper_action_sequence(self._h, buf, 8192))
         return [s for s in buf.value.decode().split("\n") if s]
+
+
+MODEL_CLASSES = {
+    "klein_nishina": 0,
+    "livermore_pe": 1,
+    "bethe_heitler": 2,
+    "moller_bhabha": 3,
+    "seltzer_berger": 4,
+    "relativistic_brem": 5,
+    "eplusgg": 6,
+}
+
+
+def verify_interactor(params: CoreParams, state: CoreState, model: str, particle: int,
+                      energies, dirs, uniforms, stride: int, material: int = 0,
+                      element: int = 0) -> np.ndarray:
+    """Run a device interactor on replayed uniforms (17 doubles per sample)."""
+    lib_ = _load()
+    f = lib_.celer_verify_interactor
+    f.argtypes = [ctypes.c_void_p] * 2 + [ctypes.c_uint32] * 2 + [ctypes.c_void_p] * 2 \
+        + [ctypes.c_uint32] * 3 + [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+                                   ctypes.c_void_p]
+    f.restype = ctypes.c_int
+    lib_.celer_params_device_ref.restype = ctypes.c_void_p
+    lib_.celer_params_device_ref.argtypes = [ctypes.c_void_p]
+    lib_.celer_state_device_ref.restype = ctypes.c_void_p
+    lib_.celer_state_device_ref.argtypes = [ctypes.c_void_p]
+    lib_.celer_state_stream.restype = ctypes.c_void_p
+    lib_.celer_state_stream.argtypes = [ctypes.c_void_p]
+    e = np.ascontiguousarray(energies, np.float64)
+    d = np.ascontiguousarray(dirs, np.float64)
+    u = np.ascontiguousarray(uniforms, np.float64)
+    n = len(e)
+    out = np.zeros((n, 17), np.float64)
+    rc = f(lib_.celer_params_device_ref(params.handle), lib_.celer_state_device_ref(state._h),
+           MODEL_CLASSES[model], particle, e.ctypes.data, d.ctypes.data, material, element, n,
+           u.ctypes.data, stride, out.ctypes.data, lib_.celer_state_stream(state._h))
+    if rc != 0:
+        raise CeleritasError(f"celer_verify_interactor failed with HIP error {rc}")
+    return out

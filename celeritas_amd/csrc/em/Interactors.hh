@@ -9,6 +9,7 @@
 #include "geo/OrangeLiteTracker.hh"
 #include "InteractorCommon.hh"
 #include "KleinNishina.hh"
+#include "StandardEm.hh"
 
 namespace celeritas_amd
 {
@@ -57,10 +58,22 @@ CELER_FUNCTION Interaction sample_interaction(ModelClass model,
                                               double const* dir,
                                               Engine& rng)
 {
+    PhysicsParamsRef const& p = ctx.params;
     switch (model)
     {
         case ModelClass::klein_nishina:
-            return sample_klein_nishina(ctx.params, ctx.state, energy, dir, rng);
+            return sample_klein_nishina(p, ctx.state, energy, dir, rng);
+        case ModelClass::bethe_heitler: {
+            MaterialRecord const& m = p.materials[ctx.material];
+            ElementRecord const& el
+                = p.elements[p.components[m.elem_begin + ctx.element].element];
+            return sample_bethe_heitler(p, ctx.state, el, energy, dir, rng);
+        }
+        case ModelClass::moller_bhabha:
+            return sample_moller_bhabha(p, ctx.state, ctx.particle, ctx.material,
+                                        energy, dir, rng);
+        case ModelClass::eplusgg:
+            return sample_eplusgg(p, ctx.state, energy, dir, rng);
         default:
             break;
     }

@@ -55,15 +55,36 @@ calc_grid(XsGrid const& g, double const* __restrict__ reals, double energy)
     return result;
 }
 
+inline constexpr double r_electron = 2.8179403262e-13;  //!< [cm]
+
+//! Heitler e+ annihilation macroscopic cross section [1/cm]
+//! (em/xs/EPlusGGMacroXsCalculator.hh)
+CELER_FUNCTION double
+eplusgg_macro_xs(double electron_mass, double electron_density, double energy)
+{
+    double const gamma = (energy > 1e-6 ? energy : 1e-6) / electron_mass;
+    double const sqrt_gg2 = std::sqrt(gamma * (gamma + 2));
+    double const gp1 = gamma + 1;
+    double const poly = 1 + gp1 * (4 + gp1 * 1);
+    return pi * r_electron * r_electron * electron_density
+           * (poly * std::log(gamma + 1 + sqrt_gg2) - (gamma + 4) * sqrt_gg2)
+           / (gamma * (gamma + 2) * (gamma + 2));
+}
+
 //! Macroscopic cross section of a process in a material
 CELER_FUNCTION double calc_xs(PhysicsParamsRef const& p,
                               ProcessRecord const& proc,
                               std::uint32_t mat,
                               double energy)
 {
-    if (proc.xs_grid == invalid_id)
-        return 0;
-    return calc_grid(p.grids[proc.xs_grid + mat], p.reals, energy);
+    if (proc.xs_grid != invalid_id)
+        return calc_grid(p.grids[proc.xs_grid + mat], p.reals, energy);
+    if (proc.process_class == static_cast<std::uint32_t>(ProcessClass::annihilation))
+    {
+        return eplusgg_macro_xs(p.model.electron_mass,
+                                p.materials[mat].electron_density, energy);
+    }
+    return 0;
 }
 
 //! Upper bound of the XS over the step (PhysicsTrackView.hh:446-478)

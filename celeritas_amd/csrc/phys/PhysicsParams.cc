@@ -143,6 +143,53 @@ std::vector<GridHost> grids_of(JsonValue const& proc,
     return result;
 }
 
+//---------------------------------------------------------------------------//
+constexpr double alpha_fine = 1 / 137.035999139;
+constexpr double r_electron_cm = 2.8179403262e-13;
+constexpr double avogadro = 6.02214076e23;
+
+/*!
+ * Coulomb correction f(Z) of Davies, Bethe and Maximon (1954).
+ *
+ * f = a^2 sum_{n>=1} 1/(n (n^2 + a^2)) with a = alpha Z, evaluated through
+ * its expansion a^2 [1/(1+a^2) + sum_k (-a^2)^k (zeta(2k+3) - 1)], with the
+ * zeta values summed directly.
+ */
+double calc_coulomb_correction(int z)
+{
+    double const a2 = (alpha_fine * z) * (alpha_fine * z);
+    double result = 1 / (1 + a2);
+    double apow = 1;
+    for (int k = 0; k < 16; ++k)
+    {
+        int const n = 2 * k + 3;
+        double zm1 = 0;
+        for (int m = 2; m < 200; ++m)
+            zm1 += std::pow(static_cast<double>(m), -n);
+        result += apow * zm1;
+        apow *= -a2;
+    }
+    return result * a2;
+}
+
+//! Inverse radiation length per unit density [cm^2/g] (Tsai, PDG review)
+double calc_mass_radiation_coeff(int z, double atomic_mass)
+{
+    double lrad, lrad_prime;
+    switch (z)
+    {
+        case 1: lrad = 5.31; lrad_prime = 6.144; break;
+        case 2: lrad = 4.79; lrad_prime = 5.621; break;
+        case 3: lrad = 4.74; lrad_prime = 5.805; break;
+        case 4: lrad = 4.71; lrad_prime = 5.924; break;
+        default:
+            lrad = std::log(184.15 * std::pow(z, -1.0 / 3));
+            lrad_prime = std::log(1194.0 * std::pow(z, -2.0 / 3));
+    }
+    return 4 * alpha_fine * r_electron_cm * r_electron_cm * avogadro / atomic_mass
+           * (z * z * (lrad - calc_coulomb_correction(z)) + z * lrad_prime);
+}
+
 //! Value of a grid at point i, unscaled (XsCalculator::operator[])
 double grid_point_xs(GridHost const& g, std::uint32_t i)
 {
@@ -247,8 +294,10 @@ PhysicsParams::PhysicsParams(JsonValue const& inp, ActionRegistry* reg)
             r.cbrt_z = std::cbrt(r.z);
             r.cbrt_zzp = std::cbrt(r.z * (r.z + 1));
             r.log_z = r.z > 0 ? std::log(r.z) : 0;
-            r.coulomb_correction = e.get("coulomb_correction", 0.0);
-            r.mass_radiation_coeff = e.get("mass_radiation_coeff", 0.0);
+            int const z = static_cast<int>(r.z);
+            r.coulomb_correction = z > 0 ? calc_coulomb_correction(z) : 0;
+            r.mass_radiation_coeff
+                = z > 0 ? calc_mass_radiation_coeff(z, r.atomic_mass) : 0;
             h.elements.push_back(r);
         }
     }

@@ -263,6 +263,27 @@ int celer_hip_reseed(const void* params_ref,
                      uint64_t event_id,
                      void* stream);
 
+/*---------------------------------------------------------------------------*
+ * VERIFICATION: run one interactor model (ModelClass) on the device with
+ * replayed uniform streams (stride doubles per sample). Writes 17 doubles
+ * per sample: action, energy, dir[3], edep, nsec, then (particle, energy,
+ * dir[3]) for up to two secondaries. Counterpart of the reference's device
+ * interactor tests (test/celeritas/em/*.test.cc with *.test.cu wrappers).
+ *---------------------------------------------------------------------------*/
+int celer_verify_interactor(const void* params_ref,
+                            const void* state_ref,
+                            uint32_t model,
+                            uint32_t particle,
+                            const double* energies,
+                            const double* dirs,
+                            uint32_t material,
+                            uint32_t element,
+                            uint32_t n,
+                            const double* uniforms,
+                            uint32_t stride,
+                            double* out,
+                            void* stream);
+
 #ifdef __cplusplus
 }
 #endif

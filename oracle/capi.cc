@@ -607,3 +607,49 @@ int orc_interactor_sequence(void* problem, const char* models, std::uint32_t nit
     });
 }
 }
+
+extern "C" {
+//! Replay interactors sample by sample with per-sample energies/directions
+int orc_interactor_replay_batch(void* problem, const char* model, std::uint32_t particle,
+                                double const* energies, double const* dirs,
+                                std::uint32_t material, std::uint32_t element,
+                                std::uint32_t n, double const* uniforms,
+                                std::uint32_t stride, double* out)
+{
+    return guard([&] {
+        auto prob = static_cast<OrcProblem*>(problem)->p;
+        auto st = make_state(prob, 1, 0);
+        st->stack.assign(std::size_t(2) * n + 2, SecondaryP{});
+        InteractorInput in;
+        in.pd = prob->physics.get();
+        in.particle = particle;
+        in.material = material;
+        in.element = element;
+        for (std::uint32_t k = 0; k < n; ++k)
+        {
+            in.energy = energies[k];
+            for (int i = 0; i < 3; ++i)
+                in.dir[i] = dirs[3 * k + i];
+            ReplayUniform rep{uniforms + std::size_t(k) * stride};
+            Interaction r = sample_model(model, in, *st, rep);
+            double* o = out + 17 * std::size_t(k);
+            o[0] = r.action;
+            o[1] = r.energy;
+            for (int i = 0; i < 3; ++i)
+                o[2 + i] = r.dir[i];
+            o[5] = r.energy_deposition;
+            o[6] = r.num_secondaries;
+            for (std::uint32_t j = 0; j < 2; ++j)
+            {
+                double* so = o + 7 + 5 * j;
+                bool ok = j < r.num_secondaries && r.action != Interaction::failed;
+                SecondaryP const* sec = ok ? &st->stack[r.sec_start + j] : nullptr;
+                so[0] = (!sec || sec->particle == none) ? -1.0 : double(sec->particle);
+                so[1] = sec ? sec->energy : 0;
+                for (int i = 0; i < 3; ++i)
+                    so[2 + i] = sec ? sec->dir[i] : 0;
+            }
+        }
+    });
+}
+}

@@ -325,3 +325,20 @@ def interactor_sequence(problem, items, count=False):
              mat.ctypes.data, el.ctypes.data, rep.ctypes.data, reset.ctypes.data,
              out.ctypes.data, ctypes.byref(cnt)))
     return (out, cnt.value) if count else out
+
+
+def interactor_replay_batch(problem, model, particle, energies, dirs, uniforms, stride,
+                            material=0, element=0):
+    e = np.ascontiguousarray(energies, np.float64)
+    d = np.ascontiguousarray(dirs, np.float64)
+    u = np.ascontiguousarray(uniforms, np.float64)
+    n = len(e)
+    out = np.zeros((n, INTERACTION_FIELDS), np.float64)
+    f = load().orc_interactor_replay_batch
+    f.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_uint32, ctypes.c_void_p,
+                  ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                  ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]
+    f.restype = ctypes.c_int
+    _check(f(problem.handle, model.encode(), particle, e.ctypes.data, d.ctypes.data,
+             material, element, n, u.ctypes.data, stride, out.ctypes.data))
+    return out
