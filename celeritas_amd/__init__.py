@@ -432,6 +432,7 @@ MODEL_CLASSES = {
     "seltzer_berger": 4,
     "relativistic_brem": 5,
     "eplusgg": 6,
+    "combined_brem": 7,
 }
 
 

@@ -7,6 +7,7 @@
 #pragma once
 
 #include "geo/OrangeLiteTracker.hh"
+#include "Brems.hh"
 #include "InteractorCommon.hh"
 #include "KleinNishina.hh"
 #include "StandardEm.hh"
@@ -74,6 +75,15 @@ CELER_FUNCTION Interaction sample_interaction(ModelClass model,
                                         energy, dir, rng);
         case ModelClass::eplusgg:
             return sample_eplusgg(p, ctx.state, energy, dir, rng);
+        case ModelClass::seltzer_berger:
+            return sample_brems(p, ctx.state, ctx.particle, ctx.material,
+                                ctx.element, energy, dir, 0, rng);
+        case ModelClass::relativistic_brem:
+            return sample_brems(p, ctx.state, ctx.particle, ctx.material,
+                                ctx.element, energy, dir, 1, rng);
+        case ModelClass::combined_brem:
+            return sample_brems(p, ctx.state, ctx.particle, ctx.material,
+                                ctx.element, energy, dir, 2, rng);
         default:
             break;
     }

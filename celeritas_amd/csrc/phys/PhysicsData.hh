@@ -33,6 +33,7 @@ enum class ModelClass : std::uint32_t
     seltzer_berger,
     relativistic_brem,
     eplusgg,
+    combined_brem,
     size_
 };
 
@@ -105,7 +106,20 @@ struct ElementRecord
     double log_z;
     double coulomb_correction;  //!< f(Z) of Davies-Bethe-Maximon
     double mass_radiation_coeff;  //!< 1/X0 per unit density [cm^2/g]
-    double pad_;
+    // Relativistic brems (RelativisticBremModel.cc:150-185)
+    double rb_fz;
+    double rb_factor1;
+    double rb_factor2;
+    double rb_gamma_factor;
+    double rb_epsilon_factor;
+    // Seltzer-Berger table: offsets into reals (x = log E, y = k/E, values
+    // x-major) and into sb_argmax; sb_nx == 0 when absent
+    std::uint32_t sb_x;
+    std::uint32_t sb_y;
+    std::uint32_t sb_values;
+    std::uint32_t sb_argmax;
+    std::uint32_t sb_nx;
+    std::uint32_t sb_ny;
 };
 
 //! Component of a material
@@ -184,6 +198,7 @@ struct PhysicsParamsRef
     MaterialRecord const* materials{nullptr};
     ElementComponent const* components{nullptr};
     ElementRecord const* elements{nullptr};
+    std::uint32_t const* sb_argmax{nullptr};  //!< per-x argmax of SB tables
     ModelData model{};
 };
 

@@ -5,6 +5,7 @@
 //---------------------------------------------------------------------------//
 #include "PhysicsParams.hh"
 
+#include <algorithm>
 #include <cmath>
 
 #include "global/ActionInterface.hh"
@@ -20,7 +21,7 @@ ModelClass model_from_string(std::string const& s)
     static char const* const names[] = {"klein_nishina", "livermore_pe",
                                         "bethe_heitler", "moller_bhabha",
                                         "seltzer_berger", "relativistic_brem",
-                                        "eplusgg"};
+                                        "eplusgg", "combined_brem"};
     for (int i = 0; i < static_cast<int>(ModelClass::size_); ++i)
     {
         if (s == names[i])
@@ -144,16 +145,39 @@ std::vector<GridHost> grids_of(JsonValue const& proc,
 }
 
 //---------------------------------------------------------------------------//
-constexpr double alpha_fine = 1 / 137.035999139;
+constexpr double alpha_fine = 7.2973525693e-3;  // CODATA 2018
 constexpr double r_electron_cm = 2.8179403262e-13;
 constexpr double avogadro = 6.02214076e23;
+
+/*!
+ * Riemann zeta(n) - 1 for integer n >= 3.
+ *
+ * Direct sum over 2 <= m < 64 (smallest terms first) plus the
+ * Euler-Maclaurin remainder from m = 64: the integral, the half end term
+ * and the B2, B4, B6 corrections; the next term is below 1e-18.
+ */
+double zeta_minus_one(int n)
+{
+    constexpr int big_m = 64;
+    double sum = 0;
+    for (int m = big_m - 1; m >= 2; --m)
+        sum += std::pow(static_cast<double>(m), -n);
+    double const x = big_m;
+    double const xn = std::pow(x, -n);
+    double const dn = n;
+    double tail = x * xn / (dn - 1) + xn / 2 + dn * xn / (12 * x)
+                  - dn * (dn + 1) * (dn + 2) * xn / (720 * x * x * x)
+                  + dn * (dn + 1) * (dn + 2) * (dn + 3) * (dn + 4) * xn
+                        / (30240 * x * x * x * x * x);
+    return tail + sum;
+}
 
 /*!
  * Coulomb correction f(Z) of Davies, Bethe and Maximon (1954).
  *
  * f = a^2 sum_{n>=1} 1/(n (n^2 + a^2)) with a = alpha Z, evaluated through
- * its expansion a^2 [1/(1+a^2) + sum_k (-a^2)^k (zeta(2k+3) - 1)], with the
- * zeta values summed directly.
+ * its expansion a^2 [1/(1+a^2) + sum_k (-a^2)^k (zeta(2k+3) - 1)] to 16
+ * terms (exact to double precision through Z = 92).
  */
 double calc_coulomb_correction(int z)
 {
@@ -162,11 +186,7 @@ double calc_coulomb_correction(int z)
     double apow = 1;
     for (int k = 0; k < 16; ++k)
     {
-        int const n = 2 * k + 3;
-        double zm1 = 0;
-        for (int m = 2; m < 200; ++m)
-            zm1 += std::pow(static_cast<double>(m), -n);
-        result += apow * zm1;
+        result += apow * zeta_minus_one(2 * k + 3);
         apow *= -a2;
     }
     return result * a2;
@@ -223,6 +243,8 @@ struct PhysicsParams::HostData
     DeviceVector<MaterialRecord> d_materials;
     DeviceVector<ElementComponent> d_components;
     DeviceVector<ElementRecord> d_elements;
+    std::vector<std::uint32_t> sb_argmax;
+    DeviceVector<std::uint32_t> d_sb_argmax;
 
     //! Append grids (one per material); returns first index or invalid
     std::uint32_t add_grids(std::vector<GridHost> const& gs)
@@ -243,6 +265,37 @@ struct PhysicsParams::HostData
             grids.push_back(x);
         }
         return first;
+    }
+
+    /*!
+     * Append an SB table {x: log(E/MeV), y: k/E, values: x-major} with the
+     * first maximum of each x row (SeltzerBergerModel.cc:append_table).
+     */
+    void append_sb_table(JsonValue const& j, ElementRecord& r)
+    {
+        auto x = j.at("x").as_doubles();
+        auto y = j.at("y").as_doubles();
+        auto v = j.at("values").as_doubles();
+        CELER_VALIDATE(x.size() >= 2 && y.size() >= 2 && v.size() == x.size() * y.size(),
+                       << "inconsistent Seltzer-Berger table");
+        CELER_VALIDATE(std::is_sorted(x.begin(), x.end())
+                           && std::is_sorted(y.begin(), y.end()),
+                       << "Seltzer-Berger grids must increase");
+        r.sb_nx = static_cast<std::uint32_t>(x.size());
+        r.sb_ny = static_cast<std::uint32_t>(y.size());
+        r.sb_x = static_cast<std::uint32_t>(reals.size());
+        reals.insert(reals.end(), x.begin(), x.end());
+        r.sb_y = static_cast<std::uint32_t>(reals.size());
+        reals.insert(reals.end(), y.begin(), y.end());
+        r.sb_values = static_cast<std::uint32_t>(reals.size());
+        reals.insert(reals.end(), v.begin(), v.end());
+        r.sb_argmax = static_cast<std::uint32_t>(sb_argmax.size());
+        for (std::size_t i = 0; i < x.size(); ++i)
+        {
+            auto first = v.begin() + i * y.size();
+            sb_argmax.push_back(static_cast<std::uint32_t>(
+                std::max_element(first, first + y.size()) - first));
+        }
     }
 };
 
@@ -298,6 +351,23 @@ PhysicsParams::PhysicsParams(JsonValue const& inp, ActionRegistry* reg)
             r.coulomb_correction = z > 0 ? calc_coulomb_correction(z) : 0;
             r.mass_radiation_coeff
                 = z > 0 ? calc_mass_radiation_coeff(z, r.atomic_mass) : 0;
+            if (z >= 5 && ref_.electron_id != invalid_id)
+            {
+                // Thomas-Fermi form factors (the Dirac-Fock values for
+                // Z < 5 are not provided)
+                double const me = particles_[ref_.electron_id].mass;
+                double const ff_el = std::log(184.15) - r.log_z / 3;
+                double const ff_inel = std::log(1194.0) - 2 * r.log_z / 3;
+                double const fc = r.coulomb_correction;
+                double const invz = 1.0 / z;
+                r.rb_fz = r.log_z / 3 + fc;
+                r.rb_factor1 = (ff_el - fc) + ff_inel * invz;
+                r.rb_factor2 = (1 + invz) / 12;
+                r.rb_gamma_factor = 100 * me / r.cbrt_z;
+                r.rb_epsilon_factor = 100 * me / (r.cbrt_z * r.cbrt_z);
+            }
+            if (e.has("sb_table"))
+                h.append_sb_table(e.at("sb_table"), r);
             h.elements.push_back(r);
         }
     }
@@ -485,6 +555,8 @@ PhysicsParams::PhysicsParams(JsonValue const& inp, ActionRegistry* reg)
     h.d_materials = to_device(h.materials);
     h.d_components = to_device(h.components);
     h.d_elements = to_device(h.elements);
+    h.d_sb_argmax = to_device(h.sb_argmax);
+    ref_.sb_argmax = h.d_sb_argmax.data();
     ref_.num_particles = this->num_particles();
     ref_.num_materials = num_materials_;
     ref_.particle = h.d_particle.data();

@@ -15,7 +15,7 @@
  *             corecel/sys/ActionRegistry.hh:42-105 (find_action, labels)
  *   stepper   celeritas/global/Stepper.hh:43-201 (Stepper<device>)
  *   offload   accel/LocalTransporter.hh:52-109 (Push/Flush/InitializeEvent)
- *   launch    celeritas/track/*Action.cu, celeritas/random/RngReseed.cu:59
+ *   launch    celeritas/track/{ExtendFromPrimaries,InitializeTracks,ExtendFromSecondaries}Action.cu, celeritas/random/RngReseed.cu:59
  *---------------------------------------------------------------------------*/
 #ifndef CELERITAS_AMD_H
 #define CELERITAS_AMD_H
@@ -268,7 +268,7 @@ int celer_hip_reseed(const void* params_ref,
  * replayed uniform streams (stride doubles per sample). Writes 17 doubles
  * per sample: action, energy, dir[3], edep, nsec, then (particle, energy,
  * dir[3]) for up to two secondaries. Counterpart of the reference's device
- * interactor tests (test/celeritas/em/*.test.cc with *.test.cu wrappers).
+ * interactor tests (test/celeritas/em/ interactor tests and their .test.cu wrappers).
  *---------------------------------------------------------------------------*/
 int celer_verify_interactor(const void* params_ref,
                             const void* state_ref,

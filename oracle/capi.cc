@@ -502,7 +502,7 @@ int orc_interactor_sample(void* problem, const char* model, std::uint32_t partic
             }
             else
             {
-                ReplayUniform rep{uniforms + std::size_t(k) * stride};
+                ReplayUniform rep{uniforms + std::size_t(k) * stride, 0, stride};
                 r = sample_model(model, in, *st, rep);
             }
             double* o = out + 17 * std::size_t(k);
@@ -542,6 +542,87 @@ void orc_mt_uniforms(std::uint32_t seed, std::uint64_t n, double* out)
     MtUniform u{mt};
     for (std::uint64_t i = 0; i < n; ++i)
         out[i] = u();
+}
+
+//---------------------------------------------------------------------------//
+// Bremsstrahlung pieces (SeltzerBerger.test.cc, RelativisticBrem.test.cc)
+//! A default-seeded std::mt19937 that persists across calls
+void* orc_mt_create() { return new std::mt19937; }
+void orc_mt_destroy(void* mt) { delete static_cast<std::mt19937*>(mt); }
+
+int orc_sb_argmax(void* problem, std::uint32_t element, std::uint32_t* out,
+                  std::uint32_t n)
+{
+    return guard([&] {
+        auto const& t = static_cast<OrcProblem*>(problem)->p->physics->elements.at(element).sb;
+        if (t.argmax.size() != n)
+            throw std::runtime_error("argmax size mismatch");
+        std::copy(t.argmax.begin(), t.argmax.end(), out);
+    });
+}
+
+double orc_sb_positron_scale(int z, double mass, double kc, double inc, double exit)
+{
+    return sb_positron_scale(z, mass, kc, inc, exit);
+}
+
+//! Sample n SB photon energies on a persistent engine: out = {max_xs, mean
+//! exit fraction, mean 32-bit draws per sample}
+int orc_sb_sample_many(void* problem, void* mt, std::uint32_t material,
+                       std::uint32_t element, double energy, double mass,
+                       double kc, int corrector, std::uint32_t n, double* out)
+{
+    return guard([&] {
+        auto const& pd = *static_cast<OrcProblem*>(problem)->p->physics;
+        MaterialDef const& mat = pd.materials.at(material);
+        ElementDef const& el = pd.elements.at(mat.elements.at(element).first);
+        MtUniform u{*static_cast<std::mt19937*>(mt)};
+        double total = 0;
+        double max_xs = 0;
+        for (std::uint32_t i = 0; i < n; ++i)
+            total += sample_sb_energy(el, mat.electron_density, energy, mass, kc,
+                                      corrector, u, &max_xs);
+        out[0] = max_xs;
+        out[1] = total / (n * energy);
+        out[2] = double(u.count) / n;
+    });
+}
+
+//! Run n interactions on a persistent engine with a stack of n secondaries
+//! per call (the reference's stress tests); returns the 32-bit draw count
+int orc_interactor_stress(void* problem, void* mt, const char* model,
+                          std::uint32_t particle, double energy, double const* dir,
+                          std::uint32_t material, std::uint32_t n,
+                          unsigned long long* count)
+{
+    return guard([&] {
+        auto prob = static_cast<OrcProblem*>(problem)->p;
+        auto st = make_state(prob, 1, 0);
+        st->stack.assign(std::max<std::uint32_t>(n, 1) * 2, SecondaryP{});
+        InteractorInput in;
+        in.pd = prob->physics.get();
+        in.particle = particle;
+        in.energy = energy;
+        for (int i = 0; i < 3; ++i)
+            in.dir[i] = dir[i];
+        make_unit(in.dir);
+        in.material = material;
+        MtUniform u{*static_cast<std::mt19937*>(mt)};
+        for (std::uint32_t k = 0; k < n; ++k)
+        {
+            Interaction r = sample_model(model, in, *st, u);
+            if (r.action == Interaction::failed)
+                throw std::runtime_error("interaction failed in stress loop");
+        }
+        *count = u.count;
+    });
+}
+
+double orc_rb_dxsec(void* problem, std::uint32_t element, double total_energy,
+                    double gamma_energy)
+{
+    auto const& pd = *static_cast<OrcProblem*>(problem)->p->physics;
+    return rb_dxsec(pd.elements.at(element), total_energy, gamma_energy);
 }
 }
 
@@ -630,7 +711,7 @@ int orc_interactor_replay_batch(void* problem, const char* model, std::uint32_t 
             in.energy = energies[k];
             for (int i = 0; i < 3; ++i)
                 in.dir[i] = dirs[3 * k + i];
-            ReplayUniform rep{uniforms + std::size_t(k) * stride};
+            ReplayUniform rep{uniforms + std::size_t(k) * stride, 0, stride};
             Interaction r = sample_model(model, in, *st, rep);
             double* o = out + 17 * std::size_t(k);
             o[0] = r.action;

@@ -3,6 +3,7 @@
 //---------------------------------------------------------------------------//
 #include "em.hh"
 
+#include <algorithm>
 #include <cmath>
 #include <stdexcept>
 
@@ -14,7 +15,7 @@ namespace
 {
 constexpr double pi = 3.14159265358979323846;
 constexpr double r_electron = 2.8179403262e-13;  // [cm] (CODATA 2014)
-constexpr double alpha_fine = 1 / 137.035999139;
+constexpr double alpha_fine = 7.2973525693e-3;  // CODATA 2018
 
 double dot3(double const* a, double const* b)
 {
@@ -349,8 +350,243 @@ Interaction eplusgg(InteractorInput const& in, State& s, Engine& rng)
     momentum_direction(eplus_moment, in.dir, E, in.dir, sec[1].dir);
     return r;
 }
+
+//---------------------------------------------------------------------------//
+// BREMSSTRAHLUNG
+//---------------------------------------------------------------------------//
+constexpr double lambdabar_electron = 3.8615926796e-11;  // [cm] (CODATA 2018)
+constexpr double sb_upper_limit = 1e3;  // [MeV] (PhysicsConstants.hh:62-65)
+constexpr double high_energy_limit = 1e8;  // [MeV]
+
+struct Interp
+{
+    std::size_t index;
+    double fraction;
+};
+
+//! Lower grid index and fraction (corecel/grid/FindInterp.hh and
+//! NonuniformGrid::find: lower_bound, stepping back unless exact)
+Interp find_interp(std::vector<double> const& g, double v)
+{
+    if (!(v >= g.front() && v < g.back()))
+        throw std::runtime_error("oracle: value outside interpolation grid");
+    auto it = std::lower_bound(g.begin(), g.end(), v);
+    std::size_t i = static_cast<std::size_t>(it - g.begin());
+    if (*it != v)
+        --i;
+    return {i, (v - g[i]) / (g[i + 1] - g[i])};
+}
+
+double positron_invbeta(double inc_energy, double mass, double gamma_energy)
+{
+    double const e = inc_energy - gamma_energy;
+    return (e + mass) / std::sqrt(e * (e + 2 * mass));
+}
+
+//! BremFinalStateHelper (em/interactor/detail/BremFinalStateHelper.hh)
+template<class Engine>
+Interaction brem_final_state(InteractorInput const& in, double mass,
+                             double gamma_energy, double costheta,
+                             SecondaryP* sec, std::uint32_t start, Engine& rng)
+{
+    PhysicsData const& pd = *in.pd;
+    double const E = in.energy;
+    double const inc_momentum = std::sqrt(E * E + 2 * mass * E);
+    exiting_direction(costheta, in.dir, rng, sec->dir);
+    sec->particle = pd.gamma;
+    sec->energy = gamma_energy;
+    Interaction r;
+    r.energy = E - gamma_energy;
+    momentum_direction(inc_momentum, in.dir, gamma_energy, sec->dir, r.dir);
+    r.sec_start = start;
+    r.num_secondaries = 1;
+    return r;
+}
+
+//! RBEnergySampler (em/interactor/detail/RBEnergySampler.hh), LPM off
+template<class Engine>
+double sample_rb_energy(ElementDef const& el, double electron_density,
+                        double inc_energy, double mass, double kc, Engine& rng)
+{
+    double const total = inc_energy + mass;
+    double const dens = electron_density * migdal_constant() * (total * total);
+    double const tmin = std::min(kc, inc_energy);
+    double const tmax = std::min(high_energy_limit, inc_energy);
+    double const a = tmin * tmin + dens;
+    double const logratio = std::log((1 / a) * (tmax * tmax + dens));
+    double const max_xs = el.rb_factor1 + el.rb_factor2;
+    double k, dsigma;
+    do
+    {
+        k = std::sqrt(a * std::exp(logratio * rng()) - dens);
+        dsigma = rb_dxsec(el, total, k);
+    } while (dsigma < max_xs * rng());
+    return k;
+}
+
+//! Shared body of the SB, relativistic and combined brems interactors
+//! (SeltzerBergerInteractor.hh:142-163, RelativisticBremInteractor.hh,
+//! CombinedBremInteractor.hh:129-170)
+template<class Engine>
+Interaction brems(InteractorInput const& in, State& s, Engine& rng, int which)
+{
+    PhysicsData const& pd = *in.pd;
+    Interaction r;
+    std::uint32_t start;
+    SecondaryP* sec = allocate_secondaries(s, 1, &start);
+    if (!sec)
+    {
+        r.action = Interaction::failed;
+        return r;
+    }
+    MaterialDef const& mat = pd.materials[in.material];
+    ElementDef const& el = pd.elements[mat.elements[in.element].first];
+    double const mass = pd.particles[in.particle].mass;
+    double const kc = pd.cutoff_energy[pd.gamma][in.material];
+    bool const use_rb = which == 1 || (which == 2 && in.energy >= sb_upper_limit);
+    double k;
+    if (use_rb)
+        k = sample_rb_energy(el, mat.electron_density, in.energy, mass, kc, rng);
+    else
+        k = sample_sb_energy(el, mat.electron_density, in.energy, mass, kc,
+                             in.particle == pd.electron ? 0 : 1, rng);
+    double const costheta = tsai_urban(in.energy, mass, rng);
+    return brem_final_state(in, mass, k, costheta, sec, start, rng);
+}
+
 //---------------------------------------------------------------------------//
 }  // namespace
+
+//---------------------------------------------------------------------------//
+double migdal_constant()
+{
+    return 4 * pi * r_electron * (lambdabar_electron * lambdabar_electron);
+}
+
+SBTableO make_sb_table(Json const& j)
+{
+    SBTableO t;
+    auto vec = [&j](char const* k) { return j(k).nums(); };
+    t.x = vec("x");
+    t.y = vec("y");
+    t.values = vec("values");
+    if (t.x.size() < 2 || t.y.size() < 2 || t.values.size() != t.x.size() * t.y.size())
+        throw std::runtime_error("oracle: inconsistent SB table");
+    for (std::size_t i = 0; i < t.x.size(); ++i)
+    {
+        auto first = t.values.begin() + i * t.y.size();
+        t.argmax.push_back(static_cast<std::uint32_t>(
+            std::max_element(first, first + t.y.size()) - first));
+    }
+    return t;
+}
+
+void set_rb_element_data(double electron_mass, ElementDef& el)
+{
+    if (el.z < 5)
+        return;  // Dirac-Fock form factors (Z < 5) are not restated
+    double const ff_el = std::log(184.15) - el.log_z / 3;
+    double const ff_inel = std::log(1194.0) - 2 * el.log_z / 3;
+    double const fc = el.coulomb_correction;
+    double const invz = 1.0 / el.z;
+    double const z13 = el.cbrt_z;
+    double const z23 = z13 * z13;
+    el.rb_fz = el.log_z / 3 + fc;
+    el.rb_factor1 = (ff_el - fc) + ff_inel * invz;
+    el.rb_factor2 = (1 + invz) / 12;
+    el.rb_gamma_factor = 100 * electron_mass / z13;
+    el.rb_epsilon_factor = 100 * electron_mass / z23;
+}
+
+double rb_dxsec(ElementDef const& el, double total_energy, double gamma_energy)
+{
+    if (el.z < 5)
+        throw std::runtime_error("oracle: relativistic brems needs Z >= 5");
+    double const y = gamma_energy / total_energy;
+    double const onemy = 1 - y;
+    double const term0 = onemy + 0.75 * (y * y);
+    double const invz = 1 / static_cast<double>(el.z);
+    double const term1 = y / (total_energy - gamma_energy);
+    double const gam = term1 * el.rb_gamma_factor;
+    double const eps = term1 * el.rb_epsilon_factor;
+    // Tsai's screening functions (RBDiffXsCalculator::compute_screen_functions)
+    double const gam2 = gam * gam;
+    double const eps2 = eps * eps;
+    double const phi1 = 16.863 - 2 * std::log(1 + 0.311877 * gam2)
+                        + 2.4 * std::exp(-0.9 * gam) + 1.6 * std::exp(-1.5 * gam);
+    double const phi2 = 2 / (3 + 19.5 * gam + 18 * gam2);
+    double const psi1 = 24.34 - 2 * std::log(1 + 13.111641 * eps2)
+                        + 2.8 * std::exp(-8 * eps) + 1.2 * std::exp(-29.2 * eps);
+    double const psi2 = 2 / (3 + 120 * eps + 1200 * eps2);
+    double const dxsec
+        = term0 * ((0.25 * phi1 - el.rb_fz) + (0.25 * psi1 - 2 * el.log_z / 3) * invz)
+          + 0.125 * onemy * (phi2 + psi2 * invz);
+    return std::max(dxsec, 0.0);
+}
+
+double sb_positron_scale(int z, double mass, double kc, double inc_energy,
+                         double exit_energy)
+{
+    double const alpha_z = 2 * pi * alpha_fine * z;
+    double const delta = positron_invbeta(inc_energy, mass, kc)
+                         - positron_invbeta(inc_energy, mass, exit_energy);
+    return std::exp(alpha_z * std::min(delta, 0.0));
+}
+
+template<class Engine>
+double sample_sb_energy(ElementDef const& el, double electron_density,
+                        double inc_energy, double mass, double kc,
+                        int corrector, Engine& rng, double* max_xs_out)
+{
+    SBTableO const& t = el.sb;
+    if (!t)
+        throw std::runtime_error("oracle: element has no SB table");
+    std::size_t const ny = t.y.size();
+    auto at = [&t, ny](std::size_t ix, std::size_t iy) { return t.values[ix * ny + iy]; };
+    double const total = inc_energy + mass;
+    double const dens = electron_density * migdal_constant() * (total * total);
+    Interp const xl = find_interp(t.x, std::log(inc_energy));
+    double const max_xs = (1 - xl.fraction) * at(xl.index, t.argmax[xl.index])
+                          + xl.fraction * at(xl.index + 1, t.argmax[xl.index + 1]);
+    if (max_xs_out)
+        *max_xs_out = max_xs;
+    double const inv_e = 1 / inc_energy;
+    double const a = kc * kc + dens;
+    double const logratio = std::log((1 / a) * (inc_energy * inc_energy + dens));
+    double const alpha_z = 2 * pi * alpha_fine * el.z;
+    double const cutoff_invbeta
+        = corrector == 1 ? positron_invbeta(inc_energy, mass, kc) : 0;
+    double exit, xs;
+    do
+    {
+        exit = std::sqrt(a * std::exp(logratio * rng()) - dens);
+        Interp const yl = find_interp(t.y, exit * inv_e);
+        double const v
+            = (1 - xl.fraction)
+                  * ((1 - yl.fraction) * at(xl.index, yl.index)
+                     + yl.fraction * at(xl.index, yl.index + 1))
+              + xl.fraction
+                    * ((1 - yl.fraction) * at(xl.index + 1, yl.index)
+                       + yl.fraction * at(xl.index + 1, yl.index + 1));
+        double scale = 1;
+        if (corrector == 2)
+        {
+            scale = 0.5;  // constant scaling (SeltzerBerger.test.cc:250-262)
+        }
+        else if (corrector == 1)
+        {
+            double const delta
+                = cutoff_invbeta - positron_invbeta(inc_energy, mass, exit);
+            scale = std::exp(alpha_z * std::min(delta, 0.0));
+        }
+        xs = v * scale;
+    } while (xs < max_xs * rng());
+    return exit;
+}
+
+template double sample_sb_energy<MtUniform>(ElementDef const&, double, double,
+                                            double, double, int, MtUniform&,
+                                            double*);
 
 //---------------------------------------------------------------------------//
 void make_unit(double* v)
@@ -418,6 +654,12 @@ Interaction sample_model(std::string const& model, InteractorInput const& in,
         return moller_bhabha(in, s, rng);
     if (model == "eplusgg")
         return eplusgg(in, s, rng);
+    if (model == "seltzer_berger")
+        return brems(in, s, rng, 0);
+    if (model == "relativistic_brem")
+        return brems(in, s, rng, 1);
+    if (model == "combined_brem")
+        return brems(in, s, rng, 2);
     throw std::runtime_error("oracle: unsupported model " + model);
 }
 
@@ -487,15 +729,27 @@ double coulomb_correction(int z)
     // Davies-Bethe-Maximon: f(Z) = a^2 sum_n 1 / (n (n^2 + a^2)), written as
     // a^2 [1/(1+a^2) + sum_k (-a^2)^k (zeta(2k+3) - 1)]
     double const a2 = (alpha_fine * z) * (alpha_fine * z);
+    // zeta(n) - 1: partial sum below m = 40 plus the Euler-Maclaurin tail
+    auto zeta_minus_one = [](int n) {
+        double s = 0;
+        for (int m = 39; m >= 2; --m)
+            s += std::pow(m, -n);
+        double const x = 40;
+        double const f = std::pow(x, -n);
+        double const d = n;
+        double const tail = x * f / (d - 1) + f / 2 + d * f / (12 * x)
+                            - d * (d + 1) * (d + 2) * f / (720 * std::pow(x, 3))
+                            + d * (d + 1) * (d + 2) * (d + 3) * (d + 4) * f
+                                  / (30240 * std::pow(x, 5))
+                            - d * (d + 1) * (d + 2) * (d + 3) * (d + 4) * (d + 5)
+                                  * (d + 6) * f / (1209600 * std::pow(x, 7));
+        return tail + s;
+    };
     double fz = 1 / (1 + a2);
     double apow = 1;
     for (int k = 0; k < 16; ++k)
     {
-        int n = 2 * k + 3;
-        double zeta_minus_one = 0;
-        for (int m = 2; m < 200; ++m)
-            zeta_minus_one += std::pow(m, -n);
-        fz += apow * zeta_minus_one;
+        fz += apow * zeta_minus_one(2 * k + 3);
         apow *= -a2;
     }
     return fz * a2;

@@ -19,6 +19,7 @@
 
 #include <cstdint>
 #include <random>
+#include <stdexcept>
 
 #include "physics.hh"
 
@@ -80,7 +81,13 @@ struct ReplayUniform
 {
     double const* u;
     std::uint32_t i = 0;
-    double operator()() { return u[i++]; }
+    std::uint32_t size = 0xffffffffu;  //!< uniforms available to this sample
+    double operator()()
+    {
+        if (i >= size)
+            throw std::runtime_error("oracle: replayed uniform stream exhausted");
+        return u[i++];
+    }
 };
 
 //! Inputs common to all interactors
@@ -115,6 +122,27 @@ std::uint32_t sample_element(State& s, Track& t, ProcessDef const& p,
 //! Element constants derived from Z (mat/detail/Utils.cc equivalents)
 double coulomb_correction(int z);
 double mass_radiation_coeff(int z, double atomic_mass);
+
+//---------------------------------------------------------------------------//
+// Bremsstrahlung (SeltzerBerger, RelativisticBrem, CombinedBrem)
+//! Build an SB table with its per-x argmax (SeltzerBergerModel.cc:append_table)
+SBTableO make_sb_table(Json const& j);
+//! Relativistic brems element data (RelativisticBremModel.cc:150-185)
+void set_rb_element_data(double electron_mass, ElementDef& el);
+//! Relativistic DCS without LPM (RBDiffXsCalculator::dxsec_per_atom)
+double rb_dxsec(ElementDef const& el, double total_energy, double gamma_energy);
+//! Positron DCS scaling (SBPositronXsCorrector)
+double sb_positron_scale(int z, double mass, double kc, double inc_energy,
+                         double exit_energy);
+//! SB photon energy sampler (SBEnergySampler / SBEnergyDistribution);
+//! corrector 0 = electron, 1 = positron (SBPositronXsCorrector), 2 = a
+//! constant one-half; also reports the bounding max_xs
+template<class Engine>
+double sample_sb_energy(ElementDef const& el, double electron_density,
+                        double inc_energy, double mass, double kc,
+                        int corrector, Engine& rng, double* max_xs = nullptr);
+//! Migdal constant 4 pi r_e lambdabar_e^2 [cm^3]
+double migdal_constant();
 
 //---------------------------------------------------------------------------//
 // Math helpers shared by interactors (corecel/math/ArrayUtils.hh)

@@ -232,6 +232,10 @@ void build_physics(Json const& inp, PhysicsData& out)
             d.log_z = std::log(static_cast<double>(d.z));
             d.coulomb_correction = coulomb_correction(d.z);
             d.mass_radiation_coeff = mass_radiation_coeff(d.z, d.atomic_mass);
+            if (auto sb = e.find("sb_table"))
+                d.sb = make_sb_table(*sb);
+            if (out.electron != none)
+                set_rb_element_data(out.particles[out.electron].mass, d);
             out.elements.push_back(d);
         }
     }

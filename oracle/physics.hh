@@ -50,6 +50,15 @@ struct ParticleDef
     double charge = 0;
 };
 
+//! Seltzer-Berger scaled DCS table: x = log(E/MeV), y = k/E, values x-major
+struct SBTableO
+{
+    std::vector<double> x, y, values;
+    std::vector<std::uint32_t> argmax;  // per x: first y index of the max
+
+    explicit operator bool() const { return !x.empty(); }
+};
+
 struct ElementDef
 {
     int z = 0;
@@ -58,6 +67,10 @@ struct ElementDef
     double log_z = 0;
     double coulomb_correction = 0;
     double mass_radiation_coeff = 0;  // [cm^2/g]
+    SBTableO sb;
+    // Relativistic brems element data (RelativisticBremModel.cc:150-185)
+    double rb_fz = 0, rb_factor1 = 0, rb_factor2 = 0;
+    double rb_gamma_factor = 0, rb_epsilon_factor = 0;
 };
 
 struct MaterialDef

@@ -1,0 +1,48 @@
+"""Material setups of the reference's bremsstrahlung interactor tests.
+
+- SeltzerBerger.test.cc:40-100: Cu (0.141 mol/cm^3), br29 table, gamma
+  cutoff 0.02064384 MeV.
+- RelativisticBrem.test.cc:40-95: Pb (0.05477 mol/cm^3), gamma cutoff
+  0.0945861 MeV.
+"""
+
+from __future__ import annotations
+
+from tests import g4emlow
+from tests import problems as pb
+
+AVOGADRO = 6.02214076e23
+EMASS = pb.EMASS
+
+
+def _particles():
+    return [
+        {"label": "electron", "pdg": 11, "mass": EMASS, "charge": -1.0},
+        {"label": "positron", "pdg": -11, "mass": EMASS, "charge": 1.0},
+        {"label": "gamma", "pdg": 22, "mass": 0.0, "charge": 0.0},
+    ]
+
+
+def single_element(z, amass, mol_cc, gamma_cut, label, sb_table=None, model=None):
+    elem = {"z": z, "atomic_mass": amass}
+    if sb_table is not None:
+        elem["sb_table"] = sb_table
+    nd = mol_cc * AVOGADRO
+    phys = {
+        "particles": _particles(),
+        "elements": [elem],
+        "materials": [{"label": label, "number_density": nd, "electron_density": z * nd,
+                       "elements": [{"element": 0, "fraction": 1.0}]}],
+        "cutoffs": {"gamma": [{"energy": gamma_cut}], "electron": [{"energy": 0.001}]},
+    }
+    return {"geometry": pb.load_geometry("two-boxes.org.json"),
+            "volume_to_material": [-1, 0, 0], "physics": phys}
+
+
+def sb_copper():
+    return single_element(29, 63.546, 0.141, 0.02064384, "Cu",
+                          sb_table=g4emlow.read_sb_table(29))
+
+
+def rb_lead():
+    return single_element(82, 207.2, 0.05477, 0.0945861, "Pb")

@@ -48,10 +48,11 @@ def test_interactor_parity(amd, setup, model, particle, erange, material):
     energies = np.exp(rng.uniform(np.log(erange[0]), np.log(erange[1]), N))
     dirs = random_dirs(rng, N)
     uniforms = ob.mt_uniforms(1234 + particle, N * STRIDE)
-    gpu = amd.verify_interactor(params, state, model, particle, energies, dirs, uniforms,
-                                STRIDE, material=material)
+    # Oracle first: it raises if a sample would read past its STRIDE uniforms
     cpu = ob.interactor_replay_batch(oprob, model, particle, energies, dirs, uniforms,
                                      STRIDE, material=material)
+    gpu = amd.verify_interactor(params, state, model, particle, energies, dirs, uniforms,
+                                STRIDE, material=material)
     np.testing.assert_array_equal(gpu[:, 0], cpu[:, 0])  # action
     np.testing.assert_array_equal(gpu[:, 6], cpu[:, 6])  # secondary count
     np.testing.assert_array_equal(gpu[:, 7], cpu[:, 7])  # secondary particle
