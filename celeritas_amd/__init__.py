@@ -463,3 +463,22 @@ def verify_interactor(params: CoreParams, state: CoreState, model: str, particle
     if rc != 0:
         raise CeleritasError(f"celer_verify_interactor failed with HIP error {rc}")
     return out
+
+
+def generate_physics(spec: dict | str) -> dict:
+    """Expand a compact physics description into full tables (host only).
+
+    The result is the "physics" section of a problem, with cross-section,
+    stopping-power, range, Seltzer-Berger and Livermore tables.
+    """
+    lib_ = _load()
+    f = lib_.celer_generate_physics
+    f.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t,
+                  ctypes.POINTER(ctypes.c_size_t)]
+    f.restype = ctypes.c_int
+    text = (spec if isinstance(spec, str) else json.dumps(spec)).encode()
+    needed = ctypes.c_size_t(0)
+    _check(f(text, None, 0, ctypes.byref(needed)))
+    buf = ctypes.create_string_buffer(needed.value)
+    _check(f(text, buf, needed.value, ctypes.byref(needed)))
+    return json.loads(buf.value.decode())

@@ -11,6 +11,7 @@
 
 #include "global/CoreParams.hh"
 #include "global/CoreState.hh"
+#include "phys/TableGen.hh"
 #include "global/Stepper.hh"
 #include "track/TrackActions.hh"
 #include "track/TrackKernels.hh"
@@ -483,6 +484,19 @@ int celer_hip_reseed(const void* params_ref,
     auto const& s = *static_cast<CoreStateRef const*>(state_ref);
     return launch_reseed_rng(s.rng, p.rng_jump, p.rng_seed, event_id,
                              static_cast<hipStream_t>(stream));
+}
+
+int celer_generate_physics(const char* spec_json, char* buf, size_t len, size_t* needed)
+{
+    return guarded([&] {
+        std::string s = dump_json(generate_physics(parse_json(spec_json)));
+        if (needed)
+            *needed = s.size() + 1;
+        CELER_VALIDATE(buf == nullptr || len > s.size(),
+                       << "output buffer too small (" << s.size() + 1 << " bytes needed)");
+        if (buf)
+            std::memcpy(buf, s.c_str(), s.size() + 1);
+    });
 }
 
 int celer_params_export_json(const CelerParams*, char* buf, size_t len, size_t* needed)

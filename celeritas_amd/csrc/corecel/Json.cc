@@ -6,6 +6,7 @@
 #include "Json.hh"
 
 #include <cctype>
+#include <cstdio>
 #include <cstdlib>
 
 #include "DeviceRuntime.hh"
@@ -271,6 +272,119 @@ std::vector<long long> JsonValue::as_ints() const
     for (auto const& v : arr)
         result.push_back(v.as_int());
     return result;
+}
+
+//---------------------------------------------------------------------------//
+namespace
+{
+void dump_to(JsonValue const& v, std::string& out)
+{
+    switch (v.type)
+    {
+        case JsonValue::Type::null:
+            out += "null";
+            break;
+        case JsonValue::Type::boolean:
+            out += v.b ? "true" : "false";
+            break;
+        case JsonValue::Type::number: {
+            char buf[32];
+            std::snprintf(buf, sizeof(buf), "%.17g", v.num);
+            out += buf;
+            break;
+        }
+        case JsonValue::Type::string:
+            out += '"';
+            for (char c : v.str)
+            {
+                if (c == '"' || c == '\\')
+                    out += '\\';
+                out += c;
+            }
+            out += '"';
+            break;
+        case JsonValue::Type::array:
+            out += '[';
+            for (std::size_t i = 0; i < v.arr.size(); ++i)
+            {
+                if (i)
+                    out += ',';
+                dump_to(v.arr[i], out);
+            }
+            out += ']';
+            break;
+        case JsonValue::Type::object: {
+            out += '{';
+            bool first = true;
+            for (auto const& kv : v.obj)
+            {
+                if (!first)
+                    out += ',';
+                first = false;
+                out += '"';
+                out += kv.first;
+                out += "\":";
+                dump_to(kv.second, out);
+            }
+            out += '}';
+            break;
+        }
+    }
+}
+}  // namespace
+
+std::string dump_json(JsonValue const& v)
+{
+    std::string out;
+    dump_to(v, out);
+    return out;
+}
+
+JsonValue JsonValue::number(double x)
+{
+    JsonValue v;
+    v.type = Type::number;
+    v.num = x;
+    return v;
+}
+
+JsonValue JsonValue::string(std::string s)
+{
+    JsonValue v;
+    v.type = Type::string;
+    v.str = std::move(s);
+    return v;
+}
+
+JsonValue JsonValue::array(std::vector<double> const& xs)
+{
+    JsonValue v;
+    v.type = Type::array;
+    for (double x : xs)
+        v.arr.push_back(number(x));
+    return v;
+}
+
+JsonValue JsonValue::array()
+{
+    JsonValue v;
+    v.type = Type::array;
+    return v;
+}
+
+JsonValue JsonValue::object()
+{
+    JsonValue v;
+    v.type = Type::object;
+    return v;
+}
+
+JsonValue JsonValue::boolean(bool x)
+{
+    JsonValue v;
+    v.type = Type::boolean;
+    v.b = x;
+    return v;
 }
 
 //---------------------------------------------------------------------------//

@@ -60,10 +60,23 @@ class JsonValue
 
     std::vector<double> as_doubles() const;
     std::vector<long long> as_ints() const;
+
+    //!@{
+    //! Construction helpers
+    static JsonValue number(double x);
+    static JsonValue string(std::string s);
+    static JsonValue boolean(bool x);
+    static JsonValue array(std::vector<double> const& xs);
+    static JsonValue array();
+    static JsonValue object();
+    //!@}
 };
 
 //! Parse a JSON document; throws RuntimeError with a position on error
 JsonValue parse_json(std::string const& text);
+
+//! Serialize a JSON value (numbers with 17 significant digits)
+std::string dump_json(JsonValue const& v);
 
 //---------------------------------------------------------------------------//
 }  // namespace celeritas_amd

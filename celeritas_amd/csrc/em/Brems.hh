@@ -88,6 +88,12 @@ rb_dxsec(ElementRecord const& el, double total_energy, double gamma_energy)
     double const y = gamma_energy / total_energy;
     double const onemy = 1 - y;
     double const term0 = onemy + 0.75 * (y * y);
+    if (el.z < 5)
+    {
+        // Dirac-Fock model
+        double const d = term0 * el.rb_factor1 + onemy * el.rb_factor2;
+        return d > 0 ? d : 0;
+    }
     double const invz = 1 / el.z;
     double const term1 = y / (total_energy - gamma_energy);
     double const gam = term1 * el.rb_gamma_factor;

@@ -10,6 +10,7 @@
 #include "Brems.hh"
 #include "InteractorCommon.hh"
 #include "KleinNishina.hh"
+#include "LivermorePE.hh"
 #include "StandardEm.hh"
 
 namespace celeritas_amd
@@ -75,6 +76,16 @@ CELER_FUNCTION Interaction sample_interaction(ModelClass model,
                                         energy, dir, rng);
         case ModelClass::eplusgg:
             return sample_eplusgg(p, ctx.state, energy, dir, rng);
+        case ModelClass::livermore_pe: {
+            // No element table: multi-element materials select here
+            MaterialRecord const& m = p.materials[ctx.material];
+            std::uint32_t comp = ctx.element;
+            if (m.num_elements > 1)
+                comp = select_pe_element(p, ctx.material, energy, rng);
+            return sample_livermore_pe(p, ctx.state,
+                                       p.components[m.elem_begin + comp].element,
+                                       energy, dir, rng);
+        }
         case ModelClass::seltzer_berger:
             return sample_brems(p, ctx.state, ctx.particle, ctx.material,
                                 ctx.element, energy, dir, 0, rng);

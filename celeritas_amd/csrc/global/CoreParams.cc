@@ -5,6 +5,7 @@
 //---------------------------------------------------------------------------//
 #include "CoreParams.hh"
 
+#include "phys/TableGen.hh"
 #include "track/TrackActions.hh"
 #include "CoreActions.hh"
 
@@ -97,7 +98,12 @@ std::shared_ptr<CoreParams> CoreParams::from_json(std::string const& text)
     // Physics first: its actions take the lowest IDs as in the reference
     JsonValue empty;
     empty.type = JsonValue::Type::object;
-    JsonValue const& phys_doc = doc.has("physics") ? doc.at("physics") : empty;
+    JsonValue const& phys_in = doc.has("physics") ? doc.at("physics") : empty;
+    // A compact description ({"generate": ...}) is expanded into tables
+    JsonValue generated;
+    if (phys_in.has("generate"))
+        generated = generate_physics(phys_in.at("generate"));
+    JsonValue const& phys_doc = phys_in.has("generate") ? generated : phys_in;
     inp.physics = std::make_shared<PhysicsParams>(phys_doc, inp.action_reg.get());
 
     // Along-step action for charged particles

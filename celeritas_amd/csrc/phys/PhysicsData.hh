@@ -122,6 +122,34 @@ struct ElementRecord
     std::uint32_t sb_ny;
 };
 
+//! Tabulated function on a nonuniform grid: offsets into reals
+struct GenericGrid
+{
+    std::uint32_t x;
+    std::uint32_t y;
+    std::uint32_t size;  //!< 0: absent
+    std::uint32_t pad_;
+};
+
+//! Livermore photoelectric subshell (em/data/LivermorePEData.hh)
+struct LivermoreShell
+{
+    double binding_energy;  //!< [MeV]
+    double param[2][6];  //!< cumulative 1/E fits below/above thresh_hi [b]
+    GenericGrid xs;  //!< sigma * E^3 [b MeV^3] vs E [MeV]
+};
+
+//! Livermore photoelectric element data
+struct LivermoreElement
+{
+    GenericGrid xs_lo;  //!< total sigma E^3 below the K edge
+    GenericGrid xs_hi;  //!< total sigma E^3 from the K edge to thresh_lo
+    double thresh_lo;  //!< [MeV]
+    double thresh_hi;  //!< [MeV]
+    std::uint32_t shell_begin;
+    std::uint32_t num_shells;  //!< 0: no data
+};
+
 //! Component of a material
 struct ElementComponent
 {
@@ -199,6 +227,8 @@ struct PhysicsParamsRef
     ElementComponent const* components{nullptr};
     ElementRecord const* elements{nullptr};
     std::uint32_t const* sb_argmax{nullptr};  //!< per-x argmax of SB tables
+    LivermoreElement const* pe_elements{nullptr};  //!< per element
+    LivermoreShell const* pe_shells{nullptr};
     ModelData model{};
 };
 

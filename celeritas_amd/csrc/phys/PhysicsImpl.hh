@@ -9,6 +9,7 @@
 #include <cmath>
 
 #include "global/CoreData.hh"
+#include "LivermoreXs.hh"
 
 namespace celeritas_amd
 {
@@ -84,6 +85,8 @@ CELER_FUNCTION double calc_xs(PhysicsParamsRef const& p,
         return eplusgg_macro_xs(p.model.electron_mass,
                                 p.materials[mat].electron_density, energy);
     }
+    if (proc.process_class == static_cast<std::uint32_t>(ProcessClass::photoelectric))
+        return livermore_pe_macro_xs(p, mat, energy);
     return 0;
 }
 

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
 
 #include "global/ActionInterface.hh"
 #include "PhysicsActions.hh"
@@ -245,6 +246,66 @@ struct PhysicsParams::HostData
     DeviceVector<ElementRecord> d_elements;
     std::vector<std::uint32_t> sb_argmax;
     DeviceVector<std::uint32_t> d_sb_argmax;
+    std::vector<LivermoreElement> pe_elements;
+    std::vector<LivermoreShell> pe_shells;
+    DeviceVector<LivermoreElement> d_pe_elements;
+    DeviceVector<LivermoreShell> d_pe_shells;
+
+    //! Append a tabulated function {x, y} (increasing x)
+    GenericGrid add_generic(JsonValue const& j)
+    {
+        auto x = j.at("x").as_doubles();
+        auto y = j.at("y").as_doubles();
+        CELER_VALIDATE(x.size() >= 2 && x.size() == y.size()
+                           && std::is_sorted(x.begin(), x.end()),
+                       << "invalid tabulated function");
+        GenericGrid g{};
+        g.x = static_cast<std::uint32_t>(reals.size());
+        reals.insert(reals.end(), x.begin(), x.end());
+        g.y = static_cast<std::uint32_t>(reals.size());
+        reals.insert(reals.end(), y.begin(), y.end());
+        g.size = static_cast<std::uint32_t>(x.size());
+        return g;
+    }
+
+    /*!
+     * Append Livermore photoelectric data {thresh_lo, thresh_hi, xs_lo,
+     * xs_hi, shells: [{binding_energy, param_lo, param_hi, x, y}]} with
+     * shells in order of decreasing binding energy
+     * (em/model/detail/LivermoreXsInserter.hh).
+     */
+    void append_pe(JsonValue const* j)
+    {
+        LivermoreElement el{};
+        el.shell_begin = static_cast<std::uint32_t>(pe_shells.size());
+        if (j)
+        {
+            el.thresh_lo = j->at("thresh_lo").as_double();
+            el.thresh_hi = j->at("thresh_hi").as_double();
+            CELER_VALIDATE(el.thresh_lo <= el.thresh_hi, << "bad Livermore thresholds");
+            if (j->has("xs_lo"))
+                el.xs_lo = add_generic(j->at("xs_lo"));
+            el.xs_hi = add_generic(j->at("xs_hi"));
+            for (auto const& sj : j->at("shells").arr)
+            {
+                LivermoreShell sh{};
+                sh.binding_energy = sj.at("binding_energy").as_double();
+                auto lo = sj.at("param_lo").as_doubles();
+                auto hi = sj.at("param_hi").as_doubles();
+                CELER_VALIDATE(lo.size() == 6 && hi.size() == 6,
+                               << "Livermore fits need 6 parameters");
+                std::copy(lo.begin(), lo.end(), sh.param[0]);
+                std::copy(hi.begin(), hi.end(), sh.param[1]);
+                CELER_VALIDATE(sh.binding_energy <= el.thresh_lo,
+                               << "binding energy above thresh_lo");
+                sh.xs = add_generic(sj);
+                pe_shells.push_back(sh);
+            }
+            el.num_shells = static_cast<std::uint32_t>(pe_shells.size()) - el.shell_begin;
+            CELER_VALIDATE(el.num_shells > 0, << "Livermore data without shells");
+        }
+        pe_elements.push_back(el);
+    }
 
     //! Append grids (one per material); returns first index or invalid
     std::uint32_t add_grids(std::vector<GridHost> const& gs)
@@ -351,13 +412,17 @@ PhysicsParams::PhysicsParams(JsonValue const& inp, ActionRegistry* reg)
             r.coulomb_correction = z > 0 ? calc_coulomb_correction(z) : 0;
             r.mass_radiation_coeff
                 = z > 0 ? calc_mass_radiation_coeff(z, r.atomic_mass) : 0;
-            if (z >= 5 && ref_.electron_id != invalid_id)
+            if (z >= 1 && ref_.electron_id != invalid_id)
             {
-                // Thomas-Fermi form factors (the Dirac-Fock values for
-                // Z < 5 are not provided)
+                // Thomas-Fermi form factors; Dirac-Fock values for Z < 5
+                // (RelativisticBremModel.cc:150-200)
+                static double const dirac_fock[4][2] = {
+                    {5.3104, 5.9173}, {4.7935, 5.6125}, {4.7402, 5.5377}, {4.7112, 5.4728}};
                 double const me = particles_[ref_.electron_id].mass;
-                double const ff_el = std::log(184.15) - r.log_z / 3;
-                double const ff_inel = std::log(1194.0) - 2 * r.log_z / 3;
+                double const ff_el = z < 5 ? dirac_fock[z - 1][0]
+                                           : std::log(184.15) - r.log_z / 3;
+                double const ff_inel = z < 5 ? dirac_fock[z - 1][1]
+                                             : std::log(1194.0) - 2 * r.log_z / 3;
                 double const fc = r.coulomb_correction;
                 double const invz = 1.0 / z;
                 r.rb_fz = r.log_z / 3 + fc;
@@ -368,6 +433,7 @@ PhysicsParams::PhysicsParams(JsonValue const& inp, ActionRegistry* reg)
             }
             if (e.has("sb_table"))
                 h.append_sb_table(e.at("sb_table"), r);
+            h.append_pe(e.has("livermore_pe") ? &e.at("livermore_pe") : nullptr);
             h.elements.push_back(r);
         }
     }
@@ -463,6 +529,7 @@ PhysicsParams::PhysicsParams(JsonValue const& inp, ActionRegistry* reg)
         pp.dedx_grid = pp.range_grid = pp.inv_range_grid = pp.msc_grid = invalid_id;
     }
     std::uint32_t num_models = 0;
+    std::map<std::string, std::pair<std::uint32_t, ModelClass>> model_actions;
     for (JsonValue const* pj : procs)
     {
         std::string const plabel = pj->get_str("particle", "");
@@ -477,6 +544,13 @@ PhysicsParams::PhysicsParams(JsonValue const& inp, ActionRegistry* reg)
             process_from_string(pj->get_str("class", "")));
         rec.integral = pj->get_bool("integral", false) ? 1 : 0;
         rec.elem_cdf = invalid_id;
+        if (rec.process_class == static_cast<std::uint32_t>(ProcessClass::photoelectric))
+        {
+            for (auto const& c : h.components)
+                CELER_VALIDATE(h.pe_elements.at(c.element).num_shells > 0,
+                               << "photoelectric process needs Livermore data for "
+                                  "every element in use");
+        }
 
         auto xs = grids_of(*pj, "xs", "lambda", "lambda_prim", num_materials_);
         rec.xs_grid = h.add_grids(xs);
@@ -520,11 +594,23 @@ PhysicsParams::PhysicsParams(JsonValue const& inp, ActionRegistry* reg)
         {
             auto const& mj = models[k];
             ModelClass mc = model_from_string(mj.at("model").as_string());
+            std::string const mlabel = mj.at("label").as_string();
             rec.models[k].energy_low = mj.get("low", 0.0);
             rec.models[k].model = static_cast<std::uint32_t>(mc);
+            // One model (and action) may serve several particles, as the
+            // reference's MollerBhabha and brems models serve e- and e+
+            auto found = model_actions.find(mlabel);
+            if (found != model_actions.end())
+            {
+                CELER_VALIDATE(found->second.second == mc,
+                               << "model '" << mlabel << "' reused with another class");
+                rec.models[k].action = found->second.first;
+                continue;
+            }
             rec.models[k].action = reg->next_id();
+            model_actions[mlabel] = {reg->next_id(), mc};
             reg->insert(std::make_shared<ModelAction>(
-                reg->next_id(), mj.at("label").as_string(), static_cast<std::uint32_t>(mc)));
+                reg->next_id(), mlabel, static_cast<std::uint32_t>(mc)));
             ++num_models;
         }
         process_labels_.push_back(pj->get_str("label", ""));
@@ -557,6 +643,10 @@ PhysicsParams::PhysicsParams(JsonValue const& inp, ActionRegistry* reg)
     h.d_elements = to_device(h.elements);
     h.d_sb_argmax = to_device(h.sb_argmax);
     ref_.sb_argmax = h.d_sb_argmax.data();
+    h.d_pe_elements = to_device(h.pe_elements);
+    h.d_pe_shells = to_device(h.pe_shells);
+    ref_.pe_elements = h.d_pe_elements.data();
+    ref_.pe_shells = h.d_pe_shells.data();
     ref_.num_particles = this->num_particles();
     ref_.num_materials = num_materials_;
     ref_.particle = h.d_particle.data();

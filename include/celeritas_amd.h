@@ -170,6 +170,16 @@ uint32_t celer_params_find_action(const CelerParams* params, const char* label);
 int celer_params_action_order(const CelerParams* params, uint32_t action_id);
 uint32_t celer_params_find_particle(const CelerParams* params, int pdg);
 uint32_t celer_params_num_volumes(const CelerParams* params);
+/*
+ * Expand a compact physics description (elements, materials, cutoffs,
+ * options) into a full "physics" input with cross-section, stopping-power,
+ * range, Seltzer-Berger and Livermore tables. Host only (no device needed).
+ * Replaces the reference's Geant4 import feeding PhysicsParams::build_xs
+ * (celeritas/phys/PhysicsParams.cc:445-636, io/ImportPhysicsTable.hh).
+ * With buf == NULL only *needed (bytes incl. NUL) is set.
+ */
+int celer_generate_physics(const char* spec_json, char* buf, size_t len,
+                           size_t* needed);
 /* Serialize host-side tables (physics, geometry) as JSON for checkers */
 int celer_params_export_json(const CelerParams* params, char* buf, size_t len,
                              size_t* needed);

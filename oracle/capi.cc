@@ -593,7 +593,7 @@ int orc_sb_sample_many(void* problem, void* mt, std::uint32_t material,
 int orc_interactor_stress(void* problem, void* mt, const char* model,
                           std::uint32_t particle, double energy, double const* dir,
                           std::uint32_t material, std::uint32_t n,
-                          unsigned long long* count)
+                          unsigned long long* count, double* sums)
 {
     return guard([&] {
         auto prob = static_cast<OrcProblem*>(problem)->p;
@@ -608,14 +608,28 @@ int orc_interactor_stress(void* problem, void* mt, const char* model,
         make_unit(in.dir);
         in.material = material;
         MtUniform u{*static_cast<std::mt19937*>(mt)};
+        // sums: secondary count, energy, cosine to the (unnormalized) dir
         for (std::uint32_t k = 0; k < n; ++k)
         {
             Interaction r = sample_model(model, in, *st, u);
             if (r.action == Interaction::failed)
                 throw std::runtime_error("interaction failed in stress loop");
+            for (std::uint32_t j = 0; sums && j < r.num_secondaries; ++j)
+            {
+                SecondaryP const& sec = st->stack[r.sec_start + j];
+                sums[0] += 1;
+                sums[1] += sec.energy;
+                sums[2] += dir[0] * sec.dir[0] + dir[1] * sec.dir[1] + dir[2] * sec.dir[2];
+            }
         }
         *count = u.count;
     });
+}
+
+double orc_pe_macro_xs(void* problem, std::uint32_t material, double energy)
+{
+    auto const& pd = *static_cast<OrcProblem*>(problem)->p->physics;
+    return livermore_pe_macro_xs(pd, material, energy);
 }
 
 double orc_rb_dxsec(void* problem, std::uint32_t element, double total_energy,

@@ -455,7 +455,193 @@ Interaction brems(InteractorInput const& in, State& s, Engine& rng, int which)
 }
 
 //---------------------------------------------------------------------------//
+// LIVERMORE PHOTOELECTRIC (LivermorePEInteractor.hh:145-358)
+//---------------------------------------------------------------------------//
+double poly5(double const* a, double x)
+{
+    double r = std::fma(x, a[5], a[4]);
+    for (int i = 3; i >= 0; --i)
+        r = std::fma(x, r, a[i]);
+    return r;
+}
+
+template<class Engine>
+Interaction livermore_pe(InteractorInput const& in, State& s, Engine& rng)
+{
+    PhysicsData const& pd = *in.pd;
+    Interaction r;
+    std::uint32_t start;
+    SecondaryP* sec = allocate_secondaries(s, 1, &start);
+    if (!sec)
+    {
+        r.action = Interaction::failed;
+        return r;
+    }
+    MaterialDef const& mat = pd.materials[in.material];
+    double const E = in.energy;
+    std::size_t comp = in.element;
+    if (mat.elements.size() > 1)
+    {
+        // ElementSelector (mat/ElementSelector.hh) by micro cross section
+        double total = 0;
+        for (auto const& c : mat.elements)
+            total += c.second * livermore_pe_micro_xs(pd.elements[c.first], E);
+        double accum = -total * rng();
+        std::size_t i = 0;
+        for (; i + 1 < mat.elements.size(); ++i)
+        {
+            accum += mat.elements[i].second
+                     * livermore_pe_micro_xs(pd.elements[mat.elements[i].first], E);
+            if (accum > 0)
+                break;
+        }
+        comp = i;
+    }
+    ElementDef const& el = pd.elements[mat.elements[comp].first];
+    LivermoreO const& pe = el.pe;
+    double const inv_energy = 1 / E;
+    std::size_t shell = 0;
+    double const cutoff = rng() * livermore_pe_micro_xs(el, E);
+    if (E < pe.thresh_lo)
+    {
+        double xs = 0;
+        double const inv_cube = inv_energy * inv_energy * inv_energy;
+        for (; shell < pe.shells.size(); ++shell)
+        {
+            if (E < pe.shells[shell].binding)
+                continue;
+            xs += inv_cube * pe.shells[shell].xs(E);
+            if (xs > cutoff)
+                break;
+        }
+    }
+    else
+    {
+        int const pidx = E < pe.thresh_hi ? 0 : 1;
+        for (; shell + 1 < pe.shells.size(); ++shell)
+        {
+            double xs = inv_energy * poly5(pe.shells[shell].param[pidx], inv_energy);
+            if (xs > cutoff)
+                break;
+        }
+    }
+    r.action = Interaction::absorbed;
+    r.energy = 0;
+    r.sec_start = start;
+    r.num_secondaries = 1;
+    if (shell == pe.shells.size())
+    {
+        r.energy_deposition = E;
+        r.num_secondaries = 0;
+        return r;
+    }
+    double const binding = pe.shells[shell].binding;
+    sec->particle = pd.electron;
+    sec->energy = E - binding;
+    // Sauter-Gavrila direction (sample_direction)
+    if (E > 100.)
+    {
+        for (int i = 0; i < 3; ++i)
+            sec->dir[i] = in.dir[i];
+    }
+    else
+    {
+        double const epm = std::max(1e-6, E) / pd.particles[pd.electron].mass;
+        double const gamma = epm + 1;
+        double const beta = std::sqrt(epm * (gamma + 1)) / gamma;
+        double const a = (1 - beta) / beta;
+        double const b = 0.5 * beta * gamma * epm * (gamma - 2);
+        double const g_max = 2 * (1 / a + b);
+        double g, nu;
+        do
+        {
+            double const u = rng();
+            nu = 2 * a * (2 * u + (a + 2) * std::sqrt(u)) / ((a + 2) * (a + 2) - 4 * u);
+            g = (2 - nu) * (1 / (a + nu) + b);
+        } while (g < g_max * rng());
+        exiting_direction(1 - nu, in.dir, rng, sec->dir);
+    }
+    r.energy_deposition = binding;
+    return r;
+}
+
+//---------------------------------------------------------------------------//
 }  // namespace
+
+//---------------------------------------------------------------------------//
+double GenericO::operator()(double v) const
+{
+    if (v <= x.front())
+        return y.front();
+    if (v >= x.back())
+        return y.back();
+    auto it = std::lower_bound(x.begin(), x.end(), v);
+    std::size_t i = static_cast<std::size_t>(it - x.begin());
+    if (*it != v)
+        --i;
+    double const slope = ((-y[i]) + y[i + 1]) / ((-x[i]) + x[i + 1]);
+    return std::fma(slope, (-x[i]) + v, y[i]);
+}
+
+LivermoreO make_livermore(Json const& j)
+{
+    auto generic = [](Json const& t) {
+        GenericO g;
+        g.x = t("x").nums();
+        g.y = t("y").nums();
+        if (g.x.size() < 2 || g.x.size() != g.y.size())
+            throw std::runtime_error("oracle: bad Livermore table");
+        return g;
+    };
+    LivermoreO r;
+    r.thresh_lo = j("thresh_lo").n;
+    r.thresh_hi = j("thresh_hi").n;
+    if (auto lo = j.find("xs_lo"))
+        r.xs_lo = generic(*lo);
+    r.xs_hi = generic(j("xs_hi"));
+    for (auto const& sj : j("shells").a)
+    {
+        LivermoreO::Shell sh;
+        sh.binding = sj("binding_energy").n;
+        auto lo = sj("param_lo").nums();
+        auto hi = sj("param_hi").nums();
+        for (int i = 0; i < 6; ++i)
+        {
+            sh.param[0][i] = lo.at(i);
+            sh.param[1][i] = hi.at(i);
+        }
+        sh.xs = generic(sj);
+        r.shells.push_back(sh);
+    }
+    return r;
+}
+
+double livermore_pe_micro_xs(ElementDef const& el, double inc_energy)
+{
+    LivermoreO const& pe = el.pe;
+    if (!pe)
+        throw std::runtime_error("oracle: element has no Livermore data");
+    double const energy = std::max(inc_energy, pe.shells.back().binding);
+    double const inv_energy = 1. / energy;
+    if (energy >= pe.thresh_lo)
+    {
+        auto const& param = pe.shells.back().param[energy < pe.thresh_hi ? 0 : 1];
+        return inv_energy * poly5(param, inv_energy);
+    }
+    double const inv3 = inv_energy * inv_energy * inv_energy;
+    if (energy >= pe.shells.front().binding)
+        return inv3 * pe.xs_hi(energy);
+    return inv3 * pe.xs_lo(energy);
+}
+
+double livermore_pe_macro_xs(PhysicsData const& pd, std::uint32_t mat, double energy)
+{
+    MaterialDef const& m = pd.materials[mat];
+    double result = 0.;
+    for (auto const& c : m.elements)
+        result += livermore_pe_micro_xs(pd.elements[c.first], energy) * c.second;
+    return result * 1e-24 * m.number_density;
+}
 
 //---------------------------------------------------------------------------//
 double migdal_constant()
@@ -483,10 +669,15 @@ SBTableO make_sb_table(Json const& j)
 
 void set_rb_element_data(double electron_mass, ElementDef& el)
 {
-    if (el.z < 5)
-        return;  // Dirac-Fock form factors (Z < 5) are not restated
-    double const ff_el = std::log(184.15) - el.log_z / 3;
-    double const ff_inel = std::log(1194.0) - 2 * el.log_z / 3;
+    // Dirac-Fock form factors for Z < 5 (RelativisticBremModel.cc:190-200)
+    static double const dirac_fock[4][2]
+        = {{5.3104, 5.9173}, {4.7935, 5.6125}, {4.7402, 5.5377}, {4.7112, 5.4728}};
+    if (el.z < 1)
+        return;
+    double const ff_el = el.z < 5 ? dirac_fock[el.z - 1][0]
+                                  : std::log(184.15) - el.log_z / 3;
+    double const ff_inel = el.z < 5 ? dirac_fock[el.z - 1][1]
+                                    : std::log(1194.0) - 2 * el.log_z / 3;
     double const fc = el.coulomb_correction;
     double const invz = 1.0 / el.z;
     double const z13 = el.cbrt_z;
@@ -500,11 +691,11 @@ void set_rb_element_data(double electron_mass, ElementDef& el)
 
 double rb_dxsec(ElementDef const& el, double total_energy, double gamma_energy)
 {
-    if (el.z < 5)
-        throw std::runtime_error("oracle: relativistic brems needs Z >= 5");
     double const y = gamma_energy / total_energy;
     double const onemy = 1 - y;
     double const term0 = onemy + 0.75 * (y * y);
+    if (el.z < 5)
+        return std::max(term0 * el.rb_factor1 + onemy * el.rb_factor2, 0.0);
     double const invz = 1 / static_cast<double>(el.z);
     double const term1 = y / (total_energy - gamma_energy);
     double const gam = term1 * el.rb_gamma_factor;
@@ -660,6 +851,8 @@ Interaction sample_model(std::string const& model, InteractorInput const& in,
         return brems(in, s, rng, 1);
     if (model == "combined_brem")
         return brems(in, s, rng, 2);
+    if (model == "livermore_pe")
+        return livermore_pe(in, s, rng);
     throw std::runtime_error("oracle: unsupported model " + model);
 }
 
@@ -702,6 +895,8 @@ double hardwired_xs(State const& s, ProcessDef const& p, std::uint32_t mat, doub
                * (poly * std::log(gamma + 1 + sqrt_gg2) - (gamma + 4) * sqrt_gg2)
                / (gamma * (gamma + 2) * (gamma + 2));
     }
+    if (p.cls == "photoelectric")
+        return livermore_pe_macro_xs(pd, mat, e);
     throw std::runtime_error("oracle: no cross section for process " + p.label);
 }
 

@@ -145,6 +145,15 @@ double sample_sb_energy(ElementDef const& el, double electron_density,
 double migdal_constant();
 
 //---------------------------------------------------------------------------//
+// Livermore photoelectric (LivermorePEInteractor.hh, LivermorePEMicroXs-
+// Calculator.hh); pinned by LivermorePE.test.cc:137-203 and :484-517
+LivermoreO make_livermore(Json const& j);
+//! Micro cross section [barn]
+double livermore_pe_micro_xs(ElementDef const& el, double energy);
+//! Macro cross section [1/cm] (MacroXsCalculator)
+double livermore_pe_macro_xs(PhysicsData const& pd, std::uint32_t mat, double energy);
+
+//---------------------------------------------------------------------------//
 // Math helpers shared by interactors (corecel/math/ArrayUtils.hh)
 void rotate(double const* dir, double const* rot, double* out);
 void from_spherical(double costheta, double phi, double* out);

@@ -3,6 +3,7 @@
 //---------------------------------------------------------------------------//
 #include "physics.hh"
 
+#include <algorithm>
 #include <cmath>
 #include <limits>
 #include <stdexcept>
@@ -234,6 +235,8 @@ void build_physics(Json const& inp, PhysicsData& out)
             d.mass_radiation_coeff = mass_radiation_coeff(d.z, d.atomic_mass);
             if (auto sb = e.find("sb_table"))
                 d.sb = make_sb_table(*sb);
+            if (auto pe = e.find("livermore_pe"))
+                d.pe = make_livermore(*pe);
             if (out.electron != none)
                 set_rb_element_data(out.particles[out.electron].mass, d);
             out.elements.push_back(d);
@@ -311,7 +314,10 @@ void build_physics(Json const& inp, PhysicsData& out)
                 m.low = mj.num("low", 0);
                 m.high = mj.num("high", 0);
                 p.models.push_back(m);
-                out.model_labels.push_back(m.label);
+                // A model label names one action shared by all its particles
+                if (std::find(out.model_labels.begin(), out.model_labels.end(), m.label)
+                    == out.model_labels.end())
+                    out.model_labels.push_back(m.label);
             }
             if (p.integral)
             {
@@ -351,10 +357,13 @@ void PhysicsData::finalize_actions(ActionIds const& ids)
     integral_rejection_action = ids.integral_rejected;
     failure_action = ids.failure;
     fixed_step_action = ids.eloss_range;
-    std::size_t k = 0;
     for (auto& p : processes)
         for (auto& m : p.models)
-            m.action = ids.model.at(k++);
+        {
+            auto k = std::find(model_labels.begin(), model_labels.end(), m.label)
+                     - model_labels.begin();
+            m.action = ids.model.at(static_cast<std::size_t>(k));
+        }
 }
 
 //---------------------------------------------------------------------------//

@@ -59,8 +59,32 @@ struct SBTableO
     explicit operator bool() const { return !x.empty(); }
 };
 
+//! Tabulated function on a nonuniform grid (GenericCalculator)
+struct GenericO
+{
+    std::vector<double> x, y;
+    explicit operator bool() const { return !x.empty(); }
+    double operator()(double v) const;
+};
+
+//! Livermore photoelectric data (em/data/LivermorePEData.hh)
+struct LivermoreO
+{
+    struct Shell
+    {
+        double binding = 0;
+        double param[2][6] = {};
+        GenericO xs;
+    };
+    double thresh_lo = 0, thresh_hi = 0;
+    GenericO xs_lo, xs_hi;
+    std::vector<Shell> shells;
+    explicit operator bool() const { return !shells.empty(); }
+};
+
 struct ElementDef
 {
+    LivermoreO pe;
     int z = 0;
     double atomic_mass = 0;  // [amu]
     double cbrt_z = 0;

@@ -23,7 +23,8 @@ def _lib():
             "orc_sb_positron_scale": (f64, [i32, f64, f64, f64, f64]),
             "orc_sb_sample_many": (i32, [vp, vp, u32, u32, f64, f64, f64, i32, u32, vp]),
             "orc_interactor_stress": (i32, [vp, vp, ctypes.c_char_p, u32, f64, vp, u32, u32,
-                                            ctypes.POINTER(ctypes.c_ulonglong)]),
+                                            ctypes.POINTER(ctypes.c_ulonglong), vp]),
+            "orc_pe_macro_xs": (f64, [vp, u32, f64]),
             "orc_rb_dxsec": (f64, [vp, u32, f64, f64]),
         }
         for name, (res, args) in sigs.items():
@@ -65,12 +66,19 @@ def sb_sample_many(problem, mt: MtEngine, energy, mass, kc, corrector, n,
 
 
 def interactor_stress(problem, mt: MtEngine, model, particle, energy, direction, n,
-                      material=0) -> int:
+                      material=0, sums=None) -> int:
+    """Run n interactions; ``sums`` (3 doubles) accumulates the secondary
+    count, energy and cosine with ``direction``."""
     d = np.ascontiguousarray(direction, dtype=np.float64)
     count = ctypes.c_ulonglong(0)
     ob._check(_lib().orc_interactor_stress(problem.handle, mt.h, model.encode(), particle, energy,
-                                           d.ctypes.data, material, n, ctypes.byref(count)))
+                                           d.ctypes.data, material, n, ctypes.byref(count),
+                                           None if sums is None else sums.ctypes.data))
     return count.value
+
+
+def pe_macro_xs(problem, material, energy) -> float:
+    return _lib().orc_pe_macro_xs(problem.handle, material, energy)
 
 
 def rb_dxsec(problem, element, total_energy, gamma_energy) -> float:

@@ -46,3 +46,14 @@ def sb_copper():
 
 def rb_lead():
     return single_element(82, 207.2, 0.05477, 0.0945861, "Pb")
+
+
+def pe_potassium():
+    """LivermorePE.test.cc:40-95: K (Z=19) at 1e-5 mol/cm^3 with the
+    pe-*-19 data files."""
+    prob = single_element(19, 39.0983, 1e-5, 0.0, "K")
+    prob["physics"]["elements"][0]["livermore_pe"] = g4emlow.read_livermore_pe(19)
+    prob["physics"]["processes"] = [{
+        "label": "phot", "class": "photoelectric", "particle": "gamma",
+        "models": [{"label": "photoel-livermore", "model": "livermore_pe"}]}]
+    return prob
