@@ -423,6 +423,51 @@ class Stepper:
         _check(_load().celer_stepper_action_sequence(self._h, buf, 8192))
         return [s for s in buf.value.decode().split("\n") if s]
 
+    def diagnostics(self, enable: Optional[bool] = None, reset: bool = False) -> dict:
+        """Per-kernel work counters of the stepper's state (see
+        celer_state_diagnostics); ``enable`` turns them on or off."""
+        f = _load().celer_state_diagnostics
+        f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint32,
+                      ctypes.c_int]
+        f.restype = ctypes.c_int
+        flag = -1 if enable is None else int(bool(enable))
+        if enable is False:
+            _check(f(self.state._h, 0, None, 0, 0))
+            return {}
+        out = np.zeros(8, np.uint64)
+        _check(f(self.state._h, flag, out.ctypes.data, 8, int(reset)))
+        names = ("pre_step", "along_charged", "along_neutral", "boundary", "discrete",
+                 "interaction")
+        return {k: int(v) for k, v in zip(names, out)}
+
+    def action_times(self, reset: bool = False) -> dict:
+        """Accumulated device seconds and timed-step counts per step action
+        (stepper created with ``action_times=True``)."""
+        labels = self.action_sequence()
+        n = len(labels)
+        secs = np.zeros(n, np.float64)
+        counts = np.zeros(n, np.uint64)
+        f = _load().celer_stepper_action_times
+        f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                      ctypes.c_int]
+        f.restype = ctypes.c_int
+        _check(f(self._h, secs.ctypes.data, counts.ctypes.data, n, int(reset)))
+        return {lab: (float(s), int(c)) for lab, s, c in zip(labels, secs, counts)}
+
+    def run(self, primaries: np.ndarray, max_steps: int = 1_000_000) -> dict:
+        """Transport primaries to completion (celer-sim's Transporter loop:
+        Runner.cc/Transporter.cc); returns step iterations and the sum of
+        active tracks over iterations (track-steps)."""
+        result = self(primaries)
+        iters, track_steps = 1, result["active"]
+        while result:
+            if iters >= max_steps:
+                raise CeleritasError(f"transport did not finish in {max_steps} steps")
+            result = self()
+            iters += 1
+            track_steps += result["active"]
+        return {"iterations": iters, "track_steps": int(track_steps)}
+
 
 MODEL_CLASSES = {
     "klein_nishina": 0,

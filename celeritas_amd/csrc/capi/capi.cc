@@ -419,6 +419,46 @@ int celer_stepper_warm_up(CelerStepper* stepper)
     return guarded([&] { stepper->stepper->warm_up(); });
 }
 
+int celer_state_diagnostics(CelerState* state, int enable, uint64_t* out, uint32_t n,
+                            int reset)
+{
+    return guarded([&] {
+        CoreState& s = *state->state;
+        if (enable >= 0)
+            s.enable_diagnostics(enable != 0);
+        if (out)
+        {
+            auto v = s.read_diagnostics(reset != 0);
+            CELER_VALIDATE(!v.empty(), << "diagnostics are not enabled");
+            for (uint32_t i = 0; i < n && i < v.size(); ++i)
+                out[i] = v[i];
+        }
+    });
+}
+
+int celer_stepper_action_times(CelerStepper* stepper,
+                               double* seconds,
+                               uint64_t* counts,
+                               uint32_t n,
+                               int reset)
+{
+    return guarded([&] {
+        auto const& t = stepper->stepper->action_times();
+        auto const& c = stepper->stepper->action_counts();
+        CELER_VALIDATE(n == t.size(),
+                       << "expected " << t.size() << " step actions, got " << n);
+        for (uint32_t i = 0; i < n; ++i)
+        {
+            if (seconds)
+                seconds[i] = t[i];
+            if (counts)
+                counts[i] = c[i];
+        }
+        if (reset)
+            stepper->stepper->reset_action_times();
+    });
+}
+
 int celer_stepper_kill_active(CelerStepper* stepper)
 {
     return guarded([&] { stepper->stepper->kill_active(); });

@@ -40,6 +40,15 @@ __device__ __forceinline__ unsigned int prefix_count(std::uint64_t mask)
         __builtin_amdgcn_mbcnt_lo(static_cast<unsigned int>(mask), 0u));
 }
 
+//! Add the number of currently active lanes to a global counter with one
+//! atomic per wave (the lowest active lane issues it)
+__device__ __forceinline__ void count_active_lanes(unsigned long long* counter)
+{
+    std::uint64_t const mask = __ballot(1);
+    if (prefix_count(mask) == 0)
+        atomicAdd(counter, static_cast<unsigned long long>(__popcll(mask)));
+}
+
 //! Inclusive scan of a 32-bit value over the wave
 __device__ __forceinline__ unsigned int inclusive_scan(unsigned int v)
 {

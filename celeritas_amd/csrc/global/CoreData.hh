@@ -149,6 +149,21 @@ struct CoreStateRef
     std::uint32_t* track_slots{nullptr};  //!< thread -> slot indirection
     std::uint32_t* action_offsets{nullptr};  //!< sorted ranges per action
     double* calo_edep{nullptr};  //!< per-volume energy deposition tally
+    //! Optional work counters (null unless diagnostics are enabled), see
+    //! DiagCounter
+    unsigned long long* diag{nullptr};
+};
+
+//! Indices of the optional per-kernel work counters
+enum DiagCounter : int
+{
+    diag_pre_step = 0,  //!< tracks processed by pre-step
+    diag_along_charged,  //!< tracks processed by the charged along-step
+    diag_along_neutral,  //!< tracks processed by the neutral along-step
+    diag_boundary,  //!< boundary crossings
+    diag_discrete,  //!< discrete selections
+    diag_interaction,  //!< interactor calls
+    num_diag_counters = 8
 };
 
 //---------------------------------------------------------------------------//

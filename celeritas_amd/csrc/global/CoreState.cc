@@ -191,4 +191,39 @@ std::uint32_t* CoreState::primary_rank_buffer(size_type count)
 }
 
 //---------------------------------------------------------------------------//
+void CoreState::enable_diagnostics(bool enable)
+{
+    CELER_HIP_CALL(hipStreamSynchronize(stream_));
+    if (enable)
+    {
+        if (diag_.size() == 0)
+            diag_.resize(num_diag_counters);
+        CELER_HIP_CALL(hipMemsetAsync(diag_.data(), 0,
+                                      sizeof(unsigned long long) * num_diag_counters,
+                                      stream_));
+        ref_.diag = diag_.data();
+    }
+    else
+    {
+        ref_.diag = nullptr;
+    }
+}
+
+std::vector<std::uint64_t> CoreState::read_diagnostics(bool reset)
+{
+    std::vector<std::uint64_t> result;
+    if (!ref_.diag)
+        return result;
+    std::vector<unsigned long long> tmp(num_diag_counters);
+    CELER_HIP_CALL(hipMemcpyAsync(tmp.data(), ref_.diag, sizeof(unsigned long long) * tmp.size(),
+                                  hipMemcpyDeviceToHost, stream_));
+    CELER_HIP_CALL(hipStreamSynchronize(stream_));
+    if (reset)
+        CELER_HIP_CALL(hipMemsetAsync(ref_.diag, 0, sizeof(unsigned long long) * tmp.size(),
+                                      stream_));
+    result.assign(tmp.begin(), tmp.end());
+    return result;
+}
+
+//---------------------------------------------------------------------------//
 }  // namespace celeritas_amd

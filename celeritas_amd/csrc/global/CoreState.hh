@@ -61,6 +61,11 @@ class CoreState
     std::size_t memory_bytes() const { return arena_.size(); }
     size_type num_detectors() const { return num_detectors_; }
 
+    //! Turn the per-kernel work counters on (zeroed) or off
+    void enable_diagnostics(bool enable);
+    //! Read the work counters (synchronizes the stream); empty if off
+    std::vector<std::uint64_t> read_diagnostics(bool reset);
+
   private:
     CoreStateRef ref_;
     DeviceArena arena_;
@@ -68,6 +73,7 @@ class CoreState
     CoreCounters* host_counters_{nullptr};
     DeviceVector<Primary> primaries_;
     DeviceVector<std::uint32_t> primary_ranks_;
+    DeviceVector<unsigned long long> diag_;
     size_type num_detectors_{0};
     bool warming_up_{false};
     std::vector<std::uint32_t> initial_rng_;

@@ -64,6 +64,8 @@ __global__ void __launch_bounds__(block_size)
         state.sim.along_step_action[slot] = invalid_id8;
         return;
     }
+    if (state.diag)
+        wave::count_active_lanes(state.diag + diag_pre_step);
     PhysicsStateRef const& ps = state.phys;
     ps.energy_deposition[slot] = 0;
     ps.sec_start[slot] = 0;
@@ -111,7 +113,7 @@ __global__ void __launch_bounds__(block_size)
         if (part.eloss_process >= 0)
         {
             double const range
-                = phys::calc_grid(pp.grids[part.range_grid + mat], pp.reals, energy);
+                = phys::calc_range(pp.grids[part.range_grid + mat], pp.reals, energy);
             ps.dedx_range[slot] = range;
             double const eloss_step = phys::range_to_step(pp.scalars, range);
             if (eloss_step <= step)
@@ -152,6 +154,8 @@ __global__ void __launch_bounds__(block_size)
     auto const status = status_of(state, slot);
     if (status == TrackStatus::inactive || state.sim.along_step_action[slot] != action)
         return;
+    if (state.diag)
+        wave::count_active_lanes(state.diag + diag_along_neutral);
 
     double step = state.sim.step_length[slot];
     std::uint8_t post = state.sim.post_step_action[slot];
@@ -294,6 +298,8 @@ __global__ void __launch_bounds__(block_size)
     if (state.sim.post_step_action[slot] != params.scalars.boundary_action
         || status_of(state, slot) != TrackStatus::alive)
         return;
+    if (state.diag)
+        wave::count_active_lanes(state.diag + diag_boundary);
     geo::GeoTrack g = geo::GeoTrack::load(state.geo, slot);
     geo::cross_boundary(params.geo, g);
     g.store(state.geo, slot);
@@ -407,14 +413,6 @@ hipError_t launch_along_step_neutral(CoreParamsRef const& p,
 {
     along_step_neutral_kernel<<<num_blocks(s.size), block_size, 0, st>>>(p, s, action);
     return hipGetLastError();
-}
-
-hipError_t launch_along_step_charged(CoreParamsRef const&,
-                                     CoreStateRef const&,
-                                     std::uint8_t,
-                                     hipStream_t)
-{
-    return hipErrorNotSupported;
 }
 
 hipError_t launch_fused_step(CoreParamsRef const&, CoreStateRef const&, hipStream_t)

@@ -35,6 +35,7 @@ Stepper::Stepper(StepperInput input)
                < std::make_pair(b->order(), b->action_id());
     });
     times_.assign(actions_.size(), 0.0);
+    counts_.assign(actions_.size(), 0);
 
     ActionId pid = reg.find_action(ExtendFromPrimariesAction::label_str());
     CELER_VALIDATE(pid != invalid_id,
@@ -50,22 +51,38 @@ Stepper::Stepper(StepperInput input)
     }
 }
 
-Stepper::~Stepper() = default;
+Stepper::~Stepper()
+{
+    for (auto e : events_)
+        (void)hipEventDestroy(e);
+}
+
+void Stepper::reset_action_times()
+{
+    std::fill(times_.begin(), times_.end(), 0.0);
+    std::fill(counts_.begin(), counts_.end(), 0);
+}
 
 //---------------------------------------------------------------------------//
 void Stepper::enqueue_step()
 {
     if (action_times_ && !state_->warming_up())
     {
+        // Device time per action from events bracketing its launches on the
+        // state's stream; read back after the step's counter sync
+        if (events_.empty())
+        {
+            events_.resize(2 * actions_.size());
+            for (auto& e : events_)
+                CELER_HIP_CALL(hipEventCreate(&e));
+        }
         for (std::size_t i = 0; i < actions_.size(); ++i)
         {
-            auto start = std::chrono::steady_clock::now();
+            CELER_HIP_CALL(hipEventRecord(events_[2 * i], state_->stream()));
             actions_[i]->step(*params_, *state_);
-            CELER_HIP_CALL(hipStreamSynchronize(state_->stream()));
-            times_[i] += std::chrono::duration<double>(
-                             std::chrono::steady_clock::now() - start)
-                             .count();
+            CELER_HIP_CALL(hipEventRecord(events_[2 * i + 1], state_->stream()));
         }
+        events_pending_ = true;
     }
     else
     {
@@ -77,6 +94,17 @@ void Stepper::enqueue_step()
 StepperResult Stepper::make_result()
 {
     CoreCounters const& c = state_->sync_counters();
+    if (events_pending_)
+    {
+        for (std::size_t i = 0; i < actions_.size(); ++i)
+        {
+            float ms = 0;
+            CELER_HIP_CALL(hipEventElapsedTime(&ms, events_[2 * i], events_[2 * i + 1]));
+            times_[i] += 1e-3 * ms;
+            counts_[i] += 1;
+        }
+        events_pending_ = false;
+    }
     CELER_VALIDATE(!(c.error_flags & err_initializer_overflow),
                    << "insufficient capacity (" << params_->init_capacity()
                    << ") for track initializers (created " << c.num_secondaries

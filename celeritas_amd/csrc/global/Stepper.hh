@@ -76,8 +76,11 @@ class Stepper
     CoreState& state() { return *state_; }
     std::shared_ptr<CoreState> const& sp_state() const { return state_; }
     CoreParams const& params() const { return *params_; }
-    //! Accumulated per-action times [s] when action_times is on
+    //! Accumulated per-action device times [s] when action_times is on
     std::vector<double> const& action_times() const { return times_; }
+    //! Number of timed steps per action
+    std::vector<std::uint64_t> const& action_counts() const { return counts_; }
+    void reset_action_times();
 
   private:
     std::shared_ptr<CoreParams const> params_;
@@ -86,6 +89,9 @@ class Stepper
     std::shared_ptr<ExtendFromPrimariesAction const> primaries_action_;
     bool action_times_{false};
     std::vector<double> times_;
+    std::vector<std::uint64_t> counts_;
+    std::vector<hipEvent_t> events_;
+    bool events_pending_{false};
 
     StepperResult make_result();
 };

@@ -28,7 +28,7 @@ generic_calc(GenericGrid const& g, double const* __restrict__ reals, double x)
     double const* xs = reals + g.x;
     double const* ys = reals + g.y;
     std::uint32_t const n = g.size;
-    if (x <= xs[0])
+    if (!(x > xs[0]))  // also catches NaN: never index with it
         return ys[0];
     if (x >= xs[n - 1])
         return ys[n - 1];

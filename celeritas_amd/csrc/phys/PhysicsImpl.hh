@@ -31,7 +31,7 @@ calc_grid(XsGrid const& g, double const* __restrict__ reals, double energy)
 {
     double const* v = reals + g.offset;
     double const loge = std::log(energy);
-    if (loge <= g.loge_front)
+    if (!(loge > g.loge_front))  // also catches NaN: never index with it
     {
         double r = v[0];
         return (0 >= g.prime_index) ? r / energy : r;
@@ -103,6 +103,89 @@ CELER_FUNCTION double calc_max_xs(PhysicsParamsRef const& p,
     double const a = calc_xs(p, proc, mat, energy);
     double const b = calc_xs(p, proc, mat, exi);
     return a > b ? a : b;
+}
+
+/*!
+ * Range at an energy (grid/RangeCalculator.hh:80-110): sqrt(E/Emin) scaling
+ * below the grid, clipped above it, linear in E between grid points.
+ */
+CELER_FUNCTION double
+calc_range(XsGrid const& g, double const* __restrict__ reals, double energy)
+{
+    double const* v = reals + g.offset;
+    double const loge = std::log(energy);
+    if (!(loge > g.loge_front))  // also catches NaN: never index with it
+        return v[0] * std::exp(0.5 * (loge - g.loge_front));
+    if (loge >= g.loge_back)
+        return v[g.size - 1];
+    auto const i = static_cast<std::uint32_t>((loge - g.loge_front) / g.loge_delta);
+    double const xl = std::exp(g.loge_front + g.loge_delta * i);
+    double const xr = std::exp(g.loge_front + g.loge_delta * (i + 1));
+    double const slope = ((-v[i]) + v[i + 1]) / ((-xl) + xr);
+    return std::fma(slope, (-xl) + energy, v[i]);
+}
+
+/*!
+ * Energy for a range (grid/InverseRangeCalculator.hh:87-117): quadratic
+ * below the first range value, linear in E between tabulated ranges.
+ */
+CELER_FUNCTION double
+calc_inverse_range(XsGrid const& g, double const* __restrict__ reals, double range)
+{
+    double const* r = reals + g.offset;
+    std::uint32_t const n = g.size;
+    if (!(range >= r[0]))  // also catches NaN: never index with it
+    {
+        double const f = range / r[0];
+        return std::exp(g.loge_front) * (f * f);
+    }
+    if (range >= r[n - 1])
+        return std::exp(g.loge_back);
+    std::uint32_t lo = 0;
+    std::uint32_t len = n;
+    while (len > 0)
+    {
+        std::uint32_t const half = len / 2;
+        if (r[lo + half] < range)
+        {
+            lo += half + 1;
+            len -= half + 1;
+        }
+        else
+        {
+            len = half;
+        }
+    }
+    std::uint32_t i = lo;
+    if (r[i] != range)
+        --i;
+    double const el = std::exp(g.loge_front + g.loge_delta * i);
+    double const er = std::exp(g.loge_front + g.loge_delta * (i + 1));
+    double const slope = ((-el) + er) / ((-r[i]) + r[i + 1]);
+    return std::fma(slope, (-r[i]) + range, el);
+}
+
+/*!
+ * Mean energy loss over a step (PhysicsStepUtils.hh:177-233): dE/dx times
+ * the step, or the inverse-range difference when that exceeds the linear
+ * loss limit.
+ */
+CELER_FUNCTION double calc_mean_eloss(PhysicsParamsRef const& p,
+                                      ParticlePhysics const& part,
+                                      std::uint32_t mat,
+                                      double energy,
+                                      double range,
+                                      double step)
+{
+    double eloss = step * calc_grid(p.grids[part.dedx_grid + mat], p.reals, energy);
+    if (eloss >= energy * p.scalars.linear_loss_limit)
+    {
+        if (step == range)
+            return energy;
+        eloss = energy
+                - calc_inverse_range(p.grids[part.range_grid + mat], p.reals, range - step);
+    }
+    return eloss;
 }
 
 //! Scaled range limit (PhysicsTrackView.hh:624-640)

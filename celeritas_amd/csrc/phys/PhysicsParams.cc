@@ -544,6 +544,11 @@ PhysicsParams::PhysicsParams(JsonValue const& inp, ActionRegistry* reg)
             process_from_string(pj->get_str("class", "")));
         rec.integral = pj->get_bool("integral", false) ? 1 : 0;
         rec.elem_cdf = invalid_id;
+        if (rec.process_class == static_cast<std::uint32_t>(ProcessClass::annihilation))
+        {
+            // e+ annihilation applies at rest (EPlusAnnihilationProcess)
+            pp.has_at_rest = 1;
+        }
         if (rec.process_class == static_cast<std::uint32_t>(ProcessClass::photoelectric))
         {
             for (auto const& c : h.components)
@@ -595,6 +600,13 @@ PhysicsParams::PhysicsParams(JsonValue const& inp, ActionRegistry* reg)
             auto const& mj = models[k];
             ModelClass mc = model_from_string(mj.at("model").as_string());
             std::string const mlabel = mj.at("label").as_string();
+            if (mc == ModelClass::seltzer_berger || mc == ModelClass::combined_brem)
+            {
+                for (auto const& c : h.components)
+                    CELER_VALIDATE(h.elements.at(c.element).sb_nx >= 2,
+                                   << "model '" << mlabel
+                                   << "' needs a Seltzer-Berger table for every element in use");
+            }
             rec.models[k].energy_low = mj.get("low", 0.0);
             rec.models[k].model = static_cast<std::uint32_t>(mc);
             // One model (and action) may serve several particles, as the

@@ -707,6 +707,25 @@ JsonValue generate_physics(JsonValue const& spec)
         }
         return arr;
     };
+    // Tables of processes with a production threshold start at it (as the
+    // Geant4 lambda tables do), so that interpolation gives exactly zero
+    // below threshold
+    auto per_material_from = [&](auto&& threshold, auto&& fn) {
+        JsonValue arr = JsonValue::array();
+        for (std::size_t mi = 0; mi < nmat; ++mi)
+        {
+            double const e0 = std::max(threshold(mi), g.emin);
+            int const n = std::max(
+                2, static_cast<int>(std::ceil(bpd * std::log10(g.emax / e0) - 1e-9)) + 1);
+            LogGrid const gm{e0, g.emax, n};
+            std::vector<double> v(n);
+            for (int i = 0; i < n; ++i)
+                v[i] = fn(materials[mi], mi, gm.energy(i));
+            v[0] = 0;
+            arr.arr.push_back(grid_json(gm, v));
+        }
+        return arr;
+    };
     auto atom_sum = [&](Material const& m, auto&& micro) {
         double s = 0;
         for (auto const& c : m.comps)
@@ -749,7 +768,8 @@ JsonValue generate_physics(JsonValue const& spec)
     for (bool electron : {true, false})
     {
         char const* pname = electron ? "electron" : "positron";
-        auto const& ecut = electron ? cut_electron : cut_positron;
+        // Delta rays are electrons: both leptons use the electron cut
+        auto const& ecut = cut_electron;
         std::vector<std::vector<double>> dedx(nmat, std::vector<double>(npts));
         for (std::size_t mi = 0; mi < nmat; ++mi)
         {
@@ -780,9 +800,11 @@ JsonValue generate_physics(JsonValue const& spec)
         }
         JsonValue* ioni = add_process(electron ? "eIoni" : "eIoni+", "e_ioni", pname, true,
                                       model_json("ioni-moller-bhabha", "moller_bhabha"));
-        ioni->obj["xs"] = per_material([&](Material const& m, std::size_t mi, double e) {
-            return m.electron_density * ioni_xs(e, ecut[mi], electron);
-        });
+        ioni->obj["xs"] = per_material_from(
+            [&](std::size_t mi) { return (electron ? 2 : 1) * ecut[mi]; },
+            [&](Material const& m, std::size_t mi, double e) {
+                return m.electron_density * ioni_xs(e, ecut[mi], electron);
+            });
         JsonValue dg = JsonValue::array(), rg = JsonValue::array();
         for (std::size_t mi = 0; mi < nmat; ++mi)
         {
@@ -794,15 +816,17 @@ JsonValue generate_physics(JsonValue const& spec)
 
         JsonValue* brems = add_process(electron ? "eBrem" : "eBrem+", "e_brems", pname, true,
                                        model_json("brems-combined", "combined_brem"));
-        brems->obj["xs"] = per_material([&](Material const& m, std::size_t mi, double e) {
-            double const kc = cut_gamma[mi];
-            if (e <= kc)
-                return 0.0;
-            return atom_sum(m, [&](Element const& el) {
-                return integrate_log(
-                    [&](double k) { return brems_k_dsigma(el, m, e, k) / k; }, kc, e, 8);
+        brems->obj["xs"] = per_material_from(
+            [&](std::size_t mi) { return cut_gamma[mi]; },
+            [&](Material const& m, std::size_t mi, double e) {
+                double const kc = cut_gamma[mi];
+                if (e <= kc)
+                    return 0.0;
+                return atom_sum(m, [&](Element const& el) {
+                    return integrate_log(
+                        [&](double k) { return brems_k_dsigma(el, m, e, k) / k; }, kc, e, 8);
+                });
             });
-        });
     }
 
     // Assemble

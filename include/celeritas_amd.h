@@ -244,6 +244,30 @@ CelerState* celer_stepper_state(CelerStepper* stepper);
 int celer_stepper_action_sequence(const CelerStepper* stepper,
                                   char* buf,
                                   size_t len);
+/*
+ * Accumulated device time [s] and number of timed steps per step action
+ * (execution order), for a stepper created with action_times != 0. Device
+ * time is measured with events bracketing each action's launches on the
+ * state stream. Counterpart of ActionSequence::accum_time
+ * (celeritas/global/ActionSequence.cc:78-138) and Stepper::actions().
+ * n must equal the number of step actions; reset != 0 zeroes the totals
+ * after reading.
+ */
+int celer_stepper_action_times(CelerStepper* stepper,
+                               double* seconds,
+                               uint64_t* counts,
+                               uint32_t n,
+                               int reset);
+/*
+ * Per-kernel work counters (tracks processed by pre-step, charged and
+ * neutral along-step, boundary crossings, ...; up to 8 values). enable: 1
+ * on (zeroed), 0 off, -1 unchanged; out (may be NULL) receives n values,
+ * reset != 0 zeroes them after reading. Counterpart of the reference's
+ * ActionDiagnostic/StepDiagnostic counters (celeritas/user/
+ * ActionDiagnostic.hh): one atomic per wavefront when on, none when off.
+ */
+int celer_state_diagnostics(CelerState* state, int enable, uint64_t* out,
+                            uint32_t n, int reset);
 
 /*---------------------------------------------------------------------------*
  * LOW-LEVEL LAUNCH ABI: enqueue kernels on a caller stream with the POD

@@ -424,7 +424,7 @@ void pre_step(State& s)
             if (ep >= 0)
             {
                 ProcessDef const& p = pd.processes[procs[ep]];
-                double range = p.range[mat].calc(t.energy);
+                double range = calc_range(p.range[mat], t.energy);
                 t.dedx_range = range;
                 double eloss_step = range_to_step(pd, range);
                 if (eloss_step <= step)
@@ -462,11 +462,20 @@ void propagate_linear(State& s, Track& t)
     if (t.step_length == 0)
         return;
     bool boundary_hit;
+    double const orig[3] = {t.geo.pos[0], t.geo.pos[1], t.geo.pos[2]};
     double dist = geo_find_next_step(s.problem->geo, t.geo, t.step_length, &boundary_hit);
     if (boundary_hit)
         geo_move_to_boundary(t.geo);
     else
         geo_move_internal(t.geo, t.step_length);
+    if (t.geo.pos[0] == orig[0] && t.geo.pos[1] == orig[1] && t.geo.pos[2] == orig[2])
+    {
+        // CoreTrackView::apply_errored
+        t.status = errored;
+        t.along_action = none;
+        t.post_action = ids.tracking_cut;
+        return;
+    }
     if (boundary_hit)
     {
         t.step_length = dist;
@@ -669,6 +678,101 @@ void tracking_cut(State& s)
 }
 
 //---------------------------------------------------------------------------//
+// ALONG-STEP (CHARGED, NO FIELD): AlongStepGeneralLinearAction.cc
+//---------------------------------------------------------------------------//
+double calc_range(XsGridO const& g, double energy)
+{
+    // RangeCalculator::operator() (grid/RangeCalculator.hh:80-110)
+    double const loge = std::log(energy);
+    if (loge <= g.front)
+        return g.values[0] * std::exp(0.5 * (loge - g.front));
+    if (loge >= g.back)
+        return g.values[g.size - 1];
+    auto const i = static_cast<std::uint32_t>((loge - g.front) / g.delta);
+    double const xl = std::exp(g.front + g.delta * i);
+    double const xr = std::exp(g.front + g.delta * (i + 1));
+    double const slope = ((-g.values[i]) + g.values[i + 1]) / ((-xl) + xr);
+    return std::fma(slope, (-xl) + energy, g.values[i]);
+}
+
+double calc_inverse_range(XsGridO const& g, double range)
+{
+    // InverseRangeCalculator::operator() (InverseRangeCalculator.hh:87-117)
+    auto const& r = g.values;
+    if (range < r.front())
+    {
+        double const f = range / r.front();
+        return std::exp(g.front) * (f * f);
+    }
+    if (range >= r.back())
+        return std::exp(g.back);
+    auto it = std::lower_bound(r.begin(), r.end(), range);
+    auto i = static_cast<std::uint32_t>(it - r.begin());
+    if (*it != range)
+        --i;
+    double const el = std::exp(g.front + g.delta * i);
+    double const er = std::exp(g.front + g.delta * (i + 1));
+    double const slope = ((-el) + er) / ((-r[i]) + r[i + 1]);
+    return std::fma(slope, (-r[i]) + range, el);
+}
+
+namespace
+{
+//! ElossApplier with MeanELoss (ElossApplier.hh:41-90, MeanELoss.hh:65-92)
+void apply_mean_eloss(State& s, Track& t)
+{
+    PhysicsData const& pd = phys(s);
+    ActionIds const& ids = s.problem->actions;
+    int const ep = pd.eloss_process[t.particle];
+    if (t.status == errored || ep < 0 || t.energy == 0)
+        return;
+    auto const& procs = pd.particle_processes[t.particle];
+    ProcessDef const& p = pd.processes[procs[ep]];
+    std::uint32_t const mat = material_of(s, t);
+    bool const apply_cut = t.post_action != ids.boundary;
+    double const lowest = pd.lowest_electron_energy;
+    double deposited;
+    if (apply_cut && t.energy < lowest)
+    {
+        deposited = t.energy;
+    }
+    else
+    {
+        // calc_mean_energy_loss (PhysicsStepUtils.hh:177-233)
+        double const e = t.energy;
+        double eloss = t.step_length * p.dedx[mat].calc(e);
+        if (eloss >= e * pd.linear_loss_limit)
+        {
+            if (t.step_length == t.dedx_range)
+                eloss = e;
+            else
+                eloss = e - calc_inverse_range(p.range[mat], t.dedx_range - t.step_length);
+        }
+        deposited = (apply_cut && e - eloss <= lowest) ? e : eloss;
+    }
+    if (deposited > 0)
+    {
+        t.edep += deposited;
+        t.energy -= deposited;
+    }
+    if (t.energy == 0)
+    {
+        bool has_at_rest = false;
+        for (auto pi : procs)
+            has_at_rest = has_at_rest || pd.processes[pi].cls == "annihilation";
+        if (!has_at_rest)
+        {
+            t.status = killed;
+            t.post_action = pd.range_action;
+        }
+        else
+        {
+            t.post_action = pd.discrete_action;
+        }
+    }
+}
+}  // namespace
+
 void along_step_charged(State& s)
 {
     std::uint32_t const aid = s.problem->actions.along_user;
@@ -676,7 +780,10 @@ void along_step_charged(State& s)
     {
         if (t.status == inactive || t.along_action != aid)
             continue;
-        throw std::runtime_error("oracle: charged along-step not available");
+        propagate_linear(s, t);
+        update_time(s, t);
+        apply_mean_eloss(s, t);
+        update_track(s, t);
     }
 }
 

@@ -160,6 +160,11 @@ struct PhysicsData
 //! Parse the "physics" section (same schema as the product)
 void build_physics(Json const& inp, PhysicsData& out);
 
+//! Range and inverse range on a range grid (grid/RangeCalculator.hh,
+//! grid/InverseRangeCalculator.hh)
+double calc_range(XsGridO const& g, double energy);
+double calc_inverse_range(XsGridO const& g, double range);
+
 // Step actions (each loops over all slots, applying to matching tracks)
 void pre_step(State& s);
 void along_step_neutral(State& s);
