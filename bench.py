@@ -62,7 +62,31 @@ def parse_args():
     ap.add_argument("--cpu-primaries", type=int, default=48,
                     help="primaries in the CPU baseline sample (0: skip)")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run the plain copy kernel on a buffer of this size "
+                         "(reference for memory-counter profiles) and exit")
     return ap.parse_args()
+
+
+def calibrate_copy(mib):
+    """Plain copy of known size: FETCH_SIZE and WRITE_SIZE of this kernel
+    convert the counters of the transport kernels into bytes."""
+    import ctypes
+
+    import celeritas_amd as amd
+
+    f = amd.lib().celer_calibration_copy
+    f.argtypes = [ctypes.c_size_t, ctypes.c_uint32, ctypes.POINTER(ctypes.c_float)]
+    f.restype = ctypes.c_int
+    ms = ctypes.c_float(0)
+    nbytes = mib << 20
+    rc = f(nbytes, 10, ctypes.byref(ms))
+    if rc != 0:
+        raise SystemExit(f"calibration copy failed: {rc}")
+    print(json.dumps({"calibration_copy_bytes_read": nbytes,
+                      "calibration_copy_bytes_written": nbytes,
+                      "kernel_ms": ms.value,
+                      "gb_per_s": 2 * nbytes / (ms.value * 1e-3) / 1e9}), flush=True)
 
 
 def log(msg):
@@ -76,6 +100,9 @@ def main():
               "independent and the loop would shard by event, but no multi-GPU "
               "path is built", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
 
     import celeritas_amd as amd
     from celeritas_amd import workloads as wl
@@ -85,7 +112,7 @@ def main():
 
     log("generating physics tables")
     physics = amd.generate_physics(wl.testem3_physics_spec())
-    problem = wl.testem3_problem(physics=physics, init_capacity=1 << 23,
+    problem = wl.testem3_problem(physics=physics, init_capacity=1 << 26,
                                  max_events=max(args.primaries, 1))
     primaries = wl.testem3_primaries(args.primaries, seed=args.seed)
     params = amd.CoreParams(problem)

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <string>
 
+#include "accel/LocalTransporter.hh"
 #include "global/CoreParams.hh"
 #include "global/CoreState.hh"
 #include "phys/TableGen.hh"
@@ -39,6 +40,13 @@ struct CelerStepper
 {
     std::shared_ptr<CoreParams> params;
     std::shared_ptr<Stepper> stepper;
+    CelerState state_handle;
+};
+
+struct CelerTransporter
+{
+    std::shared_ptr<CoreParams> params;
+    std::unique_ptr<LocalTransporter> transporter;
     CelerState state_handle;
 };
 
@@ -417,6 +425,70 @@ int celer_stepper_step(CelerStepper* stepper,
 int celer_stepper_warm_up(CelerStepper* stepper)
 {
     return guarded([&] { stepper->stepper->warm_up(); });
+}
+
+//---------------------------------------------------------------------------//
+CelerTransporter* celer_transporter_create(CelerParams* params,
+                                           uint32_t max_num_tracks,
+                                           uint32_t auto_flush,
+                                           uint32_t max_steps,
+                                           uint32_t stream_id,
+                                           int nonfatal_flush,
+                                           int action_times)
+{
+    return guarded_new<CelerTransporter>([&] {
+        SetupOptions opts;
+        opts.max_num_tracks = max_num_tracks;
+        opts.auto_flush = auto_flush;
+        opts.max_steps = max_steps;
+        opts.stream_id = stream_id;
+        opts.nonfatal_flush = nonfatal_flush != 0;
+        opts.action_times = action_times != 0;
+        auto* result = new CelerTransporter;
+        result->params = params->params;
+        result->transporter = std::make_unique<LocalTransporter>(opts, params->params);
+        result->state_handle.params = params->params;
+        result->state_handle.state = result->transporter->stepper().sp_state();
+        return result;
+    });
+}
+
+void celer_transporter_destroy(CelerTransporter* t)
+{
+    delete t;
+}
+
+int celer_transporter_initialize_event(CelerTransporter* t, int event_id)
+{
+    return guarded([&] { t->transporter->InitializeEvent(event_id); });
+}
+
+int celer_transporter_push(CelerTransporter* t, const CelerPrimary* tracks, uint32_t count)
+{
+    return guarded([&] {
+        for (uint32_t i = 0; i < count; ++i)
+            t->transporter->Push(reinterpret_cast<Primary const&>(tracks[i]));
+    });
+}
+
+int celer_transporter_flush(CelerTransporter* t)
+{
+    return guarded([&] { t->transporter->Flush(); });
+}
+
+int celer_transporter_finalize(CelerTransporter* t)
+{
+    return guarded([&] { t->transporter->Finalize(); });
+}
+
+uint32_t celer_transporter_buffer_size(const CelerTransporter* t)
+{
+    return t->transporter->GetBufferSize();
+}
+
+CelerState* celer_transporter_state(CelerTransporter* t)
+{
+    return &t->state_handle;
 }
 
 int celer_state_diagnostics(CelerState* state, int enable, uint64_t* out, uint32_t n,

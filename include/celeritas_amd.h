@@ -258,6 +258,45 @@ int celer_stepper_action_times(CelerStepper* stepper,
                                uint64_t* counts,
                                uint32_t n,
                                int reset);
+/*---------------------------------------------------------------------------*
+ * OFFLOAD FRONT END: accel/LocalTransporter.hh:52-109 without Geant4
+ *---------------------------------------------------------------------------*/
+/*
+ * SetupOptions (accel/SetupOptions.hh): max_num_tracks slots, auto_flush
+ * (0: max_num_tracks), max_steps iterations per flush; with nonfatal_flush
+ * an exceeded max_steps kills the active tracks instead of failing
+ * (LocalTransporter.cc:60-80, 247-300).
+ */
+CelerTransporter* celer_transporter_create(CelerParams* params,
+                                           uint32_t max_num_tracks,
+                                           uint32_t auto_flush,
+                                           uint32_t max_steps,
+                                           uint32_t stream_id,
+                                           int nonfatal_flush,
+                                           int action_times);
+void celer_transporter_destroy(CelerTransporter* transporter);
+/* LocalTransporter::InitializeEvent: reseed track RNG streams */
+int celer_transporter_initialize_event(CelerTransporter* transporter, int event_id);
+/* LocalTransporter::Push for each track (flushes at auto_flush) */
+int celer_transporter_push(CelerTransporter* transporter,
+                           const CelerPrimary* tracks,
+                           uint32_t count);
+/* LocalTransporter::Flush: transport buffered tracks to completion */
+int celer_transporter_flush(CelerTransporter* transporter);
+/* LocalTransporter::Finalize: fails if tracks remain buffered */
+int celer_transporter_finalize(CelerTransporter* transporter);
+/* LocalTransporter::GetBufferSize */
+uint32_t celer_transporter_buffer_size(const CelerTransporter* transporter);
+/* The transporter's state (tallies, fields), owned by the transporter */
+CelerState* celer_transporter_state(CelerTransporter* transporter);
+
+/*
+ * Copy a device buffer of `bytes` (rounded down to 16 B) `reps` times with
+ * a one-element-per-thread kernel; *ms receives the mean kernel time. The
+ * reference point for converting hardware memory counters into bytes (no
+ * counterpart in the reference). Returns 0 or a negative hipError_t value.
+ */
+int celer_calibration_copy(size_t bytes, uint32_t reps, float* ms);
 /*
  * Per-kernel work counters (tracks processed by pre-step, charged and
  * neutral along-step, boundary crossings, ...; up to 8 values). enable: 1
