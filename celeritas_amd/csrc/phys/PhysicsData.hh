@@ -150,6 +150,44 @@ struct LivermoreElement
     std::uint32_t num_shells;  //!< 0: no data
 };
 
+//! Urban MSC material coefficients (UrbanMscParams::calc_material_data)
+struct UrbanMscMaterial
+{
+    double stepmin_coeff[2];
+    double theta_coeff[2];
+    double tail_coeff[3];
+    double tail_corr;
+};
+
+//! Urban MSC particle-material data: [material][e-, e+]
+struct UrbanMscParMat
+{
+    double scaled_zeff;  //!< a Z^b
+    double d_over_r;  //!< maximum distance over range
+};
+
+//! Urban energy-loss fluctuation oscillators per material
+//! (em/data/FluctuationData.hh, FluctuationParams.cc)
+struct UrbanFluctuation
+{
+    double binding_energy[2];  //!< [MeV]
+    double log_binding_energy[2];
+    double oscillator_strength[2];
+};
+
+//! Multiple-scattering options and table index
+struct MscScalars
+{
+    double low_energy_limit{0};  //!< [MeV]
+    double high_energy_limit{0};  //!< [MeV]
+    double range_factor{0.04};
+    double safety_factor{0.6};
+    double lambda_limit{0.1};  //!< [cm]
+    std::uint32_t xs_grid{invalid_id};  //!< first XsGrid: [material][e-, e+]
+    std::uint8_t enabled{0};
+    std::uint8_t pad_[3]{};
+};
+
 //! Component of a material
 struct ElementComponent
 {
@@ -229,6 +267,12 @@ struct PhysicsParamsRef
     std::uint32_t const* sb_argmax{nullptr};  //!< per-x argmax of SB tables
     LivermoreElement const* pe_elements{nullptr};  //!< per element
     LivermoreShell const* pe_shells{nullptr};
+    // Along-step models
+    MscScalars msc{};
+    UrbanMscMaterial const* msc_material{nullptr};  //!< per material
+    UrbanMscParMat const* msc_par_mat{nullptr};  //!< [material][e-, e+]
+    UrbanFluctuation const* fluct{nullptr};  //!< per material
+    std::uint32_t fluct_enabled{0};
     ModelData model{};
 };
 

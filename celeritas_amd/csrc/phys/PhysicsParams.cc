@@ -248,6 +248,12 @@ struct PhysicsParams::HostData
     DeviceVector<std::uint32_t> d_sb_argmax;
     std::vector<LivermoreElement> pe_elements;
     std::vector<LivermoreShell> pe_shells;
+    std::vector<UrbanFluctuation> fluct;
+    std::vector<UrbanMscMaterial> msc_material;
+    std::vector<UrbanMscParMat> msc_par_mat;
+    DeviceVector<UrbanFluctuation> d_fluct;
+    DeviceVector<UrbanMscMaterial> d_msc_material;
+    DeviceVector<UrbanMscParMat> d_msc_par_mat;
     DeviceVector<LivermoreElement> d_pe_elements;
     DeviceVector<LivermoreShell> d_pe_shells;
 
@@ -448,6 +454,9 @@ PhysicsParams::PhysicsParams(JsonValue const& inp, ActionRegistry* reg)
             r.mean_exc_energy = m.get("mean_exc_energy", 0.0);
             r.log_mean_exc_energy
                 = r.mean_exc_energy > 0 ? std::log(r.mean_exc_energy) : 0;
+            // Stored as exp(log I), as MaterialParams does
+            if (r.mean_exc_energy > 0)
+                r.mean_exc_energy = std::exp(r.log_mean_exc_energy);
             r.rad_length = m.get("rad_length", 0.0);
             r.zeff = m.get("zeff", 0.0);
             r.elem_begin = static_cast<std::uint32_t>(h.components.size());
@@ -643,6 +652,94 @@ PhysicsParams::PhysicsParams(JsonValue const& inp, ActionRegistry* reg)
         ref_.model.inv_electron_mass = 1 / ref_.model.electron_mass;
     }
 
+    // Energy-loss fluctuations (FluctuationParams.cc): two oscillators
+    if (inp.has("options") && inp.at("options").get_bool("eloss_fluctuation", false))
+    {
+        CELER_VALIDATE(ref_.electron_id != invalid_id,
+                       << "energy-loss fluctuations need electrons");
+        ref_.fluct_enabled = 1;
+        for (auto const& m : h.materials)
+        {
+            UrbanFluctuation f{};
+            double const avg_z = m.number_density > 0 ? m.electron_density / m.number_density
+                                                      : 0;
+            f.oscillator_strength[1] = avg_z > 2 ? 2 / avg_z : 0;
+            f.oscillator_strength[0] = 1 - f.oscillator_strength[1];
+            f.binding_energy[1] = 1e-5 * (avg_z * avg_z);
+            f.binding_energy[0] = std::pow(
+                m.mean_exc_energy / std::pow(f.binding_energy[1], f.oscillator_strength[1]),
+                1 / f.oscillator_strength[0]);
+            f.log_binding_energy[1] = std::log(f.binding_energy[1]);
+            f.log_binding_energy[0] = std::log(f.binding_energy[0]);
+            h.fluct.push_back(f);
+        }
+    }
+
+    // Urban multiple scattering (UrbanMscParams.cc)
+    if (inp.has("msc"))
+    {
+        auto const& mj = inp.at("msc");
+        CELER_VALIDATE(mj.get_str("model", "urban") == "urban",
+                       << "only the Urban MSC model is available");
+        CELER_VALIDATE(mj.get_str("step_limit_algorithm", "safety") == "safety",
+                       << "only the 'safety' MSC step limit algorithm is available");
+        CELER_VALIDATE(ref_.electron_id != invalid_id && ref_.positron_id != invalid_id,
+                       << "MSC needs electrons and positrons");
+        MscScalars& ms = ref_.msc;
+        ms.range_factor = mj.get("range_factor", 0.04);
+        ms.safety_factor = mj.get("safety_factor", 0.6);
+        ms.lambda_limit = mj.get("lambda_limit", 0.1);
+        CELER_VALIDATE(ms.range_factor > 0 && ms.range_factor < 1 && ms.safety_factor >= 0.1
+                           && ms.lambda_limit > 0,
+                       << "invalid MSC options");
+        auto eg = grids_of(mj, "electron", nullptr, nullptr, num_materials_);
+        auto pg = grids_of(mj, "positron", nullptr, nullptr, num_materials_);
+        CELER_VALIDATE(!eg.empty() && eg[0] && !pg.empty() && pg[0],
+                       << "MSC needs cross-section grids for e- and e+");
+        std::vector<GridHost> interleaved;
+        for (std::size_t m = 0; m < num_materials_; ++m)
+        {
+            CELER_VALIDATE(eg[m] && pg[m], << "missing MSC grid for material " << m);
+            interleaved.push_back(eg[m]);
+            interleaved.push_back(pg[m]);
+        }
+        ms.xs_grid = h.add_grids(interleaved);
+        ms.low_energy_limit = std::exp(eg[0].front);
+        ms.high_energy_limit = std::exp(eg[0].back);
+        ms.enabled = 1;
+        auto poly2 = [](double c0, double c1, double c2, double x) {
+            return std::fma(x, std::fma(x, c2, c1), c0);
+        };
+        auto fastpow = [](double a, double b) { return std::exp(b * std::log(a)); };
+        for (auto const& m : h.materials)
+        {
+            double const zeff = m.zeff;
+            CELER_VALIDATE(zeff > 0, << "MSC needs the effective Z of every material");
+            UrbanMscMaterial d{};
+            d.stepmin_coeff[0] = 1e3 * 27.725 / (1 + 0.203 * zeff);
+            d.stepmin_coeff[1] = 1e3 * 6.152 / (1 + 0.111 * zeff);
+            double const z16 = fastpow(zeff, 1.0 / 6.0);
+            double const fz = poly2(0.990395, -0.168386, 0.093286, z16);
+            d.theta_coeff[0] = fz * (1 - 8.7780e-2 / zeff);
+            d.theta_coeff[1] = fz * (4.0780e-2 + 1.7315e-4 * zeff);
+            double const z13 = z16 * z16;
+            d.tail_coeff[0] = poly2(2.3785, -4.1981e-1, 6.3100e-2, z13);
+            d.tail_coeff[1] = poly2(4.7526e-1, 1.7694, -3.3885e-1, z13);
+            d.tail_coeff[2] = poly2(2.3683e-1, -1.8111, 3.2774e-1, z13);
+            d.tail_corr = poly2(1.7888e-2, 1.9659e-2, -2.6664e-3, z13);
+            h.msc_material.push_back(d);
+            static double const a_coeff[2] = {0.87, 0.70};
+            static double const b_coeff[2] = {2.0 / 3, 1.0 / 2};
+            for (int p = 0; p < 2; ++p)
+            {
+                UrbanMscParMat pm{};
+                pm.scaled_zeff = a_coeff[p] * fastpow(zeff, b_coeff[p]);
+                pm.d_over_r = 9.6280e-1 - 8.4848e-2 * std::sqrt(zeff) + 4.3769e-3 * zeff;
+                h.msc_par_mat.push_back(pm);
+            }
+        }
+    }
+
     // Upload
     h.d_particle = to_device(h.particle);
     h.d_grids = to_device(h.grids);
@@ -657,6 +754,12 @@ PhysicsParams::PhysicsParams(JsonValue const& inp, ActionRegistry* reg)
     ref_.sb_argmax = h.d_sb_argmax.data();
     h.d_pe_elements = to_device(h.pe_elements);
     h.d_pe_shells = to_device(h.pe_shells);
+    h.d_fluct = to_device(h.fluct);
+    h.d_msc_material = to_device(h.msc_material);
+    h.d_msc_par_mat = to_device(h.msc_par_mat);
+    ref_.fluct = h.d_fluct.data();
+    ref_.msc_material = h.d_msc_material.data();
+    ref_.msc_par_mat = h.d_msc_par_mat.data();
     ref_.pe_elements = h.d_pe_elements.data();
     ref_.pe_shells = h.d_pe_shells.data();
     ref_.num_particles = this->num_particles();

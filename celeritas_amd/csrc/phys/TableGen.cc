@@ -384,6 +384,29 @@ double ioni_dedx(Material const& m, double energy, double cut, bool electron)
            / beta2;
 }
 
+/*!
+ * Transport cross section per atom [cm^2] of the screened Rutherford DCS
+ * with Moliere's screening angle:
+ *   sigma_tr = 2 pi Z(Z+1) r_e^2 (mc^2)^2 / (beta^2 (pc)^2)
+ *              [ln(1 + 1/A) - 1/(1 + A)],
+ *   A = (hbar c / (2 pc a_TF))^2 (1.13 + 3.76 (alpha Z / beta)^2),
+ *   a_TF = 0.8853 a_0 Z^(-1/3).
+ */
+double msc_transport_xs(Element const& el, double energy)
+{
+    constexpr double bohr_radius = 0.529177210903e-8;  // [cm]
+    double const pc2 = energy * (energy + 2 * electron_mass);
+    double const total = energy + electron_mass;
+    double const beta2 = pc2 / (total * total);
+    double const z = el.z;
+    double const a_tf = 0.8853 * bohr_radius / el.cbrt_z;
+    double const az = alpha * z;
+    double const screen = hbarc * hbarc / (4 * pc2 * a_tf * a_tf)
+                          * (1.13 + 3.76 * az * az / beta2);
+    double const c = z * (z + 1) * r_e * r_e * electron_mass * electron_mass / (beta2 * pc2);
+    return 2 * pi * c * (std::log1p(1 / screen) - 1 / (1 + screen));
+}
+
 //---------------------------------------------------------------------------//
 // PHOTOELECTRIC SHELL MODEL
 double pe_shape(double e)
@@ -886,10 +909,30 @@ JsonValue generate_physics(JsonValue const& spec)
     }
     out.obj["cutoffs"] = cuts;
     JsonValue options = JsonValue::object();
+    options.obj["eloss_fluctuation"] = JsonValue::boolean(true);
     for (auto const& kv : opts.obj)
-        if (kv.first != "bins_per_decade" && kv.first != "emin" && kv.first != "emax")
+        if (kv.first != "bins_per_decade" && kv.first != "emin" && kv.first != "emax"
+            && kv.first != "msc")
             options.obj[kv.first] = kv.second;
     out.obj["options"] = options;
+
+    // Urban MSC: transport cross sections scaled by E^2 per material
+    if (opts.get_bool("msc", true))
+    {
+        JsonValue msc = JsonValue::object();
+        msc.obj["model"] = JsonValue::string("urban");
+        msc.obj["step_limit_algorithm"] = JsonValue::string("safety");
+        msc.obj["range_factor"] = JsonValue::number(0.04);
+        msc.obj["safety_factor"] = JsonValue::number(0.6);
+        msc.obj["lambda_limit"] = JsonValue::number(0.1);
+        JsonValue grids = per_material([&](Material const& m, std::size_t, double e) {
+            return atom_sum(m, [e](Element const& el) { return msc_transport_xs(el, e); })
+                   * e * e;
+        });
+        msc.obj["electron"] = grids;
+        msc.obj["positron"] = grids;
+        out.obj["msc"] = msc;
+    }
     out.obj["processes"] = processes;
     return out;
 }

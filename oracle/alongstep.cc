@@ -1,0 +1,716 @@
+//----------------------------------*-C++-*----------------------------------//
+// ORACLE — CPU restatement used only as a checker by tests/, smoke() and the
+// cpu_baseline leg of bench.py. Never linked into the product library.
+//---------------------------------------------------------------------------//
+//! \file oracle/alongstep.cc
+//! Along-step action for charged tracks without a field, restated from
+//! global/alongstep/AlongStepGeneralLinearAction.cc and its appliers:
+//! - MSC: em/msc/UrbanMsc.hh (is_applicable, limit_step, apply_step),
+//!   em/msc/detail/{UrbanMscHelper,UrbanMscSafetyStepLimit,MscStepToGeo,
+//!   MscStepFromGeo,UrbanMscScatter,UrbanPositronCorrector}.hh
+//! - propagation: alongstep/detail/PropagationApplier.hh:95-195
+//! - energy loss: alongstep/detail/{FluctELoss,MeanELoss,ElossApplier}.hh,
+//!   em/distribution/{EnergyLossHelper,EnergyLossUrbanDistribution,
+//!   EnergyLossGaussianDistribution}.hh, random/distribution/
+//!   {Normal,Poisson,UniformReal,Bernoulli}Distribution.hh
+//! - TimeUpdater.hh, TrackUpdater.hh
+//!
+//! Pinned by: the Urban fluctuation golden values of
+//! EnergyLossHelper.test.cc:238-285 (mean loss and the mt19937 draw count of
+//! 10,000 samples, through orc_urban_fluct_sample). The MSC pieces have no
+//! golden values the container can read (the reference's MSC tests load a
+//! ROOT file): parity unpinned beyond the GPU-vs-oracle comparisons and the
+//! invariants of tests/test_oracle_msc.py.
+//---------------------------------------------------------------------------//
+#include <algorithm>
+#include <cmath>
+#include <random>
+#include <stdexcept>
+
+#include "alongstep.hh"
+#include "em.hh"
+#include "physics.hh"
+#include "problem.hh"
+
+namespace orc
+{
+namespace
+{
+constexpr double pi = 3.14159265358979323846;
+constexpr double c_light = 29979245800.0;  // [cm/s]
+
+//---------------------------------------------------------------------------//
+// Distributions
+template<class U>
+struct NormalDist
+{
+    double mean, stddev;
+    double spare = 0;
+    bool has_spare = false;
+    double operator()(U& u)
+    {
+        if (has_spare)
+        {
+            has_spare = false;
+            return std::fma(spare, stddev, mean);
+        }
+        double const twopi = 2 * pi;
+        double const theta = twopi * u();
+        double const r = std::sqrt(-2 * std::log(u()));
+        spare = r * std::cos(theta);
+        has_spare = true;
+        return std::fma(r * std::sin(theta), stddev, mean);
+    }
+};
+
+template<class U>
+unsigned int poisson(double lambda, U& u)
+{
+    if (lambda <= 16)
+    {
+        int k = 0;
+        double p = std::exp(lambda);
+        do
+        {
+            ++k;
+            p *= u();
+        } while (p > 1);
+        return static_cast<unsigned int>(k - 1);
+    }
+    NormalDist<U> n{lambda, std::sqrt(lambda)};
+    double const x = n(u) + 0.5;
+    return x > 0 ? static_cast<unsigned int>(x) : 0u;
+}
+
+template<class U>
+double uniform_real(double a, double b, U& u)
+{
+    return std::fma(b - a, u(), a);
+}
+
+double fastpow(double a, double b)
+{
+    return std::exp(b * std::log(a));
+}
+
+//---------------------------------------------------------------------------//
+// Urban fluctuations
+template<class U>
+double fast_urban(double mean, double stddev, U& u)
+{
+    if (stddev <= 4 * mean)
+    {
+        NormalDist<U> n{mean, stddev};
+        double const max_loss = 2 * mean;
+        double r;
+        do
+        {
+            r = n(u);
+        } while (r <= 0 || r > max_loss);
+        return r;
+    }
+    return uniform_real(0.0, 2 * mean, u);
+}
+
+//---------------------------------------------------------------------------//
+// MSC helpers
+struct MscCtx
+{
+    PhysicsData const& pd;
+    ProcessDef const& eloss;
+    std::uint32_t mat;
+    int par;  // 0 e-, 1 e+
+    double energy;
+    double range;
+    double lambda = 0;
+
+    double mfp(double e) const
+    {
+        return 1 / (pd.msc.xs[par][mat].calc(e) / (e * e));
+    }
+    double inverse_range(double r) const { return calc_inverse_range(eloss.range[mat], r); }
+    double end_energy(double step) const
+    {
+        if (step <= range * 0.05)
+            return energy - step * eloss.dedx[mat].calc(energy);
+        return inverse_range(range - step);
+    }
+};
+
+constexpr double geom_limit = 5e-9;
+constexpr double limit_min_fix = 1e-9;
+constexpr double min_step = 1e-7;
+
+}  // namespace
+
+//---------------------------------------------------------------------------//
+template<class U>
+double urban_fluct_sample(FluctO const& par, MaterialDef const& mat, double unscaled_mean,
+                          double max_energy, double two_mebsgs, double beta_sq, U& u)
+{
+    constexpr double e0 = 1e-5;
+    constexpr double rate = 0.56;
+    constexpr double max_coll = 8;
+    double const scaling_loss = 0.5 * std::min(1e-3 / max_energy, 1.0) + 1;
+    double const mean_loss = unscaled_mean / scaling_loss;
+    double b[2] = {par.binding[0], par.binding[1]};
+    double xs_exc[2] = {0, 0};
+    if (max_energy > mat.mean_exc_energy)
+    {
+        double const w = std::log(two_mebsgs) - beta_sq;
+        double const w0 = mat.log_mean_exc_energy;
+        if (w > w0)
+        {
+            if (w > par.log_binding[1])
+            {
+                double const c = mean_loss * (1 - rate) / (w - w0);
+                for (int i = 0; i < 2; ++i)
+                    xs_exc[i] = c * par.strength[i] * (w - par.log_binding[i]) / par.binding[i];
+            }
+            else
+            {
+                xs_exc[0] = mean_loss * (1 - rate) / par.binding[0];
+            }
+            double scaling = 4;
+            if (xs_exc[0] < 42)
+                scaling = 0.5 + (scaling - 0.5) * std::sqrt(xs_exc[0] / 42);
+            b[0] *= scaling;
+            xs_exc[0] /= scaling;
+        }
+    }
+    double xs_ion = mean_loss * (max_energy - e0) / (max_energy * e0 * std::log(max_energy / e0));
+    if (xs_exc[0] + xs_exc[1] > 0)
+        xs_ion *= rate;
+
+    double exc = 0;
+    {
+        double mean = 0, var = 0;
+        for (int i = 0; i < 2; ++i)
+        {
+            if (xs_exc[i] > max_coll)
+            {
+                mean += xs_exc[i] * b[i];
+                var += xs_exc[i] * (b[i] * b[i]);
+            }
+            else if (xs_exc[i] > 0)
+            {
+                unsigned int n = poisson(xs_exc[i], u);
+                if (n > 0)
+                    exc += uniform_real(double(n - 1), double(n + 1), u) * b[i];
+            }
+        }
+        if (var > 0)
+            exc += fast_urban(mean, std::sqrt(var), u);
+    }
+    double ion = 0;
+    {
+        double const ratio = max_energy / e0;
+        double alpha = 1, mean_num = 0;
+        if (xs_ion > max_coll)
+        {
+            alpha = (xs_ion + max_coll) * ratio / (max_coll * ratio + xs_ion);
+            double const mean_loss_coll = alpha * std::log(alpha) / (alpha - 1);
+            mean_num = xs_ion * ratio * (alpha - 1) / ((ratio - 1) * alpha);
+            double const mean = mean_num * mean_loss_coll * e0;
+            double const sd = e0 * std::sqrt(xs_ion * (alpha - mean_loss_coll * mean_loss_coll));
+            ion += fast_urban(mean, sd, u);
+        }
+        if (xs_ion > 0 && ratio > alpha)
+        {
+            unsigned int n = poisson(xs_ion - mean_num, u);
+            for (; n > 0; --n)
+                ion += alpha * e0 / uniform_real(alpha / ratio, 1.0, u);
+        }
+    }
+    return scaling_loss * (exc + ion);
+}
+
+template double urban_fluct_sample<MtUniform>(FluctO const&, MaterialDef const&, double,
+                                              double, double, double, MtUniform&);
+
+//---------------------------------------------------------------------------//
+void along_step_charged_impl(State& s)
+{
+    Problem const& prob = *s.problem;
+    PhysicsData const& pd = *prob.physics;
+    ActionIds const& ids = prob.actions;
+    std::uint32_t const aid = ids.along_user;
+    for (auto& t : s.tracks)
+    {
+        if (t.status == inactive || t.along_action != aid)
+            continue;
+        XorwowUniform u{t.rng};
+        std::uint32_t const mat = prob.volume_to_material[t.geo.vol];
+        auto const& procs = pd.particle_processes[t.particle];
+        int const ep = pd.eloss_process[t.particle];
+        ProcessDef const* eloss = ep >= 0 ? &pd.processes[procs[ep]] : nullptr;
+
+        // --- MscStepLimitApplier / UrbanMsc::limit_step
+        bool const applicable = pd.msc.enabled && t.step_length > geom_limit
+                                && t.status == alive
+                                && (t.particle == pd.electron || t.particle == pd.positron)
+                                && t.energy > pd.msc.low && t.energy < pd.msc.high;
+        double true_path = 0, geom_path = 0, alpha = 0;
+        bool displaced = false;
+        MscCtx ctx{pd, *(eloss ? eloss : &pd.processes.front()), mat,
+                   t.particle == pd.electron ? 0 : 1, t.energy, t.dedx_range};
+        if (applicable)
+        {
+            ctx.lambda = ctx.mfp(t.energy);
+            double const phys_step = t.step_length;
+            bool have_path = false;
+            if (phys_step <= limit_min_fix)
+            {
+                true_path = phys_step;
+                have_path = true;
+            }
+            double safety = 0;
+            bool const on_boundary = t.geo.surf != none;
+            if (!have_path && !on_boundary)
+            {
+                double const max_step = ctx.range * pd.msc.mat[mat].d_over_r;
+                safety = geo_find_safety(prob.geo, t.geo);
+                if (safety >= max_step)
+                {
+                    true_path = phys_step;
+                    have_path = true;
+                }
+            }
+            if (!have_path)
+            {
+                displaced = true;
+                // UrbanMscSafetyStepLimit
+                bool const valid = t.msc_range_init > 0 && t.msc_range_factor > 0
+                                   && t.msc_limit_min > 0;
+                if (!valid || on_boundary)
+                {
+                    double factor = pd.msc.range_factor;
+                    double const init = std::max(ctx.range, ctx.lambda);
+                    if (ctx.lambda > pd.msc.lambda_limit)
+                        factor *= 0.75 + 0.25 * ctx.lambda / pd.msc.lambda_limit;
+                    MscDataO::Material const& md = pd.msc.mat[mat];
+                    double const e = t.energy;
+                    double xm = ctx.lambda / std::fma(e, std::fma(e, md.stepmin[1], md.stepmin[0]), 2.0);
+                    xm *= md.scaled_zeff[ctx.par];
+                    if (e < 5e-3)
+                        xm *= 0.5 + 0.5 * e / 5e-3;
+                    t.msc_range_init = init;
+                    t.msc_range_factor = factor;
+                    t.msc_limit_min = std::max(xm, limit_min_fix);
+                }
+                double const lmin = t.msc_limit_min;
+                double limit = ctx.range;
+                if (safety < ctx.range)
+                    limit = std::max(t.msc_range_factor * t.msc_range_init,
+                                     pd.msc.safety_factor * safety);
+                limit = std::max(limit, lmin);
+                if (phys_step <= limit)
+                {
+                    true_path = phys_step;
+                }
+                else if (limit == lmin)
+                {
+                    true_path = lmin;
+                }
+                else
+                {
+                    NormalDist<XorwowUniform> n{limit, 0.1 * (limit - lmin)};
+                    true_path = std::clamp(n(u), lmin, phys_step);
+                }
+            }
+            bool limited = true_path < phys_step;
+            // MscStepToGeo
+            {
+                double step;
+                alpha = 0;
+                if (true_path < min_step)
+                {
+                    step = true_path;
+                }
+                else if (true_path < ctx.range * 0.05)
+                {
+                    step = -ctx.lambda * std::expm1(-true_path / ctx.lambda);
+                }
+                else
+                {
+                    double slope;
+                    if (t.energy < pd.particles[pd.electron].mass || true_path == ctx.range)
+                    {
+                        alpha = 1 / ctx.range;
+                        slope = std::max(1 - alpha * true_path, 0.0);
+                    }
+                    else
+                    {
+                        double const e1 = ctx.inverse_range(ctx.range - true_path);
+                        double const lambda1 = ctx.mfp(e1);
+                        alpha = (ctx.lambda - lambda1) / (ctx.lambda * true_path);
+                        slope = lambda1 / ctx.lambda;
+                    }
+                    double const w = 1 + 1 / (alpha * ctx.lambda);
+                    step = (1 - fastpow(slope, w)) / (alpha * w);
+                }
+                geom_path = std::min(step, true_path);
+            }
+            if (geom_path > ctx.lambda)
+            {
+                geom_path = ctx.lambda;
+                limited = true;
+            }
+            t.step_length = geom_path;
+            if (limited)
+                t.post_action = pd.msc_action;
+        }
+
+        // --- PropagationApplier (linear)
+        if (t.step_length != 0)
+        {
+            double const orig[3] = {t.geo.pos[0], t.geo.pos[1], t.geo.pos[2]};
+            bool boundary;
+            double dist = geo_find_next_step(prob.geo, t.geo, t.step_length, &boundary);
+            if (boundary)
+                geo_move_to_boundary(t.geo);
+            else
+                geo_move_internal(t.geo, t.step_length);
+            if (t.geo.pos[0] == orig[0] && t.geo.pos[1] == orig[1] && t.geo.pos[2] == orig[2])
+            {
+                t.status = errored;
+                t.along_action = none;
+                t.post_action = ids.tracking_cut;
+                continue;
+            }
+            if (boundary)
+            {
+                t.step_length = dist;
+                t.post_action = ids.boundary;
+            }
+            else if (dist < t.step_length)
+            {
+                t.step_length = dist;
+                t.post_action = ids.propagation_limit;
+            }
+        }
+
+        // --- MscApplier / UrbanMsc::apply_step
+        if (applicable && geom_path > 0 && t.status == alive)
+        {
+            if (t.post_action == ids.boundary || t.post_action == ids.propagation_limit)
+            {
+                // MscStepFromGeo
+                geom_path = t.step_length;
+                double const gstep = geom_path;
+                double tstep;
+                if (gstep < min_step)
+                {
+                    tstep = gstep;
+                }
+                else
+                {
+                    if (alpha == 0)
+                    {
+                        tstep = -ctx.lambda * std::log1p(-gstep / ctx.lambda);
+                        if (tstep < min_step)
+                            tstep = gstep;
+                    }
+                    else
+                    {
+                        double const w = 1 + 1 / (alpha * ctx.lambda);
+                        double const x = std::min(alpha * w * gstep, 1.0);
+                        tstep = std::min((1 - fastpow(1 - x, 1 / w)) / alpha, ctx.range);
+                    }
+                    tstep = std::clamp(tstep, gstep, true_path);
+                }
+                true_path = tstep;
+                displaced = false;
+            }
+            t.step_length = true_path;
+            double safety = 0;
+            if (displaced)
+            {
+                double displ = 0.73 * std::sqrt((true_path - geom_path) * (true_path + geom_path));
+                displ = std::max(displ * (1 + 2 * 0.01), geom_limit);
+                safety = std::min(geo_find_safety(prob.geo, t.geo), displ);
+                if (safety == 0)
+                    displaced = false;
+            }
+            // UrbanMscScatter
+            MaterialDef const& m = pd.materials[mat];
+            MscDataO::Material const& md = pd.msc.mat[mat];
+            double const e_in = t.energy;
+            double e_end = 0, tau = 0, xmean = 0, x2mean = 0, theta0 = 0;
+            double lmin = t.msc_limit_min;
+            bool skip = false;
+            if (true_path == ctx.range || true_path < geom_limit)
+            {
+                skip = true;
+            }
+            else
+            {
+                e_end = ctx.end_energy(true_path);
+                if (e_end < 1e-6 || true_path <= ctx.lambda * 1e-16)
+                    skip = true;
+            }
+            if (!skip)
+            {
+                double const lambda_end = ctx.mfp(e_end);
+                double const denom = std::fabs(ctx.lambda - lambda_end) < ctx.lambda * 0.01
+                                         ? ctx.lambda
+                                         : (ctx.lambda - lambda_end)
+                                               / std::log(ctx.lambda / lambda_end);
+                tau = true_path / denom;
+                if (tau < 8)
+                {
+                    xmean = std::exp(-tau);
+                    x2mean = (1 + 2 * std::exp(-2.5 * tau)) / 3;
+                    if (lmin == 0)
+                        lmin = 1e-8;
+                    lmin = std::min(lmin, pd.msc.lambda_limit);
+                    double const me = pd.particles[pd.electron].mass;
+                    double y = std::max(lmin, true_path) / m.rad_length;
+                    if (t.particle == pd.positron)
+                    {
+                        // UrbanPositronCorrector
+                        double const z = m.zeff;
+                        double const a = std::fma(z, -4.08e-3, 0.994);
+                        double const b = std::fma(1 / z, std::fma(1 / z, 365.0, 52.6), 7.16);
+                        double const c = std::fma(z, -4.47e-3, 1.0);
+                        double const d = 1.21e-3 * z;
+                        double const mult
+                            = std::fma(z, std::fma(z, 1.84035e-4, -1.86427e-2), 1.41125);
+                        double const yy = std::sqrt(e_in * e_end) / me;
+                        double const x = std::sqrt(yy * (yy + 2) / ((yy + 1) * (yy + 1)));
+                        double corr;
+                        if (x < 0.6)
+                            corr = a * (1 - std::exp(-b * x));
+                        else if (x > 0.9)
+                            corr = c + d * std::exp(113 * (x - 1));
+                        else
+                        {
+                            double const yl = a * (1 - std::exp(-b * 0.6));
+                            double const yh = c + d * std::exp(113 * (0.9 - 1));
+                            double const y0 = (yh - yl) / (0.9 - 0.6);
+                            corr = y0 * x + (yl - y0 * 0.6);
+                        }
+                        y *= corr * mult;
+                    }
+                    double const invbetacp = std::sqrt(
+                        (e_in + me) * (e_end + me)
+                        / (e_in * (e_in + 2 * me) * e_end * (e_end + 2 * me)));
+                    theta0 = 13.6 * std::sqrt(y) * invbetacp;
+                    theta0 *= std::fma(std::log(y), md.theta[1], md.theta[0]);
+                    if (true_path < lmin)
+                        theta0 *= std::sqrt(true_path / lmin);
+                    theta0 = std::max(theta0, 0.0);
+                    if (theta0 < 1e-8)
+                    {
+                        if (!displaced)
+                            skip = true;
+                        else
+                            theta0 = 0;
+                    }
+                }
+            }
+            if (!skip)
+            {
+                auto simple = [&]() {
+                    double const a = (2 * xmean + 9 * x2mean - 3) / (2 * xmean - 3 * x2mean + 1);
+                    double const ppow = (a + 2) * xmean / a;
+                    double r;
+                    do
+                    {
+                        double const rdm = u();
+                        r = 2 * (u() < ppow ? fastpow(rdm, 1 / (a + 1)) : rdm) - 1;
+                    } while (std::fabs(r) > 1);
+                    return r;
+                };
+                double cost;
+                if (theta0 <= 0)
+                {
+                    cost = 1;
+                }
+                else if (tau >= 8)
+                {
+                    cost = uniform_real(-1.0, 1.0, u);
+                }
+                else if (2 * e_end < e_in || theta0 > pi / 6)
+                {
+                    cost = simple();
+                }
+                else
+                {
+                    double const maxtau = true_path < lmin ? lmin / ctx.lambda : tau;
+                    double const uu = fastpow(maxtau, 1.0 / 6);
+                    double const radlen_mfp = true_path / (tau * m.rad_length);
+                    double xsi = std::fma(uu, std::fma(uu, md.tail[2], md.tail[1]), md.tail[0])
+                                 + md.tail_corr * std::log(radlen_mfp);
+                    xsi = std::max(xsi, 1.9);
+                    double const ea = std::exp(-xsi);
+                    double const sh = 2 * std::sin(0.5 * theta0);
+                    double const x = sh * sh;
+                    double const xmean_1 = 1 - x * (1 + (xsi * ea) / (1 - ea));
+                    if (xmean_1 <= 0.999 * xmean)
+                    {
+                        cost = simple();
+                    }
+                    else
+                    {
+                        double c = xsi;
+                        if (std::fabs(xsi - 3) < 0.001)
+                            c = 3.001;
+                        else if (std::fabs(xsi - 2) < 0.001)
+                            c = 2.001;
+                        double const b1 = 2 + (c - xsi) * x;
+                        double const d = fastpow(c * x / b1, c - 1);
+                        double const x0 = 1 - xsi * x;
+                        double const xmean_2 = (x0 + d - (c * x - b1 * d) / (c - 2)) / (1 - d);
+                        double const f2x0 = (c - 1) / (c * (1 - d));
+                        double const prob = f2x0 / (ea / (1 - ea) + f2x0);
+                        double const qprob = xmean / (prob * xmean_1 + (1 - prob) * xmean_2);
+                        if (u() >= qprob)
+                        {
+                            cost = uniform_real(-1.0, 1.0, u);
+                        }
+                        else if (u() < prob)
+                        {
+                            cost = 1 + std::log(uniform_real(ea, 1.0, u)) * x;
+                        }
+                        else
+                        {
+                            double var = (1 - d) * u();
+                            if (var < 0.01 * d)
+                            {
+                                var /= (d * (c - 1));
+                                cost = -1 + var * (1 - 0.5 * var * c) * (2 + (c - xsi) * x);
+                            }
+                            else
+                            {
+                                cost = x * (c - xsi - c * fastpow(var + d, -1 / (c - 1))) + 1;
+                            }
+                        }
+                    }
+                }
+                double phi = uniform_real(0.0, 2 * pi, u);
+                bool moved = false;
+                double disp[3] = {0, 0, 0};
+                if (displaced)
+                {
+                    double length = 0.73 * std::sqrt((true_path - geom_path) * (true_path + geom_path));
+                    length = std::min(length, (1 - 0.01) * safety);
+                    if (length >= geom_limit)
+                    {
+                        double const psi = -std::log(1 - u() * 0.9988703417569197) / 2.160;
+                        double const dphi = u() < 0.5 ? phi + psi : phi - psi;
+                        double const local[3] = {std::cos(dphi), std::sin(dphi), 0};
+                        rotate(local, t.geo.dir, disp);
+                        for (auto& v : disp)
+                            v *= length;
+                        moved = true;
+                    }
+                }
+                double local[3], newdir[3];
+                from_spherical(cost, phi, local);
+                rotate(local, t.geo.dir, newdir);
+                geo_set_dir(prob.geo, t.geo, newdir);
+                if (moved)
+                {
+                    double const x[3] = {t.geo.pos[0] + disp[0], t.geo.pos[1] + disp[1],
+                                         t.geo.pos[2] + disp[2]};
+                    geo_move_internal_to(t.geo, x);
+                }
+            }
+        }
+
+        // --- TimeUpdater
+        {
+            double const mass = pd.particles[t.particle].mass;
+            double const inv_gamma = mass / (t.energy + mass);
+            double const beta_sq = 1 - inv_gamma * inv_gamma;
+            double const speed = std::sqrt(beta_sq) * c_light;
+            if (speed > 0)
+                t.time += t.step_length / speed;
+        }
+
+        // --- ElossApplier with FluctELoss / MeanELoss
+        if (eloss && t.energy > 0)
+        {
+            bool const apply_cut = t.post_action != ids.boundary;
+            double const lowest = pd.lowest_electron_energy;
+            double const e = t.energy;
+            double dep;
+            if (apply_cut && e < lowest)
+            {
+                dep = e;
+            }
+            else
+            {
+                double ml = t.step_length * eloss->dedx[mat].calc(e);
+                if (ml >= e * pd.linear_loss_limit)
+                {
+                    ml = (t.step_length == t.dedx_range)
+                             ? e
+                             : e - calc_inverse_range(eloss->range[mat], t.dedx_range - t.step_length);
+                }
+                double loss = ml;
+                if (pd.fluct && loss < e)
+                {
+                    if (ml >= 1e-5)
+                    {
+                        double const me = pd.particles[pd.electron].mass;
+                        double const mass = pd.particles[t.particle].mass;
+                        double const gamma = 1 + e / mass;
+                        double const ig = mass / (e + mass);
+                        double const beta_sq = 1 - ig * ig;
+                        double const two_mebsgs = 2 * me * beta_sq * (gamma * gamma);
+                        double max_transfer;
+                        if (t.particle == pd.electron)
+                        {
+                            max_transfer = 0.5 * e;
+                        }
+                        else
+                        {
+                            double const ratio = me / mass;
+                            max_transfer = two_mebsgs / (1 + ratio * (2 * gamma + ratio));
+                        }
+                        double const max_energy
+                            = std::min(pd.cutoff_energy[pd.electron][mat], max_transfer);
+                        if (max_energy > 1e-5)
+                        {
+                            loss = urban_fluct_sample(pd.fluct_params[mat], pd.materials[mat], ml,
+                                                      max_energy, two_mebsgs, beta_sq, u);
+                        }
+                    }
+                    if (loss >= e)
+                        loss = apply_cut ? e : ml;
+                }
+                dep = (apply_cut && e - loss <= lowest) ? e : loss;
+            }
+            if (dep > 0)
+            {
+                t.edep += dep;
+                t.energy -= dep;
+            }
+            if (t.energy == 0)
+            {
+                bool has_at_rest = false;
+                for (auto pi_ : procs)
+                    has_at_rest = has_at_rest || pd.processes[pi_].cls == "annihilation";
+                if (!has_at_rest)
+                {
+                    t.status = killed;
+                    t.post_action = pd.range_action;
+                }
+                else
+                {
+                    t.post_action = pd.discrete_action;
+                }
+            }
+        }
+
+        // --- TrackUpdater
+        if (t.status == alive && t.post_action != pd.discrete_action)
+            t.interaction_mfp -= t.step_length * t.macro_xs;
+        ++t.num_steps;
+    }
+}
+
+//---------------------------------------------------------------------------//
+}  // namespace orc

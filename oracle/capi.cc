@@ -5,6 +5,7 @@
 //! \file oracle/capi.cc
 //! C entry points of liboracle.so for the Python test harness.
 //---------------------------------------------------------------------------//
+#include <algorithm>
 #include <cstring>
 #include <string>
 
@@ -465,6 +466,7 @@ std::uint32_t orc_state_track_counters(void* s, std::uint32_t* out, std::uint32_
 }
 }
 
+#include "alongstep.hh"
 #include "em.hh"
 
 extern "C" {
@@ -623,6 +625,36 @@ int orc_interactor_stress(void* problem, void* mt, const char* model,
             }
         }
         *count = u.count;
+    });
+}
+
+//! EnergyLossHelper + EnergyLossUrbanDistribution for an electron: n
+//! samples on a default-seeded mt19937; out = {max_energy, beta_sq, mean
+//! loss, 32-bit draws}
+int orc_urban_fluct(void* problem, std::uint32_t material, double energy, double mean_loss,
+                    std::uint32_t n, double* out)
+{
+    return guard([&] {
+        auto const& pd = *static_cast<OrcProblem*>(problem)->p->physics;
+        if (!pd.fluct)
+            throw std::runtime_error("fluctuations are off");
+        double const me = pd.particles.at(pd.electron).mass;
+        double const gamma = 1 + energy / me;
+        double const ig = me / (energy + me);
+        double const beta_sq = 1 - ig * ig;
+        double const two_mebsgs = 2 * me * beta_sq * (gamma * gamma);
+        double const max_energy
+            = std::min(pd.cutoff_energy[pd.electron][material], 0.5 * energy);
+        std::mt19937 mt;
+        MtUniform u{mt};
+        double sum = 0;
+        for (std::uint32_t i = 0; i < n; ++i)
+            sum += urban_fluct_sample(pd.fluct_params.at(material), pd.materials.at(material),
+                                      mean_loss, max_energy, two_mebsgs, beta_sq, u);
+        out[0] = max_energy;
+        out[1] = beta_sq;
+        out[2] = sum / n;
+        out[3] = double(u.count);
     });
 }
 

@@ -8,6 +8,7 @@
 #include <limits>
 #include <stdexcept>
 
+#include "alongstep.hh"
 #include "em.hh"
 #include "problem.hh"
 
@@ -251,6 +252,14 @@ void build_physics(Json const& inp, PhysicsData& out)
             d.number_density = m.num("number_density", 0);
             d.electron_density = m.num("electron_density", 0);
             d.mean_exc_energy = m.num("mean_exc_energy", 0);
+            if (d.mean_exc_energy > 0)
+            {
+                // MaterialParams keeps I as exp(log I)
+                d.log_mean_exc_energy = std::log(d.mean_exc_energy);
+                d.mean_exc_energy = std::exp(d.log_mean_exc_energy);
+            }
+            d.rad_length = m.num("rad_length", 0);
+            d.zeff = m.num("zeff", 0);
             if (auto comps = m.find("elements"))
                 for (auto const& c : comps->a)
                     d.elements.emplace_back(
@@ -271,6 +280,61 @@ void build_physics(Json const& inp, PhysicsData& out)
             = opts->num("lowest_electron_energy", out.lowest_electron_energy);
         out.fixed_step_limiter = opts->num("fixed_step_limiter", 0);
         out.apply_post_interaction_cut = opts->flag("apply_cuts", false);
+        out.fluct = opts->flag("eloss_fluctuation", false);
+    }
+    if (out.fluct)
+    {
+        // FluctuationParams.cc: two oscillators per material
+        for (auto const& m : out.materials)
+        {
+            FluctO f;
+            double const avg_z = m.electron_density / m.number_density;
+            f.strength[1] = avg_z > 2 ? 2 / avg_z : 0;
+            f.strength[0] = 1 - f.strength[1];
+            f.binding[1] = 1e-5 * (avg_z * avg_z);
+            f.binding[0] = std::pow(m.mean_exc_energy / std::pow(f.binding[1], f.strength[1]),
+                                    1 / f.strength[0]);
+            f.log_binding[1] = std::log(f.binding[1]);
+            f.log_binding[0] = std::log(f.binding[0]);
+            out.fluct_params.push_back(f);
+        }
+    }
+    if (auto mj = inp.find("msc"))
+    {
+        // UrbanMscParams.cc and its calc_material_data
+        MscDataO& msc = out.msc;
+        msc.enabled = true;
+        msc.range_factor = mj->num("range_factor", 0.04);
+        msc.safety_factor = mj->num("safety_factor", 0.6);
+        msc.lambda_limit = mj->num("lambda_limit", 0.1);
+        msc.xs[0] = grids_of(*mj, "electron", nullptr, nullptr, num_mats);
+        msc.xs[1] = grids_of(*mj, "positron", nullptr, nullptr, num_mats);
+        msc.low = std::exp(msc.xs[0].at(0).front);
+        msc.high = std::exp(msc.xs[0].at(0).back);
+        auto poly2 = [](double c0, double c1, double c2, double x) {
+            return std::fma(x, std::fma(x, c2, c1), c0);
+        };
+        auto fastpow = [](double a, double b) { return std::exp(b * std::log(a)); };
+        for (auto const& m : out.materials)
+        {
+            MscDataO::Material d;
+            double const zeff = m.zeff;
+            d.stepmin[0] = 1e3 * 27.725 / (1 + 0.203 * zeff);
+            d.stepmin[1] = 1e3 * 6.152 / (1 + 0.111 * zeff);
+            double const z16 = fastpow(zeff, 1.0 / 6.0);
+            double const fz = poly2(0.990395, -0.168386, 0.093286, z16);
+            d.theta[0] = fz * (1 - 8.7780e-2 / zeff);
+            d.theta[1] = fz * (4.0780e-2 + 1.7315e-4 * zeff);
+            double const z13 = z16 * z16;
+            d.tail[0] = poly2(2.3785, -4.1981e-1, 6.3100e-2, z13);
+            d.tail[1] = poly2(4.7526e-1, 1.7694, -3.3885e-1, z13);
+            d.tail[2] = poly2(2.3683e-1, -1.8111, 3.2774e-1, z13);
+            d.tail_corr = poly2(1.7888e-2, 1.9659e-2, -2.6664e-3, z13);
+            d.scaled_zeff[0] = 0.87 * fastpow(zeff, 2.0 / 3);
+            d.scaled_zeff[1] = 0.70 * fastpow(zeff, 1.0 / 2);
+            d.d_over_r = 9.6280e-1 - 8.4848e-2 * std::sqrt(zeff) + 4.3769e-3 * zeff;
+            msc.mat.push_back(d);
+        }
     }
     // Cutoffs: {"particle label": [{"energy": E}, ...per material]}
     out.cutoff_energy.assign(out.particles.size(), std::vector<double>(num_mats, 0));
@@ -353,6 +417,7 @@ void build_physics(Json const& inp, PhysicsData& out)
 void PhysicsData::finalize_actions(ActionIds const& ids)
 {
     discrete_action = ids.discrete_select;
+    msc_action = ids.msc_range;
     range_action = ids.eloss_range;
     integral_rejection_action = ids.integral_rejected;
     failure_action = ids.failure;
@@ -716,75 +781,10 @@ double calc_inverse_range(XsGridO const& g, double range)
     return std::fma(slope, (-r[i]) + range, el);
 }
 
-namespace
-{
-//! ElossApplier with MeanELoss (ElossApplier.hh:41-90, MeanELoss.hh:65-92)
-void apply_mean_eloss(State& s, Track& t)
-{
-    PhysicsData const& pd = phys(s);
-    ActionIds const& ids = s.problem->actions;
-    int const ep = pd.eloss_process[t.particle];
-    if (t.status == errored || ep < 0 || t.energy == 0)
-        return;
-    auto const& procs = pd.particle_processes[t.particle];
-    ProcessDef const& p = pd.processes[procs[ep]];
-    std::uint32_t const mat = material_of(s, t);
-    bool const apply_cut = t.post_action != ids.boundary;
-    double const lowest = pd.lowest_electron_energy;
-    double deposited;
-    if (apply_cut && t.energy < lowest)
-    {
-        deposited = t.energy;
-    }
-    else
-    {
-        // calc_mean_energy_loss (PhysicsStepUtils.hh:177-233)
-        double const e = t.energy;
-        double eloss = t.step_length * p.dedx[mat].calc(e);
-        if (eloss >= e * pd.linear_loss_limit)
-        {
-            if (t.step_length == t.dedx_range)
-                eloss = e;
-            else
-                eloss = e - calc_inverse_range(p.range[mat], t.dedx_range - t.step_length);
-        }
-        deposited = (apply_cut && e - eloss <= lowest) ? e : eloss;
-    }
-    if (deposited > 0)
-    {
-        t.edep += deposited;
-        t.energy -= deposited;
-    }
-    if (t.energy == 0)
-    {
-        bool has_at_rest = false;
-        for (auto pi : procs)
-            has_at_rest = has_at_rest || pd.processes[pi].cls == "annihilation";
-        if (!has_at_rest)
-        {
-            t.status = killed;
-            t.post_action = pd.range_action;
-        }
-        else
-        {
-            t.post_action = pd.discrete_action;
-        }
-    }
-}
-}  // namespace
-
 void along_step_charged(State& s)
 {
-    std::uint32_t const aid = s.problem->actions.along_user;
-    for (auto& t : s.tracks)
-    {
-        if (t.status == inactive || t.along_action != aid)
-            continue;
-        propagate_linear(s, t);
-        update_time(s, t);
-        apply_mean_eloss(s, t);
-        update_track(s, t);
-    }
+    // Urban MSC, linear propagation, fluctuations: oracle/alongstep.cc
+    along_step_charged_impl(s);
 }
 
 void calo_tally(State& s)

@@ -102,8 +102,38 @@ struct MaterialDef
     std::string label;
     double number_density = 0;  // [1/cm^3]
     double electron_density = 0;  // [1/cm^3]
-    double mean_exc_energy = 0;  // [MeV]
+    double mean_exc_energy = 0;  // [MeV], exp(log I)
+    double log_mean_exc_energy = 0;
+    double rad_length = 0;  // [cm]
+    double zeff = 0;
     std::vector<std::pair<std::uint32_t, double>> elements;  // (id, fraction)
+};
+
+//! Urban MSC data (em/data/UrbanMscData.hh, params/UrbanMscParams.cc)
+struct MscDataO
+{
+    struct Material
+    {
+        double stepmin[2] = {0, 0};
+        double theta[2] = {0, 0};
+        double tail[3] = {0, 0, 0};
+        double tail_corr = 0;
+        double scaled_zeff[2] = {0, 0};  // e-, e+
+        double d_over_r = 0;
+    };
+    bool enabled = false;
+    double low = 0, high = 0;  // energy limits [MeV]
+    double range_factor = 0.04, safety_factor = 0.6, lambda_limit = 0.1;
+    std::vector<XsGridO> xs[2];  // [e-, e+][material]: sigma_tr * E^2
+    std::vector<Material> mat;
+};
+
+//! Urban fluctuation oscillators (FluctuationParams.cc)
+struct FluctO
+{
+    double binding[2] = {0, 0};
+    double log_binding[2] = {0, 0};
+    double strength[2] = {0, 0};
 };
 
 struct ModelDef
@@ -148,7 +178,12 @@ struct PhysicsData
     double lowest_electron_energy = 0.001;
     double fixed_step_limiter = 0;
     bool apply_post_interaction_cut = false;
+    // Along-step models
+    MscDataO msc;
+    bool fluct = false;
+    std::vector<FluctO> fluct_params;  // per material
     // Resolved action IDs
+    std::uint32_t msc_action = none;
     std::uint32_t discrete_action = none, range_action = none;
     std::uint32_t integral_rejection_action = none, failure_action = none;
     std::uint32_t fixed_step_action = none;
