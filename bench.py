@@ -159,7 +159,8 @@ def main():
             "primaries": args.primaries,
             "slots": args.slots,
             "physics": "Compton, Livermore PE, Bethe-Heitler, e+ annihilation, "
-                       "Moller-Bhabha, SB/relativistic brems; mean continuous loss",
+                       "Moller-Bhabha ionisation, SB/relativistic brems, Urban MSC, "
+                       "Urban energy-loss fluctuations",
             "edep_total_mev": float(edep.sum()),
         },
     }
