@@ -59,6 +59,10 @@ def parse_args():
     ap.add_argument("--primaries", type=int, default=10000)
     ap.add_argument("--slots", type=int, default=1 << 20)
     ap.add_argument("--seed", type=int, default=12345)
+    ap.add_argument("--track-order", default="none",
+                    help="track slot ordering (none, init_charge, reindex_status, "
+                         "reindex_particle_type, reindex_along_step_action, "
+                         "reindex_step_limit_action, reindex_both_action)")
     ap.add_argument("--cpu-primaries", type=int, default=48,
                     help="primaries in the CPU baseline sample (0: skip)")
     ap.add_argument("--no-roofline", action="store_true")
@@ -113,7 +117,8 @@ def main():
     log("generating physics tables")
     physics = amd.generate_physics(wl.testem3_physics_spec())
     problem = wl.testem3_problem(physics=physics, init_capacity=1 << 26,
-                                 max_events=max(args.primaries, 1))
+                                 max_events=max(args.primaries, 1),
+                                 track_order=args.track_order)
     primaries = wl.testem3_primaries(args.primaries, seed=args.seed)
     params = amd.CoreParams(problem)
 
@@ -158,6 +163,7 @@ def main():
                         "50/50, log-uniform 1-100 GeV, 5 deg cone, 2^20 slots, FP64",
             "primaries": args.primaries,
             "slots": args.slots,
+            "track_order": args.track_order,
             "physics": "Compton, Livermore PE, Bethe-Heitler, e+ annihilation, "
                        "Moller-Bhabha ionisation, SB/relativistic brems, Urban MSC, "
                        "Urban energy-loss fluctuations",

@@ -67,9 +67,9 @@ class CCounters(ctypes.Structure):
             "secondary_stack_size",
             "error_flags",
             "steps_taken",
+            "num_new_neutral",
             "pad0",
             "pad1",
-            "pad2",
         )
     ]
 

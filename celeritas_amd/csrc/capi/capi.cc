@@ -140,7 +140,7 @@ std::pair<void*, std::size_t> field_ptr(CoreStateRef const& s,
         case CELER_FIELD_TRACK_SLOTS: return {s.track_slots, 4 * n};
         case CELER_FIELD_ACTION_OFFSETS:
             return {s.action_offsets,
-                    4 * std::size_t(p.action_reg()->num_actions() + 2)};
+                    4 * std::size_t(max_sort_keys + 1)};
         case CELER_FIELD_NEXT_STEP: return {s.geo.next_step, 8 * n};
         case CELER_FIELD_GEO_FAILED: return {s.geo.failed, n};
         default: break;

@@ -181,6 +181,13 @@ solve_quadratic(double a, double half_b, double c, bool on, double* out)
     }
 }
 
+//! Component i of a 3-vector by selects (a dynamic index would force the
+//! vector out of registers into per-lane scratch memory)
+CELER_FORCEINLINE_FUNCTION double component(double const* v, int i)
+{
+    return i == 0 ? v[0] : (i == 1 ? v[1] : v[2]);
+}
+
 //! Distances to intersection (up to two; no_intersection when absent)
 CELER_FUNCTION void calc_intersections(GeoParamsRef const& p,
                                        std::uint32_t s,
@@ -195,10 +202,10 @@ CELER_FUNCTION void calc_intersections(GeoParamsRef const& p,
     if (t <= SurfaceType::pz)
     {
         int const ax = static_cast<int>(t);
-        double const n_dir = u[ax];
+        double const n_dir = component(u, ax);
         if (!on && n_dir != 0)
         {
-            double const dist = (d[0] - x[ax]) / n_dir;
+            double const dist = (d[0] - component(x, ax)) / n_dir;
             if (dist > 0)
                 out[0] = dist;
         }
@@ -227,11 +234,16 @@ CELER_FUNCTION void calc_intersections(GeoParamsRef const& p,
     int const ax = static_cast<int>(t) - static_cast<int>(SurfaceType::cxc);
     int const iu = (ax + 1) % 3;
     int const iv = (ax + 2) % 3;
-    double const a = 1 - u[ax] * u[ax];
+    double const ua = component(u, ax);
+    double const a = 1 - ua * ua;
     if (a < sqrt_quadratic * sqrt_quadratic)
         return;
-    double const half_b = u[iu] * x[iu] + u[iv] * x[iv];
-    double const c = x[iu] * x[iu] + x[iv] * x[iv] - d[0];
+    double const uu = component(u, iu);
+    double const uv = component(u, iv);
+    double const xu = component(x, iu);
+    double const xv = component(x, iv);
+    double const half_b = uu * xu + uv * xv;
+    double const c = xu * xu + xv * xv - d[0];
     solve_quadratic(a, half_b, c, on, out);
 }
 
@@ -246,7 +258,10 @@ CELER_FUNCTION void calc_normal(GeoParamsRef const& p,
     n[0] = n[1] = n[2] = 0;
     if (t <= SurfaceType::pz)
     {
-        n[static_cast<int>(t)] = 1;
+        int const ax = static_cast<int>(t);
+        n[0] = ax == 0 ? 1 : 0;
+        n[1] = ax == 1 ? 1 : 0;
+        n[2] = ax == 2 ? 1 : 0;
         return;
     }
     if (t == SurfaceType::p)
@@ -265,8 +280,9 @@ CELER_FUNCTION void calc_normal(GeoParamsRef const& p,
     else
     {
         int const ax = static_cast<int>(t) - static_cast<int>(SurfaceType::cxc);
-        n[(ax + 1) % 3] = x[(ax + 1) % 3];
-        n[(ax + 2) % 3] = x[(ax + 2) % 3];
+        n[0] = ax == 0 ? 0 : x[0];
+        n[1] = ax == 1 ? 0 : x[1];
+        n[2] = ax == 2 ? 0 : x[2];
     }
     double const norm
         = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
@@ -490,48 +506,42 @@ CELER_FUNCTION Propagation find_next_step(GeoParamsRef const& p,
     int const on_face = g.is_on_boundary() ? find_face(p, vol, g.surface) : -1;
     bool const simple = !(p.vol_flags[vol] & (internal_surfaces | implicit_vol));
 
-    // Candidate intersections: for simple volumes only the nearest is kept
-    double best = no_intersection;
-    int best_face = -1;
-    double dist[max_intersections];
-    std::uint8_t face_of[max_intersections];
-    int num_isect = 0;
-    for (int f = 0; f < nf; ++f)
-    {
-        std::uint32_t const s = p.faces[begin + f];
-        bool const on = (f == on_face);
-        auto const t = static_cast<SurfaceType>(p.surf_type[s]);
-        if (on && (t <= SurfaceType::pz || t == SurfaceType::p))
-            continue;  // single-intersection surface: cannot reintersect
-        double d2[2];
-        calc_intersections(p, s, g.pos, g.dir, on, d2);
-        for (int k = 0; k < 2; ++k)
+    // Valid candidate intersections of face f, root k (in [0, max_step])
+    auto for_each_isect = [&](auto&& visit) {
+        for (int f = 0; f < nf; ++f)
         {
-            double const d = d2[k];
-            if (!(d <= max_step) || d == no_intersection)
-                continue;
-            if (simple)
+            std::uint32_t const s = p.faces[begin + f];
+            bool const on = (f == on_face);
+            auto const t = static_cast<SurfaceType>(p.surf_type[s]);
+            if (on && (t <= SurfaceType::pz || t == SurfaceType::p))
+                continue;  // single-intersection surface: cannot reintersect
+            double d2[2];
+            calc_intersections(p, s, g.pos, g.dir, on, d2);
+            for (int k = 0; k < 2; ++k)
             {
-                if (d < best)
-                {
-                    best = d;
-                    best_face = f;
-                }
-            }
-            else
-            {
-                dist[num_isect] = d;
-                face_of[num_isect] = static_cast<std::uint8_t>(f);
-                ++num_isect;
+                double const d = d2[k];
+                if (!(d <= max_step) || d == no_intersection)
+                    continue;
+                visit(d, f, k);
             }
         }
-    }
+    };
 
     std::uint32_t isect_surf = invalid_id;
     bool isect_sense = false;
     double isect_dist = no_intersection;
     if (simple)
     {
+        // Simple volume: the nearest intersection exits
+        double best = no_intersection;
+        int best_face = -1;
+        for_each_isect([&](double d, int f, int) {
+            if (d < best)
+            {
+                best = d;
+                best_face = f;
+            }
+        });
         if (best_face >= 0)
         {
             std::uint32_t const s = p.faces[begin + best_face];
@@ -545,38 +555,46 @@ CELER_FUNCTION Propagation find_next_step(GeoParamsRef const& p,
             isect_dist = best;
         }
     }
-    else if (num_isect > 0)
+    else
     {
-        // Insertion sort by distance (stable: ties keep face order)
-        for (int i = 1; i < num_isect; ++i)
-        {
-            double const dv = dist[i];
-            std::uint8_t const fv = face_of[i];
-            int j = i - 1;
-            while (j >= 0 && dist[j] > dv)
-            {
-                dist[j + 1] = dist[j];
-                face_of[j + 1] = face_of[j];
-                --j;
-            }
-            dist[j + 1] = dv;
-            face_of[j + 1] = fv;
-        }
+        // Complex volume: visit intersections in increasing distance (ties
+        // in face, then root, order), flipping the crossed face's sense,
+        // until the logic says the track has left. The next candidate is
+        // found by a fresh scan each time, so no per-lane intersection list
+        // is kept (the usual exit is the first or second candidate).
         int dummy;
         std::uint32_t senses = calc_sense_mask(
             p, vol, g.pos, on_face, static_cast<bool>(g.sense), &dummy);
-        for (int i = 0; i < num_isect; ++i)
+        double last_d = -1;
+        int last_key = -1;
+        for (int iter = 0; iter < max_intersections; ++iter)
         {
-            int const f = face_of[i];
+            double next_d = no_intersection;
+            int next_key = 0x7fffffff;
+            for_each_isect([&](double d, int f, int k) {
+                int const key = 2 * f + k;
+                bool const after = d > last_d || (d == last_d && key > last_key);
+                bool const before = d < next_d || (d == next_d && key < next_key);
+                if (after && before)
+                {
+                    next_d = d;
+                    next_key = key;
+                }
+            });
+            if (next_key == 0x7fffffff)
+                break;
+            int const f = next_key / 2;
             senses ^= (1u << f);
             if (!eval_logic(p, vol, senses))
             {
                 bool const new_sense = (senses >> f) & 1u;
                 isect_surf = p.faces[begin + f];
                 isect_sense = !new_sense;
-                isect_dist = dist[i];
+                isect_dist = next_d;
                 break;
             }
+            last_d = next_d;
+            last_key = next_key;
         }
     }
 

@@ -31,10 +31,11 @@ constexpr int block_size = 256;
 __global__ void __launch_bounds__(block_size)
     along_step_charged_kernel(CoreParamsRef const params,
                               CoreStateRef const state,
-                              std::uint8_t const action)
+                              std::uint8_t const action,
+                              ThreadSlots const ts)
 {
-    size_type const slot = blockIdx.x * block_size + threadIdx.x;
-    if (slot >= state.size)
+    size_type slot;
+    if (!thread_to_slot(ts, blockIdx.x * block_size + threadIdx.x, state.size, &slot))
         return;
     auto const status = static_cast<TrackStatus>(state.sim.status[slot]);
     if (status == TrackStatus::inactive || state.sim.along_step_action[slot] != action)
@@ -222,10 +223,11 @@ __global__ void __launch_bounds__(block_size)
 hipError_t launch_along_step_charged(CoreParamsRef const& p,
                                      CoreStateRef const& s,
                                      std::uint8_t action,
+                                     ThreadSlots const& ts,
                                      hipStream_t st)
 {
     unsigned int const nb = (s.size + block_size - 1) / block_size;
-    along_step_charged_kernel<<<nb, block_size, 0, st>>>(p, s, action);
+    along_step_charged_kernel<<<nb, block_size, 0, st>>>(p, s, action, ts);
     return hipGetLastError();
 }
 

@@ -15,31 +15,64 @@ void AlongStepNeutralAction::step(CoreParams const& params, CoreState& state) co
 {
     CELER_HIP_CALL(launch_along_step_neutral(
         params.device_ref(), state.ref(),
-        static_cast<std::uint8_t>(this->action_id()), state.stream()));
+        static_cast<std::uint8_t>(this->action_id()),
+        make_thread_slots(params, state, *this), state.stream()));
 }
 
 void AlongStepChargedAction::step(CoreParams const& params, CoreState& state) const
 {
     CELER_HIP_CALL(launch_along_step_charged(
         params.device_ref(), state.ref(),
-        static_cast<std::uint8_t>(this->action_id()), state.stream()));
+        static_cast<std::uint8_t>(this->action_id()),
+        make_thread_slots(params, state, *this), state.stream()));
 }
 
 void BoundaryAction::step(CoreParams const& params, CoreState& state) const
 {
-    CELER_HIP_CALL(launch_boundary(params.device_ref(), state.ref(), state.stream()));
+    CELER_HIP_CALL(launch_boundary(params.device_ref(), state.ref(),
+                                   make_thread_slots(params, state, *this),
+                                   state.stream()));
 }
 
 void TrackingCutAction::step(CoreParams const& params, CoreState& state) const
 {
-    CELER_HIP_CALL(
-        launch_tracking_cut(params.device_ref(), state.ref(), state.stream()));
+    CELER_HIP_CALL(launch_tracking_cut(params.device_ref(), state.ref(),
+                                       make_thread_slots(params, state, *this),
+                                       state.stream()));
 }
 
 void CaloTallyAction::step(CoreParams const& params, CoreState& state) const
 {
     CELER_HIP_CALL(launch_calo_tally(params.device_ref(), state.ref(),
-                                     params.num_detectors(), state.stream()));
+                                     params.num_detectors(),
+                                     make_thread_slots(params, state, *this),
+                                     state.stream()));
+}
+
+//---------------------------------------------------------------------------//
+bool is_action_sorted(StepActionOrder aorder, TrackOrder torder)
+{
+    return (aorder == StepActionOrder::post
+            && torder == TrackOrder::reindex_step_limit_action)
+           || (aorder == StepActionOrder::along
+               && torder == TrackOrder::reindex_along_step_action)
+           || (torder == TrackOrder::reindex_both_action
+               && (aorder == StepActionOrder::post
+                   || aorder == StepActionOrder::along));
+}
+
+ThreadSlots make_thread_slots(CoreParams const& params,
+                              CoreState const& state,
+                              StepActionInterface const& action)
+{
+    ThreadSlots ts;
+    TrackOrder const torder = params.track_order();
+    if (!is_reindexed(torder))
+        return ts;
+    ts.slots = state.ref().track_slots;
+    if (is_action_sorted(action.order(), torder))
+        ts.range = state.ref().action_offsets + action.action_id();
+    return ts;
 }
 
 //---------------------------------------------------------------------------//

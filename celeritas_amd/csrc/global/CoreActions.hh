@@ -21,29 +21,60 @@ class PhysicsParams;
 //---------------------------------------------------------------------------//
 // KERNEL LAUNCHERS (implemented in the .hip translation units)
 //---------------------------------------------------------------------------//
-hipError_t launch_pre_step(CoreParamsRef const&, CoreStateRef const&, hipStream_t);
+hipError_t launch_pre_step(CoreParamsRef const&,
+                           CoreStateRef const&,
+                           ThreadSlots const&,
+                           hipStream_t);
 hipError_t launch_along_step_neutral(CoreParamsRef const&,
                                      CoreStateRef const&,
                                      std::uint8_t action,
+                                     ThreadSlots const&,
                                      hipStream_t);
 hipError_t launch_along_step_charged(CoreParamsRef const&,
                                      CoreStateRef const&,
                                      std::uint8_t action,
+                                     ThreadSlots const&,
                                      hipStream_t);
-hipError_t launch_discrete_select(CoreParamsRef const&, CoreStateRef const&, hipStream_t);
+hipError_t launch_discrete_select(CoreParamsRef const&,
+                                  CoreStateRef const&,
+                                  ThreadSlots const&,
+                                  hipStream_t);
 hipError_t launch_interaction(CoreParamsRef const&,
                               CoreStateRef const&,
                               std::uint8_t action,
                               std::uint32_t model,
+                              ThreadSlots const&,
                               hipStream_t);
-hipError_t launch_boundary(CoreParamsRef const&, CoreStateRef const&, hipStream_t);
-hipError_t launch_tracking_cut(CoreParamsRef const&, CoreStateRef const&, hipStream_t);
+hipError_t launch_boundary(CoreParamsRef const&,
+                           CoreStateRef const&,
+                           ThreadSlots const&,
+                           hipStream_t);
+hipError_t launch_tracking_cut(CoreParamsRef const&,
+                               CoreStateRef const&,
+                               ThreadSlots const&,
+                               hipStream_t);
 hipError_t launch_calo_tally(CoreParamsRef const&,
                              CoreStateRef const&,
                              size_type num_detectors,
+                             ThreadSlots const&,
                              hipStream_t);
 hipError_t launch_kill_active(CoreParamsRef const&, CoreStateRef const&, hipStream_t);
-hipError_t launch_fused_step(CoreParamsRef const&, CoreStateRef const&, hipStream_t);
+
+//---------------------------------------------------------------------------//
+//! Whether tracks are reindexed through track_slots (TrackOrder >= shuffle)
+inline bool is_reindexed(TrackOrder t)
+{
+    return static_cast<int>(t) >= static_cast<int>(TrackOrder::reindex_shuffle);
+}
+
+//! Whether an action of this order launches over its sorted thread range
+//! (ActionInterface.hh:63-73)
+bool is_action_sorted(StepActionOrder aorder, TrackOrder torder);
+
+//! Thread-to-slot map for launching an action on a state
+ThreadSlots make_thread_slots(CoreParams const& params,
+                              CoreState const& state,
+                              StepActionInterface const& action);
 
 //! Per-volume energy deposition tally (user/SimpleCalo, StepGather pre/post)
 class CaloTallyAction final : public StepActionInterface, public ConcreteAction

@@ -76,7 +76,8 @@ struct CoreCounters
     std::uint32_t secondary_stack_size;  //!< allocated secondaries
     std::uint32_t error_flags;  //!< bit 0: initializer overflow, bit 1: stack
     std::uint32_t steps_taken;  //!< number of step iterations on device
-    std::uint32_t pad_[3];
+    std::uint32_t num_new_neutral;  //!< neutral initializers this launch
+    std::uint32_t pad_[2];
 };
 
 enum CoreErrorFlags : std::uint32_t
@@ -147,12 +148,56 @@ struct CoreStateRef
     CoreCounters* counters{nullptr};
     std::uint32_t* pre_volume{nullptr};  //!< volume at the start of the step
     std::uint32_t* track_slots{nullptr};  //!< thread -> slot indirection
-    std::uint32_t* action_offsets{nullptr};  //!< sorted ranges per action
+    std::uint32_t* action_offsets{nullptr};  //!< sorted ranges per key
+    std::uint32_t* sort_counts{nullptr};  //!< key histogram per sort tile
     double* calo_edep{nullptr};  //!< per-volume energy deposition tally
     //! Optional work counters (null unless diagnostics are enabled), see
     //! DiagCounter
     unsigned long long* diag{nullptr};
 };
+
+//! Largest number of sort keys (actions + 1 for inactive slots)
+inline constexpr unsigned int max_sort_keys = 64;
+
+//! Per-slot key of a track reindexing
+enum class SortKey : std::uint8_t
+{
+    status,  //!< active slots first (TrackOrder::reindex_status)
+    particle,  //!< by particle type (reindex_particle_type)
+    along_action,  //!< by along-step action (reindex_along_step_action)
+    post_action,  //!< by post-step action (reindex_step_limit_action)
+};
+
+/*!
+ * Thread-to-slot map of a launch (ActionLauncher.device.hh:120-135 with the
+ * track_slots indirection, ThreadId -> TrackSlotId).
+ *
+ * With a range (an action-sorted launch) the thread count is read from the
+ * device-resident offsets, so launches never wait for the host; threads
+ * beyond the range exit at once.
+ */
+struct ThreadSlots
+{
+    std::uint32_t const* range{nullptr};  //!< &action_offsets[action], or null
+    std::uint32_t const* slots{nullptr};  //!< thread -> slot, or null (identity)
+};
+
+CELER_FORCEINLINE_FUNCTION bool
+thread_to_slot(ThreadSlots const& ts, size_type tid, size_type size, size_type* slot)
+{
+    if (ts.range)
+    {
+        size_type const begin = ts.range[0];
+        if (tid >= ts.range[1] - begin)
+            return false;
+        *slot = ts.slots[begin + tid];
+        return true;
+    }
+    if (tid >= size)
+        return false;
+    *slot = ts.slots ? ts.slots[tid] : tid;
+    return true;
+}
 
 //! Indices of the optional per-kernel work counters
 enum DiagCounter : int

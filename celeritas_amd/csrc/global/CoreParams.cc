@@ -73,6 +73,36 @@ CoreParams::CoreParams(Input inp)
     {
         reg_->insert(std::make_shared<CaloTallyAction>(reg_->next_id()));
     }
+    // Optional track-sorting actions (CoreParams.cc:259-283): keys are
+    // action IDs (plus one for inactive slots), particle IDs or status
+    {
+        std::vector<TrackOrder> sorts;
+        switch (inp.track_order)
+        {
+            case TrackOrder::reindex_status:
+            case TrackOrder::reindex_particle_type:
+            case TrackOrder::reindex_along_step_action:
+            case TrackOrder::reindex_step_limit_action:
+                sorts.push_back(inp.track_order);
+                break;
+            case TrackOrder::reindex_both_action:
+                sorts.push_back(TrackOrder::reindex_step_limit_action);
+                sorts.push_back(TrackOrder::reindex_along_step_action);
+                break;
+            default:
+                break;
+        }
+        size_type const total_actions = reg_->num_actions() + sorts.size();
+        for (TrackOrder t : sorts)
+        {
+            size_type keys = total_actions + 1;
+            if (t == TrackOrder::reindex_status)
+                keys = 2;
+            else if (t == TrackOrder::reindex_particle_type)
+                keys = phys_->num_particles() + 1;
+            reg_->insert(std::make_shared<SortTracksAction>(reg_->next_id(), t, keys));
+        }
+    }
     sc.num_actions = reg_->num_actions();
     sc.track_order = inp.track_order;
     sc.precision = inp.precision;
@@ -152,6 +182,12 @@ std::shared_ptr<CoreParams> CoreParams::from_json(std::string const& text)
             inp.track_order = TrackOrder::reindex_particle_type;
         else if (order == "reindex_step_limit_action")
             inp.track_order = TrackOrder::reindex_step_limit_action;
+        else if (order == "reindex_along_step_action")
+            inp.track_order = TrackOrder::reindex_along_step_action;
+        else if (order == "reindex_both_action")
+            inp.track_order = TrackOrder::reindex_both_action;
+        else if (order == "reindex_shuffle")
+            inp.track_order = TrackOrder::reindex_shuffle;
         else
             CELER_VALIDATE(false, << "unsupported track order '" << order << "'");
     }

@@ -7,8 +7,11 @@
 
 #include <algorithm>
 #include <functional>
+#include <numeric>
+#include <random>
 
 #include "random/XorwowParams.hh"
+#include "track/SortKernels.hh"
 #include "track/TrackKernels.hh"
 #include "CoreParams.hh"
 
@@ -98,7 +101,8 @@ CoreState::CoreState(CoreParams const& params,
     add(&s.counters, 1);
     add(&s.pre_volume, n);
     add(&s.track_slots, n);
-    add(&s.action_offsets, params.action_reg()->num_actions() + 2);
+    add(&s.action_offsets, max_sort_keys + 1);
+    add(&s.sort_counts, sort_counts_size(num_slots));
     add(&s.calo_edep, std::max<size_type>(num_detectors_, 1));
     a.allocate();
     for (auto& p : plan)
@@ -109,6 +113,16 @@ CoreState::CoreState(CoreParams const& params,
                                  sizeof(CoreCounters),
                                  hipHostMallocDefault));
     *host_counters_ = CoreCounters{};
+
+    if (params.track_order() == TrackOrder::reindex_shuffle)
+    {
+        // Same permutation as the reference's host shuffle
+        // (TrackSlotUtils.cc:21-31): mt19937 seeded with the slot count
+        shuffled_slots_.resize(n);
+        std::iota(shuffled_slots_.begin(), shuffled_slots_.end(), 0u);
+        std::mt19937 g{static_cast<unsigned int>(n)};
+        std::shuffle(shuffled_slots_.begin(), shuffled_slots_.end(), g);
+    }
 
     // RNG: Mersenne-twister-seeded states, transposed to SoA
     initial_rng_.resize(6 * n);
@@ -146,7 +160,15 @@ void CoreState::reset()
     CELER_HIP_CALL(hipMemcpyAsync(s.rng.weyl, initial_rng_.data() + 5 * n,
                                   n * 4, hipMemcpyHostToDevice, stream_));
     CELER_HIP_CALL(launch_iota_u32(s.init.vacancies, s.size, stream_));
-    CELER_HIP_CALL(launch_iota_u32(s.track_slots, s.size, stream_));
+    if (shuffled_slots_.empty())
+    {
+        CELER_HIP_CALL(launch_iota_u32(s.track_slots, s.size, stream_));
+    }
+    else
+    {
+        CELER_HIP_CALL(hipMemcpyAsync(s.track_slots, shuffled_slots_.data(), n * 4,
+                                      hipMemcpyHostToDevice, stream_));
+    }
     CELER_HIP_CALL(launch_fill_u32(s.init.parents, s.size, invalid_id, stream_));
     CELER_HIP_CALL(launch_fill_u32(s.geo.surface, s.size, invalid_id, stream_));
     CELER_HIP_CALL(launch_fill_u32(s.geo.next_surface, s.size, invalid_id, stream_));

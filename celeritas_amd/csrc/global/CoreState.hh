@@ -77,6 +77,7 @@ class CoreState
     size_type num_detectors_{0};
     bool warming_up_{false};
     std::vector<std::uint32_t> initial_rng_;
+    std::vector<std::uint32_t> shuffled_slots_;  //!< reindex_shuffle order
 };
 
 //---------------------------------------------------------------------------//

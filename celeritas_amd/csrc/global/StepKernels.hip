@@ -49,12 +49,15 @@ __device__ __forceinline__ std::uint32_t material_of(CoreParamsRef const& p,
 // PRE-STEP
 //---------------------------------------------------------------------------//
 __global__ void __launch_bounds__(block_size)
-    pre_step_kernel(CoreParamsRef const params, CoreStateRef const state)
+    pre_step_kernel(CoreParamsRef const params,
+                    CoreStateRef const state,
+                    ThreadSlots const ts)
 {
-    size_type const slot = blockIdx.x * block_size + threadIdx.x;
-    if (slot == 0)
+    size_type const tid = blockIdx.x * block_size + threadIdx.x;
+    if (tid == 0)
         state.counters->secondary_stack_size = 0;
-    if (slot >= state.size)
+    size_type slot;
+    if (!thread_to_slot(ts, tid, state.size, &slot))
         return;
     auto const status = status_of(state, slot);
     if (status == TrackStatus::inactive)
@@ -146,10 +149,11 @@ __global__ void __launch_bounds__(block_size)
 __global__ void __launch_bounds__(block_size)
     along_step_neutral_kernel(CoreParamsRef const params,
                               CoreStateRef const state,
-                              std::uint8_t const action)
+                              std::uint8_t const action,
+                              ThreadSlots const ts)
 {
-    size_type const slot = blockIdx.x * block_size + threadIdx.x;
-    if (slot >= state.size)
+    size_type slot;
+    if (!thread_to_slot(ts, blockIdx.x * block_size + threadIdx.x, state.size, &slot))
         return;
     auto const status = status_of(state, slot);
     if (status == TrackStatus::inactive || state.sim.along_step_action[slot] != action)
@@ -198,10 +202,12 @@ __global__ void __launch_bounds__(block_size)
 // DISCRETE SELECT
 //---------------------------------------------------------------------------//
 __global__ void __launch_bounds__(block_size)
-    discrete_select_kernel(CoreParamsRef const params, CoreStateRef const state)
+    discrete_select_kernel(CoreParamsRef const params,
+                           CoreStateRef const state,
+                           ThreadSlots const ts)
 {
-    size_type const slot = blockIdx.x * block_size + threadIdx.x;
-    if (slot >= state.size)
+    size_type slot;
+    if (!thread_to_slot(ts, blockIdx.x * block_size + threadIdx.x, state.size, &slot))
         return;
     PhysicsParamsRef const& pp = params.phys;
     if (state.sim.post_step_action[slot] != pp.scalars.discrete_action
@@ -260,14 +266,17 @@ __global__ void __launch_bounds__(block_size)
 //---------------------------------------------------------------------------//
 // INTERACTIONS
 //---------------------------------------------------------------------------//
+//! One instantiation per model class: each launch carries only the
+//! registers of its own sampler
+template<ModelClass M>
 __global__ void __launch_bounds__(block_size)
     interaction_kernel(CoreParamsRef const params,
                        CoreStateRef const state,
                        std::uint8_t const action,
-                       std::uint32_t const model)
+                       ThreadSlots const ts)
 {
-    size_type const slot = blockIdx.x * block_size + threadIdx.x;
-    if (slot >= state.size)
+    size_type slot;
+    if (!thread_to_slot(ts, blockIdx.x * block_size + threadIdx.x, state.size, &slot))
         return;
     if (state.sim.post_step_action[slot] != action
         || status_of(state, slot) == TrackStatus::inactive)
@@ -281,7 +290,7 @@ __global__ void __launch_bounds__(block_size)
     XorwowEngine rng = XorwowEngine::load(state.rng, slot);
     InteractionContext ctx{pp, state, slot, mat, state.phys.element[slot],
                            state.particle.particle_id[slot]};
-    Interaction r = sample_interaction(static_cast<ModelClass>(model), ctx, energy, dir, rng);
+    Interaction r = sample_interaction(M, ctx, energy, dir, rng);
     rng.store(state.rng, slot);
     apply_interaction(params, state, slot, mat, r);
 }
@@ -290,10 +299,12 @@ __global__ void __launch_bounds__(block_size)
 // BOUNDARY AND TRACKING CUT
 //---------------------------------------------------------------------------//
 __global__ void __launch_bounds__(block_size)
-    boundary_kernel(CoreParamsRef const params, CoreStateRef const state)
+    boundary_kernel(CoreParamsRef const params,
+                    CoreStateRef const state,
+                    ThreadSlots const ts)
 {
-    size_type const slot = blockIdx.x * block_size + threadIdx.x;
-    if (slot >= state.size)
+    size_type slot;
+    if (!thread_to_slot(ts, blockIdx.x * block_size + threadIdx.x, state.size, &slot))
         return;
     if (state.sim.post_step_action[slot] != params.scalars.boundary_action
         || status_of(state, slot) != TrackStatus::alive)
@@ -319,10 +330,12 @@ __global__ void __launch_bounds__(block_size)
 }
 
 __global__ void __launch_bounds__(block_size)
-    tracking_cut_kernel(CoreParamsRef const params, CoreStateRef const state)
+    tracking_cut_kernel(CoreParamsRef const params,
+                        CoreStateRef const state,
+                        ThreadSlots const ts)
 {
-    size_type const slot = blockIdx.x * block_size + threadIdx.x;
-    if (slot >= state.size)
+    size_type slot;
+    if (!thread_to_slot(ts, blockIdx.x * block_size + threadIdx.x, state.size, &slot))
         return;
     if (state.sim.post_step_action[slot] != params.scalars.tracking_cut_action
         || status_of(state, slot) == TrackStatus::inactive)
@@ -357,7 +370,8 @@ constexpr int max_lds_detectors = 512;
 __global__ void __launch_bounds__(block_size)
     calo_tally_kernel(CoreParamsRef const params,
                       CoreStateRef const state,
-                      size_type const num_det)
+                      size_type const num_det,
+                      ThreadSlots const ts)
 {
     __shared__ double bins[max_lds_detectors];
     bool const use_lds = num_det <= max_lds_detectors;
@@ -367,8 +381,9 @@ __global__ void __launch_bounds__(block_size)
             bins[i] = 0;
         __syncthreads();
     }
-    size_type const slot = blockIdx.x * block_size + threadIdx.x;
-    if (slot < state.size && status_of(state, slot) != TrackStatus::inactive)
+    size_type slot;
+    if (thread_to_slot(ts, blockIdx.x * block_size + threadIdx.x, state.size, &slot)
+        && status_of(state, slot) != TrackStatus::inactive)
     {
         double const edep = state.phys.energy_deposition[slot];
         if (edep != 0)
@@ -400,29 +415,31 @@ __global__ void __launch_bounds__(block_size)
 //---------------------------------------------------------------------------//
 // LAUNCHERS
 //---------------------------------------------------------------------------//
-hipError_t launch_pre_step(CoreParamsRef const& p, CoreStateRef const& s, hipStream_t st)
+hipError_t launch_pre_step(CoreParamsRef const& p,
+                           CoreStateRef const& s,
+                           ThreadSlots const& ts,
+                           hipStream_t st)
 {
-    pre_step_kernel<<<num_blocks(s.size), block_size, 0, st>>>(p, s);
+    pre_step_kernel<<<num_blocks(s.size), block_size, 0, st>>>(p, s, ts);
     return hipGetLastError();
 }
 
 hipError_t launch_along_step_neutral(CoreParamsRef const& p,
                                      CoreStateRef const& s,
                                      std::uint8_t action,
+                                     ThreadSlots const& ts,
                                      hipStream_t st)
 {
-    along_step_neutral_kernel<<<num_blocks(s.size), block_size, 0, st>>>(p, s, action);
+    along_step_neutral_kernel<<<num_blocks(s.size), block_size, 0, st>>>(p, s, action, ts);
     return hipGetLastError();
 }
 
-hipError_t launch_fused_step(CoreParamsRef const&, CoreStateRef const&, hipStream_t)
+hipError_t launch_discrete_select(CoreParamsRef const& p,
+                                  CoreStateRef const& s,
+                                  ThreadSlots const& ts,
+                                  hipStream_t st)
 {
-    return hipErrorNotSupported;
-}
-
-hipError_t launch_discrete_select(CoreParamsRef const& p, CoreStateRef const& s, hipStream_t st)
-{
-    discrete_select_kernel<<<num_blocks(s.size), block_size, 0, st>>>(p, s);
+    discrete_select_kernel<<<num_blocks(s.size), block_size, 0, st>>>(p, s, ts);
     return hipGetLastError();
 }
 
@@ -430,21 +447,46 @@ hipError_t launch_interaction(CoreParamsRef const& p,
                               CoreStateRef const& s,
                               std::uint8_t action,
                               std::uint32_t model,
+                              ThreadSlots const& ts,
                               hipStream_t st)
 {
-    interaction_kernel<<<num_blocks(s.size), block_size, 0, st>>>(p, s, action, model);
+    auto const nb = num_blocks(s.size);
+    switch (static_cast<ModelClass>(model))
+    {
+#define CELER_LAUNCH_MODEL(M)                                              \
+    case ModelClass::M:                                                    \
+        interaction_kernel<ModelClass::M><<<nb, block_size, 0, st>>>(p, s, action, ts); \
+        break;
+        CELER_LAUNCH_MODEL(klein_nishina)
+        CELER_LAUNCH_MODEL(livermore_pe)
+        CELER_LAUNCH_MODEL(bethe_heitler)
+        CELER_LAUNCH_MODEL(moller_bhabha)
+        CELER_LAUNCH_MODEL(seltzer_berger)
+        CELER_LAUNCH_MODEL(relativistic_brem)
+        CELER_LAUNCH_MODEL(eplusgg)
+        CELER_LAUNCH_MODEL(combined_brem)
+#undef CELER_LAUNCH_MODEL
+        default:
+            return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_boundary(CoreParamsRef const& p, CoreStateRef const& s, hipStream_t st)
+hipError_t launch_boundary(CoreParamsRef const& p,
+                           CoreStateRef const& s,
+                           ThreadSlots const& ts,
+                           hipStream_t st)
 {
-    boundary_kernel<<<num_blocks(s.size), block_size, 0, st>>>(p, s);
+    boundary_kernel<<<num_blocks(s.size), block_size, 0, st>>>(p, s, ts);
     return hipGetLastError();
 }
 
-hipError_t launch_tracking_cut(CoreParamsRef const& p, CoreStateRef const& s, hipStream_t st)
+hipError_t launch_tracking_cut(CoreParamsRef const& p,
+                               CoreStateRef const& s,
+                               ThreadSlots const& ts,
+                               hipStream_t st)
 {
-    tracking_cut_kernel<<<num_blocks(s.size), block_size, 0, st>>>(p, s);
+    tracking_cut_kernel<<<num_blocks(s.size), block_size, 0, st>>>(p, s, ts);
     return hipGetLastError();
 }
 
@@ -457,10 +499,11 @@ hipError_t launch_kill_active(CoreParamsRef const& p, CoreStateRef const& s, hip
 hipError_t launch_calo_tally(CoreParamsRef const& p,
                              CoreStateRef const& s,
                              size_type num_det,
+                             ThreadSlots const& ts,
                              hipStream_t st)
 {
     if (num_det > 0)
-        calo_tally_kernel<<<num_blocks(s.size), block_size, 0, st>>>(p, s, num_det);
+        calo_tally_kernel<<<num_blocks(s.size), block_size, 0, st>>>(p, s, num_det, ts);
     return hipGetLastError();
 }
 

@@ -14,13 +14,16 @@ namespace celeritas_amd
 //---------------------------------------------------------------------------//
 void PreStepAction::step(CoreParams const& params, CoreState& state) const
 {
-    CELER_HIP_CALL(launch_pre_step(params.device_ref(), state.ref(), state.stream()));
+    CELER_HIP_CALL(launch_pre_step(params.device_ref(), state.ref(),
+                                   make_thread_slots(params, state, *this),
+                                   state.stream()));
 }
 
 void DiscreteSelectAction::step(CoreParams const& params, CoreState& state) const
 {
-    CELER_HIP_CALL(
-        launch_discrete_select(params.device_ref(), state.ref(), state.stream()));
+    CELER_HIP_CALL(launch_discrete_select(params.device_ref(), state.ref(),
+                                          make_thread_slots(params, state, *this),
+                                          state.stream()));
 }
 
 void ModelAction::step(CoreParams const& params, CoreState& state) const
@@ -29,6 +32,7 @@ void ModelAction::step(CoreParams const& params, CoreState& state) const
                                       state.ref(),
                                       static_cast<std::uint8_t>(this->action_id()),
                                       model_class_,
+                                      make_thread_slots(params, state, *this),
                                       state.stream()));
 }
 

@@ -1,0 +1,217 @@
+//----------------------------------*-C++-*----------------------------------//
+// celeritas_amd: MI355X-native per-step transport loop
+//---------------------------------------------------------------------------//
+//! \file track/SortKernels.hip
+//! Reindexing of track slots by a small key (status, particle type, along-
+//! step or post-step action) and the per-key thread offsets.
+//!
+//! Counterpart of sort_tracks / count_tracks_per_action
+//! (celeritas/track/detail/TrackSortUtils.cu:145-206, host
+//! TrackSortUtils.cc:71-157). The reference sorts the track_slots array
+//! with thrust::sort_by_key / std::sort (unstable) and then copies the
+//! per-action offsets to the host. Here the keys are few (at most
+//! max_sort_keys) so the sort is a three-pass stable multi-split over the
+//! slot index: per-workgroup key histograms in LDS, one exclusive scan over
+//! the key-major histogram table (which also yields the offsets), and a
+//! scatter that ranks equal keys inside each wave with ballots. The result
+//! lists, for each key, its slots in increasing order, so the gathers of a
+//! sorted launch stay monotonic; the offsets stay on the device and sorted
+//! kernels read their range from there, so no host round trip is needed.
+//---------------------------------------------------------------------------//
+#include "SortKernels.hh"
+
+#include "corecel/WaveOps.hh"
+
+namespace celeritas_amd
+{
+namespace
+{
+//---------------------------------------------------------------------------//
+constexpr int block_size = 256;
+constexpr int num_waves = block_size / wave::size;
+constexpr int items_per_thread = 8;
+constexpr int tile_size = block_size * items_per_thread;
+
+inline unsigned int num_tiles(size_type n)
+{
+    return (n + tile_size - 1) / tile_size;
+}
+
+//! Sort key of a slot; inactive slots take the last key
+__device__ __forceinline__ unsigned int
+sort_key(CoreStateRef const& s, SortKey kind, size_type slot, unsigned int last)
+{
+    auto const status = static_cast<TrackStatus>(s.sim.status[slot]);
+    if (status == TrackStatus::inactive)
+        return last;
+    unsigned int k;
+    switch (kind)
+    {
+        case SortKey::status:
+            return 0;
+        case SortKey::particle:
+            k = s.particle.particle_id[slot];
+            break;
+        case SortKey::along_action:
+            k = s.sim.along_step_action[slot];
+            break;
+        default:
+            k = s.sim.post_step_action[slot];
+            break;
+    }
+    return k < last ? k : last;
+}
+
+//---------------------------------------------------------------------------//
+//! Per-tile key histogram, stored key-major: counts[key * ntiles + tile]
+__global__ void __launch_bounds__(block_size)
+    sort_count_kernel(CoreStateRef const state,
+                      SortKey const kind,
+                      unsigned int const num_keys,
+                      unsigned int const ntiles)
+{
+    __shared__ unsigned int hist[max_sort_keys];
+    for (unsigned int k = threadIdx.x; k < num_keys; k += block_size)
+        hist[k] = 0;
+    __syncthreads();
+    size_type const base = blockIdx.x * tile_size;
+    for (int i = 0; i < items_per_thread; ++i)
+    {
+        size_type const slot = base + i * block_size + threadIdx.x;
+        if (slot < state.size)
+            atomicAdd(&hist[sort_key(state, kind, slot, num_keys - 1)], 1u);
+    }
+    __syncthreads();
+    for (unsigned int k = threadIdx.x; k < num_keys; k += block_size)
+        state.sort_counts[k * ntiles + blockIdx.x] = hist[k];
+}
+
+//! Exclusive scan of the histogram table (one workgroup); the first tile
+//! of each key gives the key's thread offset, and offsets[num_keys] = size
+__global__ void __launch_bounds__(1024)
+    sort_scan_kernel(CoreStateRef const state,
+                     unsigned int const num_keys,
+                     unsigned int const ntiles)
+{
+    constexpr int nw = 1024 / wave::size;
+    __shared__ unsigned int lds[2 * nw];
+    __shared__ unsigned int carry;
+    if (threadIdx.x == 0)
+        carry = 0;
+    __syncthreads();
+    unsigned int* counts = state.sort_counts;
+    unsigned int const total = num_keys * ntiles;
+    for (unsigned int base = 0; base < total; base += 1024)
+    {
+        unsigned int const i = base + threadIdx.x;
+        unsigned int const a = i < total ? counts[i] : 0u;
+        wave::BlockScan2<nw> scan(a, 0u, lds);
+        unsigned int const c = carry;
+        if (i < total)
+        {
+            unsigned int const excl = c + scan.excl_a;
+            counts[i] = excl;
+            if (i % ntiles == 0)
+                state.action_offsets[i / ntiles] = excl;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+            carry = c + scan.total_a;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        state.action_offsets[num_keys] = state.size;
+}
+
+/*!
+ * Stable scatter of slot indices to their sorted thread positions.
+ *
+ * A tile is processed in rounds of one slot per thread, in slot order. In a
+ * round, each wave splits its lanes by key: the lowest remaining lane's key
+ * is broadcast, the lanes holding it are found with one ballot and ranked
+ * with mbcnt, and they leave the loop (one iteration per distinct key in
+ * the wave). Wave totals per key go through LDS to order the waves; a
+ * per-key running base carries the order across rounds and tiles.
+ */
+__global__ void __launch_bounds__(block_size)
+    sort_scatter_kernel(CoreStateRef const state,
+                        SortKey const kind,
+                        unsigned int const num_keys,
+                        unsigned int const ntiles)
+{
+    __shared__ unsigned int base[max_sort_keys];
+    __shared__ unsigned int wcount[num_waves][max_sort_keys];
+    int const lane = wave::lane_id();
+    int const w = threadIdx.x / wave::size;
+    for (unsigned int k = threadIdx.x; k < num_keys; k += block_size)
+        base[k] = state.sort_counts[k * ntiles + blockIdx.x];
+    size_type const tile_base = blockIdx.x * tile_size;
+    for (int r = 0; r < items_per_thread; ++r)
+    {
+        for (int k = threadIdx.x; k < num_waves * max_sort_keys; k += block_size)
+            (&wcount[0][0])[k] = 0;
+        __syncthreads();
+
+        size_type const slot = tile_base + r * block_size + threadIdx.x;
+        bool const valid = slot < state.size;
+        unsigned int const key = valid ? sort_key(state, kind, slot, num_keys - 1) : 0u;
+        unsigned int rank = 0;
+        std::uint64_t remaining = wave::ballot(valid);
+        while (remaining)
+        {
+            int const leader = __builtin_ctzll(remaining);
+            unsigned int const k = __shfl(key, leader, wave::size);
+            bool const mine = valid && key == k;
+            std::uint64_t const m = wave::ballot(mine);
+            if (mine)
+                rank = wave::prefix_count(m);
+            if (lane == leader)
+                wcount[w][k] = static_cast<unsigned int>(__popcll(m));
+            remaining &= ~m;
+        }
+        __syncthreads();
+
+        if (valid)
+        {
+            unsigned int pos = base[key] + rank;
+            for (int v = 0; v < w; ++v)
+                pos += wcount[v][key];
+            state.track_slots[pos] = slot;
+        }
+        __syncthreads();
+        for (unsigned int k = threadIdx.x; k < num_keys; k += block_size)
+        {
+            unsigned int t = 0;
+            for (int v = 0; v < num_waves; ++v)
+                t += wcount[v][k];
+            base[k] += t;
+        }
+        __syncthreads();
+    }
+}
+
+//---------------------------------------------------------------------------//
+}  // namespace
+
+//---------------------------------------------------------------------------//
+hipError_t launch_sort_tracks(CoreStateRef const& state,
+                              SortKey kind,
+                              unsigned int num_keys,
+                              hipStream_t stream)
+{
+    if (num_keys < 1 || num_keys > max_sort_keys)
+        return hipErrorInvalidValue;
+    unsigned int const nt = num_tiles(state.size);
+    sort_count_kernel<<<nt, block_size, 0, stream>>>(state, kind, num_keys, nt);
+    sort_scan_kernel<<<1, 1024, 0, stream>>>(state, num_keys, nt);
+    sort_scatter_kernel<<<nt, block_size, 0, stream>>>(state, kind, num_keys, nt);
+    return hipGetLastError();
+}
+
+size_type sort_counts_size(size_type num_slots)
+{
+    return max_sort_keys * num_tiles(num_slots);
+}
+
+//---------------------------------------------------------------------------//
+}  // namespace celeritas_amd

@@ -10,6 +10,7 @@
 
 #include "global/CoreParams.hh"
 #include "global/CoreState.hh"
+#include "SortKernels.hh"
 #include "TrackKernels.hh"
 
 namespace celeritas_amd
@@ -105,6 +106,65 @@ void ExtendFromSecondariesAction::step(CoreParams const& params,
 {
     CELER_HIP_CALL(launch_extend_from_secondaries(
         params.device_ref(), state.ref(), state.stream()));
+}
+
+//---------------------------------------------------------------------------//
+//---------------------------------------------------------------------------//
+namespace
+{
+char const* sort_label(TrackOrder order)
+{
+    switch (order)
+    {
+        case TrackOrder::reindex_status:
+            return "sort-tracks-status";
+        case TrackOrder::reindex_along_step_action:
+            return "sort-tracks-along-step";
+        case TrackOrder::reindex_step_limit_action:
+            return "sort-tracks-post-step";
+        case TrackOrder::reindex_particle_type:
+            return "sort-tracks-start";
+        default:
+            CELER_VALIDATE(false, << "track order does not sort tracks");
+    }
+    return nullptr;
+}
+}  // namespace
+
+SortTracksAction::SortTracksAction(ActionId id, TrackOrder order, size_type num_keys)
+    : ConcreteAction(id, sort_label(order), "sort tracks states")
+    , track_order_(order)
+    , num_keys_(num_keys)
+{
+    CELER_VALIDATE(num_keys_ >= 1 && num_keys_ <= max_sort_keys,
+                   << "cannot sort tracks by " << num_keys_ << " keys (at most "
+                   << max_sort_keys << ")");
+    switch (order)
+    {
+        case TrackOrder::reindex_status:
+            action_order_ = StepActionOrder::sort_start;
+            key_ = SortKey::status;
+            break;
+        case TrackOrder::reindex_particle_type:
+            action_order_ = StepActionOrder::sort_start;
+            key_ = SortKey::particle;
+            break;
+        case TrackOrder::reindex_along_step_action:
+            action_order_ = StepActionOrder::sort_pre;
+            key_ = SortKey::along_action;
+            break;
+        case TrackOrder::reindex_step_limit_action:
+            action_order_ = StepActionOrder::sort_pre_post;
+            key_ = SortKey::post_action;
+            break;
+        default:
+            CELER_VALIDATE(false, << "track order does not sort tracks");
+    }
+}
+
+void SortTracksAction::step(CoreParams const&, CoreState& state) const
+{
+    CELER_HIP_CALL(launch_sort_tracks(state.ref(), key_, num_keys_, state.stream()));
 }
 
 //---------------------------------------------------------------------------//

@@ -82,4 +82,29 @@ class ExtendFromSecondariesAction final : public StepActionInterface,
 };
 
 //---------------------------------------------------------------------------//
+/*!
+ * Reindex track slots (SortTracksAction.cc:44-135).
+ *
+ * Labels and step orders follow the reference: reindex_status and
+ * reindex_particle_type sort at sort_start, reindex_along_step_action at
+ * sort_pre (after pre-step), reindex_step_limit_action at sort_pre_post
+ * (after discrete selection). The per-key offsets stay in device memory
+ * for the sorted launches of the next actions.
+ */
+class SortTracksAction final : public StepActionInterface, public ConcreteAction
+{
+  public:
+    SortTracksAction(ActionId id, TrackOrder order, size_type num_keys);
+    void step(CoreParams const&, CoreState&) const final;
+    StepActionOrder order() const final { return action_order_; }
+    TrackOrder track_order() const { return track_order_; }
+
+  private:
+    TrackOrder track_order_;
+    StepActionOrder action_order_;
+    SortKey key_;
+    size_type num_keys_;
+};
+
+//---------------------------------------------------------------------------//
 }  // namespace celeritas_amd

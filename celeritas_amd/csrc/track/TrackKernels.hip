@@ -136,6 +136,101 @@ __global__ void __launch_bounds__(block_size)
 }
 
 //---------------------------------------------------------------------------//
+// CHARGE PARTITION (TrackOrder::init_charge)
+//---------------------------------------------------------------------------//
+__device__ __forceinline__ unsigned int
+is_neutral(CoreParamsRef const& params, TrackInitializer const& ti)
+{
+    return params.phys.charge[ti.particle_id] == 0 ? 1u : 0u;
+}
+
+//! Neutral flags of the new initializers, reduced per workgroup
+__global__ void __launch_bounds__(block_size)
+    partition_count_kernel(CoreParamsRef const params, CoreStateRef const state)
+{
+    __shared__ unsigned int lds[2 * num_waves];
+    size_type const tid = blockIdx.x * block_size + threadIdx.x;
+    CoreCounters const& c = *state.counters;
+    size_type const n = c.num_new_tracks;
+    unsigned int neutral = 0;
+    if (tid < n)
+        neutral = is_neutral(params, state.init.initializers[c.init_base - n + tid]);
+    wave::BlockScan2<num_waves> scan(neutral, 0u, lds);
+    if (threadIdx.x == 0)
+        state.init.block_scratch[2 * blockIdx.x] = scan.total_a;
+}
+
+//! Exclusive scan of the workgroup neutral counts (one workgroup)
+__global__ void __launch_bounds__(1024)
+    partition_scan_kernel(CoreStateRef const state, unsigned int nblocks)
+{
+    constexpr int nw = 1024 / wave::size;
+    __shared__ unsigned int lds[2 * nw];
+    __shared__ unsigned int carry;
+    if (threadIdx.x == 0)
+        carry = 0;
+    __syncthreads();
+    unsigned int* part = state.init.block_scratch;
+    for (unsigned int base = 0; base < nblocks; base += 1024)
+    {
+        unsigned int const i = base + threadIdx.x;
+        unsigned int const a = i < nblocks ? part[2 * i] : 0u;
+        wave::BlockScan2<nw> scan(a, 0u, lds);
+        unsigned int const ca = carry;
+        if (i < nblocks)
+            part[2 * i] = ca + scan.excl_a;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            carry = ca + scan.total_a;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        state.counters->num_new_neutral = carry;
+}
+
+/*!
+ * Stable partition of the indices 0..n-1 of the new initializers, neutral
+ * first (std::stable_partition in TrackInitAlgorithms.cc:80-94).
+ */
+__global__ void __launch_bounds__(block_size)
+    partition_scatter_kernel(CoreParamsRef const params, CoreStateRef const state)
+{
+    __shared__ unsigned int lds[2 * num_waves];
+    size_type const tid = blockIdx.x * block_size + threadIdx.x;
+    CoreCounters const& c = *state.counters;
+    size_type const n = c.num_new_tracks;
+    unsigned int neutral = 0;
+    if (tid < n)
+        neutral = is_neutral(params, state.init.initializers[c.init_base - n + tid]);
+    wave::BlockScan2<num_waves> scan(neutral, 0u, lds);
+    if (tid >= n)
+        return;
+    unsigned int const excl = state.init.block_scratch[2 * blockIdx.x] + scan.excl_a;
+    unsigned int const pos = neutral ? excl : c.num_new_neutral + (tid - excl);
+    state.init.indices[pos] = tid;
+}
+
+/*!
+ * Initialize with the charge partition (InitTracksExecutor.hh:78-104):
+ * thread tid takes initializer indices[n-1-tid]; neutral tracks fill the
+ * vacancies from the front, charged ones from the back.
+ */
+__global__ void __launch_bounds__(block_size)
+    init_tracks_charge_kernel(CoreParamsRef const params, CoreStateRef const state)
+{
+    size_type const tid = blockIdx.x * block_size + threadIdx.x;
+    CoreCounters const& c = *state.counters;
+    size_type const n = c.num_new_tracks;
+    if (tid >= n)
+        return;
+    size_type const idx = state.init.indices[n - 1 - tid] + c.init_base - n;
+    TrackInitializer const ti = state.init.initializers[idx];
+    size_type const vac = is_neutral(params, ti) ? n - 1 - tid
+                                                 : c.vacancy_base - 1 - tid;
+    init_track(params, state, state.init.vacancies[vac], ti);
+}
+
+//---------------------------------------------------------------------------//
 // EXTEND FROM SECONDARIES
 //---------------------------------------------------------------------------//
 //! Number of live secondaries of a slot, counting empty (cut) entries out
@@ -392,9 +487,19 @@ hipError_t launch_initialize_tracks(CoreParamsRef const& params,
                                     CoreStateRef const& state,
                                     hipStream_t stream)
 {
+    unsigned int const nb = num_blocks(state.size);
     init_prepare_kernel<<<1, 1, 0, stream>>>(state.counters, state.size);
-    init_tracks_kernel<<<num_blocks(state.size), block_size, 0, stream>>>(
-        params, state);
+    if (params.scalars.track_order == TrackOrder::init_charge)
+    {
+        partition_count_kernel<<<nb, block_size, 0, stream>>>(params, state);
+        partition_scan_kernel<<<1, 1024, 0, stream>>>(state, nb);
+        partition_scatter_kernel<<<nb, block_size, 0, stream>>>(params, state);
+        init_tracks_charge_kernel<<<nb, block_size, 0, stream>>>(params, state);
+    }
+    else
+    {
+        init_tracks_kernel<<<nb, block_size, 0, stream>>>(params, state);
+    }
     return hipGetLastError();
 }
 

@@ -65,7 +65,7 @@ def testem3_physics_spec() -> dict:
 
 def testem3_problem(physics: dict | None = None, seed: int = 20220511,
                     along_step: str = "general_linear", init_capacity: int = 1 << 20,
-                    max_events: int = 1 << 16) -> dict:
+                    max_events: int = 1 << 16, track_order: str = "none") -> dict:
     """TestEm3 problem; pass expanded ``physics`` to share tables."""
     geo = load_orange("testem3-flat.org.json")
     names = geo["universes"][0]["cell_names"]
@@ -90,7 +90,8 @@ def testem3_problem(physics: dict | None = None, seed: int = 20220511,
         "physics": physics if physics is not None else {"generate": testem3_physics_spec()},
         "along_step": along_step,
         "seed": seed,
-        "track_init": {"capacity": init_capacity, "max_events": max_events},
+        "track_init": {"capacity": init_capacity, "max_events": max_events,
+                       "track_order": track_order},
     }
 
 

@@ -66,7 +66,8 @@ typedef struct CelerCounters
     uint32_t secondary_stack_size;
     uint32_t error_flags;
     uint32_t steps_taken;
-    uint32_t pad_[3];
+    uint32_t num_new_neutral; /* neutral tracks in the last charge-partitioned init */
+    uint32_t pad_[2];
 } CelerCounters;
 
 /* Track initializer: celeritas/track/TrackInitData.hh:66-80 */
@@ -132,7 +133,7 @@ typedef enum CelerStateField
     CELER_FIELD_MACRO_XS,            /* f64 */
     CELER_FIELD_DEDX_RANGE,          /* f64 */
     CELER_FIELD_TRACK_SLOTS,         /* u32 */
-    CELER_FIELD_ACTION_OFFSETS,      /* u32 [num_actions + 2] */
+    CELER_FIELD_ACTION_OFFSETS,      /* u32 [65]: per-key sorted thread offsets */
     CELER_FIELD_NEXT_STEP,           /* f64 */
     CELER_FIELD_GEO_FAILED,          /* u8 */
     CELER_FIELD_SIZE_

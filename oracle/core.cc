@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <stdexcept>
 
+#include "physics.hh"
 #include "problem.hh"
 
 namespace orc
@@ -111,16 +112,46 @@ void extend_from_primaries(State& s)
     s.pending_primaries.clear();
 }
 
-//! InitializeTracksAction::step_impl + InitTracksExecutor (TrackOrder::none)
+/*!
+ * InitializeTracksAction::step_impl + InitTracksExecutor.
+ *
+ * TrackOrder::none pops initializers and vacancies from the back
+ * (InitTracksExecutor.hh:71-104). With init_charge (in_place false) the new
+ * initializers' indices are stably partitioned neutral-first
+ * (InitializeTracksAction.cc:66-78, TrackInitAlgorithms.cc:80-94); thread
+ * tid takes initializer indices[n-1-tid] and a neutral track fills vacancy
+ * n-1-tid from the front, a charged one vacancy V-1-tid from the back
+ * (Utils.hh:88-98).
+ */
 void initialize_tracks(State& s)
 {
     Counters& c = s.counters;
     std::uint32_t const n = std::min(c.num_vacancies, c.num_initializers);
     Problem const& prob = *s.problem;
+    bool const by_charge = !s.in_place;
+    std::vector<std::uint32_t> indices;
+    auto is_neutral = [&prob](Initializer const& ti) {
+        return prob.physics->particles[ti.particle].charge == 0;
+    };
+    if (by_charge)
+    {
+        indices.resize(n);
+        for (std::uint32_t i = 0; i < n; ++i)
+            indices[i] = i;
+        Initializer const* stencil = s.initializers.data() + c.num_initializers - n;
+        std::stable_partition(indices.begin(), indices.end(),
+                              [&](std::uint32_t i) { return is_neutral(stencil[i]); });
+    }
     for (std::uint32_t tid = 0; tid < n; ++tid)
     {
-        Initializer const& init = s.initializers[c.num_initializers - tid - 1];
-        Track& t = s.tracks[s.vacancies[c.num_vacancies - tid - 1]];
+        std::uint32_t const init_idx
+            = by_charge ? indices[n - 1 - tid] + c.num_initializers - n
+                        : c.num_initializers - tid - 1;
+        Initializer const& init = s.initializers[init_idx];
+        std::uint32_t vac_idx = c.num_vacancies - tid - 1;
+        if (by_charge && is_neutral(init))
+            vac_idx = n - 1 - tid;
+        Track& t = s.tracks[s.vacancies[vac_idx]];
         assign_sim(t, init);
         t.particle = init.particle;
         t.energy = init.energy;
