@@ -59,7 +59,7 @@ def parse_args():
     ap.add_argument("--primaries", type=int, default=10000)
     ap.add_argument("--slots", type=int, default=1 << 20)
     ap.add_argument("--seed", type=int, default=12345)
-    ap.add_argument("--track-order", default="none",
+    ap.add_argument("--track-order", default="reindex_both_action",
                     help="track slot ordering (none, init_charge, reindex_status, "
                          "reindex_particle_type, reindex_along_step_action, "
                          "reindex_step_limit_action, reindex_both_action)")

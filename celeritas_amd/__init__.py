@@ -510,6 +510,29 @@ def verify_interactor(params: CoreParams, state: CoreState, model: str, particle
     return out
 
 
+def verify_field_propagate(params: CoreParams, tracks, steps,
+                           max_substeps: int = 0) -> np.ndarray:
+    """Uniform-field propagation on the device: ``tracks`` rows are (pos[3],
+    dir[3], momentum, charge, field[3]); ``steps`` is (ntracks, nsteps) of
+    requests (0 crosses, -1 restarts the propagator). Returns
+    (ntracks, nsteps, 10): distance, boundary, looping, pos, dir, volume."""
+    lib_ = _load()
+    f = lib_.celer_verify_field_propagate
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint32,
+                  ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p]
+    f.restype = ctypes.c_int
+    lib_.celer_params_device_ref.restype = ctypes.c_void_p
+    lib_.celer_params_device_ref.argtypes = [ctypes.c_void_p]
+    t = np.ascontiguousarray(tracks, np.float64).reshape(-1, 11)
+    s = np.ascontiguousarray(steps, np.float64).reshape(len(t), -1)
+    out = np.zeros((len(t), s.shape[1], 10), np.float64)
+    rc = f(lib_.celer_params_device_ref(params.handle), max_substeps, t.ctypes.data, len(t),
+           s.ctypes.data, s.shape[1], out.ctypes.data, None)
+    if rc != 0:
+        raise CeleritasError(f"celer_verify_field_propagate failed with HIP error {rc}")
+    return out
+
+
 def generate_physics(spec: dict | str) -> dict:
     """Expand a compact physics description into full tables (host only).
 

@@ -17,6 +17,7 @@
 //---------------------------------------------------------------------------//
 #include "CoreActions.hh"
 #include "corecel/WaveOps.hh"
+#include "field/FieldPropagator.hh"
 #include "geo/OrangeLiteTracker.hh"
 #include "phys/AlongStepPhysics.hh"
 #include "phys/PhysicsImpl.hh"
@@ -28,6 +29,7 @@ namespace
 {
 constexpr int block_size = 256;
 
+template<bool UseField>
 __global__ void __launch_bounds__(block_size)
     along_step_charged_kernel(CoreParamsRef const params,
                               CoreStateRef const state,
@@ -109,15 +111,35 @@ __global__ void __launch_bounds__(block_size)
             post = pp.scalars.msc_action;
     }
 
-    // Propagation
+    // Propagation (PropagationApplier.hh:95-195): linear, or curved in the
+    // uniform field with looping-track handling
     if (step != 0)
     {
         double const orig[3] = {g.pos[0], g.pos[1], g.pos[2]};
-        geo::Propagation prop = geo::find_next_step(params.geo, g, step);
-        if (prop.boundary)
-            geo::move_to_boundary(g);
+        geo::Propagation prop;
+        bool looping = false;
+        if constexpr (UseField)
+        {
+            FieldParamsRef const& fp = params.field;
+            double const mass = pp.mass[pid];
+            double const momentum = std::sqrt(energy * energy + 2 * mass * energy);
+            field::UniformFieldStepper const stepper{
+                {fp.field[0], fp.field[1], fp.field[2]},
+                pp.charge[pid] * field::lorentz_per_cm_tesla};
+            field::FieldPropagation const fprop
+                = field::propagate(params.geo, g, stepper, fp.options, momentum, step);
+            prop.distance = fprop.distance;
+            prop.boundary = fprop.boundary;
+            looping = fprop.looping;
+        }
         else
-            geo::move_internal(g, step);
+        {
+            prop = geo::find_next_step(params.geo, g, step);
+            if (prop.boundary)
+                geo::move_to_boundary(g);
+            else
+                geo::move_internal(g, step);
+        }
         if (CELER_UNLIKELY(g.pos[0] == orig[0] && g.pos[1] == orig[1]
                            && g.pos[2] == orig[2]))
         {
@@ -126,15 +148,34 @@ __global__ void __launch_bounds__(block_size)
             apply_errored(params, state, slot);
             return;
         }
-        if (prop.boundary)
+        if constexpr (UseField)
         {
-            step = prop.distance;
-            post = params.scalars.boundary_action;
+            // SimTrackView::update_looping / is_looping (SimTrackView.hh:189-219)
+            std::uint32_t nloop = looping ? state.sim.num_looping_steps[slot] + 1 : 0;
+            state.sim.num_looping_steps[slot] = nloop;
+            if (looping)
+            {
+                step = prop.distance;
+                LoopingThreshold const& lt = params.field.looping;
+                std::uint32_t const limit = energy < lt.threshold_energy
+                                                ? lt.max_subthreshold_steps
+                                                : lt.max_steps;
+                post = nloop >= limit ? params.scalars.tracking_cut_action
+                                      : params.scalars.propagation_limit_action;
+            }
         }
-        else if (prop.distance < step)
+        if (!looping)
         {
-            step = prop.distance;
-            post = params.scalars.propagation_limit_action;
+            if (prop.boundary)
+            {
+                step = prop.distance;
+                post = params.scalars.boundary_action;
+            }
+            else if (prop.distance < step)
+            {
+                step = prop.distance;
+                post = params.scalars.propagation_limit_action;
+            }
         }
     }
 
@@ -227,7 +268,10 @@ hipError_t launch_along_step_charged(CoreParamsRef const& p,
                                      hipStream_t st)
 {
     unsigned int const nb = (s.size + block_size - 1) / block_size;
-    along_step_charged_kernel<<<nb, block_size, 0, st>>>(p, s, action, ts);
+    if (p.field.enabled)
+        along_step_charged_kernel<true><<<nb, block_size, 0, st>>>(p, s, action, ts);
+    else
+        along_step_charged_kernel<false><<<nb, block_size, 0, st>>>(p, s, action, ts);
     return hipGetLastError();
 }
 

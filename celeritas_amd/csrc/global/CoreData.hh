@@ -13,6 +13,7 @@
 #pragma once
 
 #include "corecel/Types.hh"
+#include "field/FieldData.hh"
 #include "geo/GeoData.hh"
 #include "phys/PhysicsData.hh"
 #include "random/XorwowEngine.hh"
@@ -100,6 +101,7 @@ struct SimStateRef
     double* step_length{nullptr};
     std::uint8_t* post_step_action{nullptr};
     std::uint8_t* along_step_action{nullptr};
+    std::uint32_t* num_looping_steps{nullptr};  //!< consecutive looping steps
 };
 
 //! Particle state (celeritas/phys/ParticleData.hh:119-135)
@@ -238,6 +240,7 @@ struct CoreParamsRef
     std::uint32_t const* volume_to_material{nullptr};
     std::uint32_t const* volume_to_detector{nullptr};  //!< calo tally map
     PhysicsParamsRef phys;
+    FieldParamsRef field;  //!< uniform magnetic field (disabled: none)
     XorwowJumpTables const* rng_jump{nullptr};
     std::uint32_t rng_seed{0};
 };

@@ -111,6 +111,7 @@ CoreParams::CoreParams(Input inp)
     ref_.volume_to_material = d_vol_to_mat_.data();
     ref_.volume_to_detector = d_vol_to_det_.data();
     ref_.phys = phys_->device_ref();
+    ref_.field = inp.field;
     ref_.rng_jump = d_jump_.data();
     ref_.rng_seed = rng_->seed();
 }
@@ -142,6 +143,43 @@ std::shared_ptr<CoreParams> CoreParams::from_json(std::string const& text)
     {
         inp.along_step_label = make_along_step_action(along, *inp.physics, doc,
                                                       inp.action_reg.get());
+    }
+    if (along == "uniform_field")
+    {
+        // Field [T], driver options (lengths in cm) and looping thresholds
+        CELER_VALIDATE(doc.has("field"), << "uniform_field along-step needs 'field'");
+        JsonValue const& f = doc.at("field");
+        auto b = f.at("value").as_doubles();
+        CELER_VALIDATE(b.size() == 3, << "field value must have 3 components");
+        FieldParamsRef& fp = inp.field;
+        for (int i = 0; i < 3; ++i)
+            fp.field[i] = b[i];
+        fp.enabled = true;
+        if (f.has("options"))
+        {
+            JsonValue const& o = f.at("options");
+            FieldDriverOptions& d = fp.options;
+            d.minimum_step = o.get("minimum_step", d.minimum_step);
+            d.delta_chord = o.get("delta_chord", d.delta_chord);
+            d.delta_intersection = o.get("delta_intersection", d.delta_intersection);
+            d.epsilon_step = o.get("epsilon_step", d.epsilon_step);
+            d.epsilon_rel_max = o.get("epsilon_rel_max", d.epsilon_rel_max);
+            d.max_nsteps = static_cast<int>(o.get_int("max_nsteps", d.max_nsteps));
+            d.max_substeps = static_cast<int>(o.get_int("max_substeps", d.max_substeps));
+            CELER_VALIDATE(d.minimum_step > 0 && d.delta_chord > 0
+                               && d.delta_intersection > 0 && d.max_substeps > 0
+                               && d.max_nsteps > 0,
+                           << "invalid field driver options");
+        }
+        if (f.has("looping"))
+        {
+            JsonValue const& l = f.at("looping");
+            LoopingThreshold& lt = fp.looping;
+            lt.max_subthreshold_steps = static_cast<std::uint32_t>(
+                l.get_int("max_subthreshold_steps", lt.max_subthreshold_steps));
+            lt.max_steps = static_cast<std::uint32_t>(l.get_int("max_steps", lt.max_steps));
+            lt.threshold_energy = l.get("threshold_energy", lt.threshold_energy);
+        }
     }
 
     // Volume -> material map: explicit, else from the ORANGE document

@@ -45,6 +45,8 @@ class CoreParams
         PrecisionMode precision{PrecisionMode::fp64};
         //! Along-step action for charged particles (label), created by caller
         std::string along_step_label;
+        //! Uniform magnetic field (enabled for the uniform-field along-step)
+        FieldParamsRef field;
     };
 
     explicit CoreParams(Input inp);

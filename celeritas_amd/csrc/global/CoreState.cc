@@ -49,6 +49,7 @@ CoreState::CoreState(CoreParams const& params,
     add(&s.sim.step_length, n);
     add(&s.sim.post_step_action, n);
     add(&s.sim.along_step_action, n);
+    add(&s.sim.num_looping_steps, n);
     add(&s.particle.particle_id, n);
     add(&s.particle.energy, n);
     for (int k = 0; k < 3; ++k)

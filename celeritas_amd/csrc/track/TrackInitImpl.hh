@@ -25,6 +25,7 @@ init_sim(SimStateRef const& sim, size_type slot, TrackInitializer const& ti)
     sim.step_length[slot] = 0;
     sim.post_step_action[slot] = invalid_id8;
     sim.along_step_action[slot] = invalid_id8;
+    sim.num_looping_steps[slot] = 0;
 }
 
 //! Reset persistent physics state (PhysicsTrackView::operator=, :207-212)

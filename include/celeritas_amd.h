@@ -358,6 +358,25 @@ int celer_verify_interactor(const void* params_ref,
                             double* out,
                             void* stream);
 
+/*---------------------------------------------------------------------------*
+ * VERIFICATION: uniform-field propagation (field/FieldPropagator.hh:150-331
+ * with FieldDriver and DormandPrinceStepper). Each of ntracks tracks (11
+ * doubles: pos[3] cm, dir[3], momentum MeV/c, charge e, field[3] T) runs
+ * nsteps requests through one propagator, as the reference's unit tests
+ * do (test/celeritas/field/FieldPropagator.test.cc); a request of 0 crosses
+ * the boundary and -1 starts a new propagator. Writes 10 doubles per
+ * request: distance, boundary, looping, pos[3], dir[3], volume. Driver
+ * options are the params' (max_substeps > 0 overrides that option).
+ *---------------------------------------------------------------------------*/
+int celer_verify_field_propagate(const void* params_ref,
+                                 int max_substeps,
+                                 const double* tracks,
+                                 uint32_t ntracks,
+                                 const double* steps,
+                                 uint32_t nsteps,
+                                 double* out,
+                                 void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -361,16 +361,36 @@ void along_step_charged_impl(State& s)
                 t.post_action = pd.msc_action;
         }
 
-        // --- PropagationApplier (linear)
+        // --- PropagationApplier (linear, or uniform field with looping
+        // tracks: PropagationApplier.hh:95-195, SimTrackView.hh:189-219)
         if (t.step_length != 0)
         {
             double const orig[3] = {t.geo.pos[0], t.geo.pos[1], t.geo.pos[2]};
             bool boundary;
-            double dist = geo_find_next_step(prob.geo, t.geo, t.step_length, &boundary);
-            if (boundary)
-                geo_move_to_boundary(t.geo);
+            bool looping = false;
+            double dist;
+            if (prob.field_enabled)
+            {
+                double const mass = pd.particles[t.particle].mass;
+                double const momentum = std::sqrt(t.energy * t.energy + 2 * mass * t.energy);
+                UniformFieldStepper stepper;
+                for (int i = 0; i < 3; ++i)
+                    stepper.field[i] = prob.field[i];
+                stepper.coeff = pd.particles[t.particle].charge * lorentz_per_cm_tesla;
+                FieldPropagation fp = field_propagate(prob.geo, t.geo, stepper, momentum,
+                                                      prob.field_options, t.step_length);
+                dist = fp.distance;
+                boundary = fp.boundary;
+                looping = fp.looping;
+            }
             else
-                geo_move_internal(t.geo, t.step_length);
+            {
+                dist = geo_find_next_step(prob.geo, t.geo, t.step_length, &boundary);
+                if (boundary)
+                    geo_move_to_boundary(t.geo);
+                else
+                    geo_move_internal(t.geo, t.step_length);
+            }
             if (t.geo.pos[0] == orig[0] && t.geo.pos[1] == orig[1] && t.geo.pos[2] == orig[2])
             {
                 t.status = errored;
@@ -378,7 +398,18 @@ void along_step_charged_impl(State& s)
                 t.post_action = ids.tracking_cut;
                 continue;
             }
-            if (boundary)
+            if (prob.field_enabled)
+                t.num_looping_steps = looping ? t.num_looping_steps + 1 : 0;
+            if (looping)
+            {
+                t.step_length = dist;
+                std::uint32_t const limit = t.energy < prob.looping_threshold_energy
+                                                ? prob.looping_max_subthreshold_steps
+                                                : prob.looping_max_steps;
+                t.post_action = t.num_looping_steps >= limit ? ids.tracking_cut
+                                                             : ids.propagation_limit;
+            }
+            else if (boundary)
             {
                 t.step_length = dist;
                 t.post_action = ids.boundary;

@@ -780,3 +780,64 @@ int orc_interactor_replay_batch(void* problem, const char* model, std::uint32_t 
     });
 }
 }
+
+//---------------------------------------------------------------------------//
+// FIELD PROPAGATION
+//---------------------------------------------------------------------------//
+#include "field.hh"
+
+extern "C" {
+/*!
+ * Propagate one charged track through the problem geometry in a uniform
+ * field with successive step requests through one propagator object, as
+ * the reference's unit tests do; a request of 0 crosses the boundary the
+ * track is on and -1 only starts a new propagator. Per
+ * request, out holds 11 values: distance, boundary, looping, position (3),
+ * direction (3), volume, and the cumulative number of Dormand-Prince steps.
+ */
+int orc_field_propagate(void* problem, double const* pos, double const* dir,
+                        double momentum, double charge, double const* field,
+                        int max_substeps, double const* steps, std::uint32_t nsteps,
+                        double* out)
+{
+    return guard([&] {
+        Geometry const& g = static_cast<OrcProblem*>(problem)->p->geo;
+        GeoState s;
+        geo_initialize(g, s, pos, dir);
+        UniformFieldStepper stepper;
+        for (int i = 0; i < 3; ++i)
+            stepper.field[i] = field[i];
+        stepper.coeff = charge * lorentz_per_cm_tesla;
+        FieldOptions opts;
+        if (max_substeps > 0)
+            opts.max_substeps = max_substeps;
+        FieldPropagatorState ps = make_field_state(s, momentum);
+        for (std::uint32_t k = 0; k < nsteps; ++k)
+        {
+            FieldPropagation r;
+            if (steps[k] > 0)
+            {
+                r = field_propagate(g, s, stepper, opts, ps, steps[k]);
+            }
+            else
+            {
+                // New propagator (after crossing the boundary for 0)
+                if (steps[k] == 0)
+                    geo_cross_boundary(g, s);
+                ps = make_field_state(s, momentum);
+            }
+            double* o = out + 11 * k;
+            o[0] = r.distance;
+            o[1] = r.boundary;
+            o[2] = r.looping;
+            for (int i = 0; i < 3; ++i)
+            {
+                o[3 + i] = s.pos[i];
+                o[6 + i] = s.dir[i];
+            }
+            o[9] = s.vol;
+            o[10] = static_cast<double>(stepper.count);
+        }
+    });
+}
+}

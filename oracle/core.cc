@@ -26,6 +26,7 @@ void assign_sim(Track& t, Initializer const& init)
     t.step_length = 0;
     t.post_action = none;
     t.along_action = none;
+    t.num_looping_steps = 0;
 }
 
 //! PhysicsTrackView::operator=(Initializer) (PhysicsTrackView.hh:207-212)

@@ -74,6 +74,7 @@ struct Track
     Status status = inactive;
     double step_length = 0;
     std::uint32_t post_action = none, along_action = none;
+    std::uint32_t num_looping_steps = 0;
     // particle
     std::uint32_t particle = none;
     double energy = 0;

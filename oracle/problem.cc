@@ -3,6 +3,8 @@
 //---------------------------------------------------------------------------//
 #include "problem.hh"
 
+#include <stdexcept>
+
 #include "physics.hh"
 
 namespace orc
@@ -77,7 +79,38 @@ std::shared_ptr<Problem> Problem::from_json(std::string const& text)
     if (along == "general_linear")
         a.along_user = add("along-step-general-linear");
     else if (along == "uniform_field")
+    {
         a.along_user = add("along-step-uniform-msc");
+        Json const* f = doc.find("field");
+        if (!f)
+            throw std::runtime_error("uniform_field along-step needs 'field'");
+        auto b = f->find("value")->nums();
+        if (b.size() != 3)
+            throw std::runtime_error("field value must have 3 components");
+        p->field_enabled = true;
+        for (int i = 0; i < 3; ++i)
+            p->field[i] = b[i];
+        if (Json const* o = f->find("options"))
+        {
+            FieldOptions& d = p->field_options;
+            d.minimum_step = o->num("minimum_step", d.minimum_step);
+            d.delta_chord = o->num("delta_chord", d.delta_chord);
+            d.delta_intersection = o->num("delta_intersection", d.delta_intersection);
+            d.epsilon_step = o->num("epsilon_step", d.epsilon_step);
+            d.epsilon_rel_max = o->num("epsilon_rel_max", d.epsilon_rel_max);
+            d.max_nsteps = static_cast<int>(o->num("max_nsteps", d.max_nsteps));
+            d.max_substeps = static_cast<int>(o->num("max_substeps", d.max_substeps));
+        }
+        if (Json const* l = f->find("looping"))
+        {
+            p->looping_max_subthreshold_steps = static_cast<std::uint32_t>(
+                l->num("max_subthreshold_steps", p->looping_max_subthreshold_steps));
+            p->looping_max_steps
+                = static_cast<std::uint32_t>(l->num("max_steps", p->looping_max_steps));
+            p->looping_threshold_energy
+                = l->num("threshold_energy", p->looping_threshold_energy);
+        }
+    }
     else if (along == "neutral")
         a.along_user = a.along_neutral = add("along-step-neutral");
     a.initialize_tracks = add("initialize-tracks");

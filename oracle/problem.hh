@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "field.hh"
 #include "geo.hh"
 #include "json.hh"
 
@@ -47,6 +48,14 @@ struct Problem
     double secondary_stack_factor = 3;
     ActionIds actions;
     std::shared_ptr<PhysicsData const> physics;
+    // Uniform field for the uniform-field along-step (SimData.hh:43-57
+    // looping thresholds)
+    bool field_enabled = false;
+    double field[3] = {0, 0, 0};  //!< [T]
+    FieldOptions field_options;
+    std::uint32_t looping_max_subthreshold_steps = 10;
+    std::uint32_t looping_max_steps = 100;
+    double looping_threshold_energy = 250;  //!< [MeV]
 
     //! Build from the same JSON problem description as the product
     static std::shared_ptr<Problem> from_json(std::string const& text);

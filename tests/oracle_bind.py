@@ -342,3 +342,27 @@ def interactor_replay_batch(problem, model, particle, energies, dirs, uniforms, 
     _check(f(problem.handle, model.encode(), particle, e.ctypes.data, d.ctypes.data,
              material, element, n, u.ctypes.data, stride, out.ctypes.data))
     return out
+
+
+def field_propagate(problem: "Problem", pos, direction, momentum: float, charge: float,
+                    field, steps, max_substeps: int = 0) -> np.ndarray:
+    """Successive uniform-field propagations of one track through one
+    propagator; a step of 0 crosses the boundary the track is on and starts
+    a new propagator, -1 only starts a new propagator.
+
+    Returns rows of (distance, boundary, looping, x, y, z, u, v, w, volume,
+    cumulative Dormand-Prince steps)."""
+    lib = load()
+    f = lib.orc_field_propagate
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                  ctypes.c_double, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                  ctypes.c_uint32, ctypes.c_void_p]
+    p = np.ascontiguousarray(pos, dtype=np.float64)
+    d = np.ascontiguousarray(direction, dtype=np.float64)
+    b = np.ascontiguousarray(field, dtype=np.float64)
+    s = np.ascontiguousarray(steps, dtype=np.float64)
+    out = np.zeros((len(s), 11))
+    _check(f(problem.handle, p.ctypes.data, d.ctypes.data, momentum, charge, b.ctypes.data,
+             max_substeps, s.ctypes.data, len(s), out.ctypes.data))
+    return out
