@@ -45,6 +45,7 @@ BYTES_PER_UNIT = {
     # cache) read and written, step, actions, energy, time, range, MFP,
     # macro XS, deposition, step count
     "along-step-general-linear": (218, 2, "along_charged"),
+    "along-step-uniform-msc": (226, 2, "along_charged"),
     "along-step-neutral": (194, 2, "along_neutral"),
     # geometry state read and written, material, status
     "geo-boundary": (120, 2, "boundary"),
@@ -54,6 +55,9 @@ BYTES_PER_UNIT = {
 def parse_args():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", choices=("testem3", "cylinder"), default="testem3",
+                    help="testem3: the TestEm3 calorimeter (default); cylinder: the "
+                         "barrel detector in a 3.8 T field")
     ap.add_argument("--steps", type=int, default=3, help="timed full transports")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--primaries", type=int, default=10000)
@@ -66,6 +70,13 @@ def parse_args():
     ap.add_argument("--cpu-primaries", type=int, default=48,
                     help="primaries in the CPU baseline sample (0: skip)")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--serve-events", type=int, default=64,
+                    help="events of the served TestEm3 workload (0: skip serving)")
+    ap.add_argument("--serve-baseline-events", type=int, default=6,
+                    help="events of the same stream run one at a time through the plain "
+                         "FP64 loop, for the serving speedup")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed-precision measurement of the same workload")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel on a buffer of this size "
                          "(reference for memory-counter profiles) and exit")
@@ -115,24 +126,35 @@ def main():
         raise SystemExit("bench.py: no HIP device available")
 
     log("generating physics tables")
-    physics = amd.generate_physics(wl.testem3_physics_spec())
-    problem = wl.testem3_problem(physics=physics, init_capacity=1 << 26,
-                                 max_events=max(args.primaries, 1),
-                                 track_order=args.track_order)
-    primaries = wl.testem3_primaries(args.primaries, seed=args.seed)
+    if args.workload == "cylinder":
+        physics = amd.generate_physics(wl.cylinder_physics_spec())
+        make_problem = wl.cylinder_problem
+        problem = wl.cylinder_problem(physics=physics, init_capacity=1 << 26,
+                                      max_events=max(args.primaries, 1),
+                                      track_order=args.track_order)
+        primaries = wl.cylinder_primaries(args.primaries, seed=args.seed)
+        ndet = max(wl.cylinder_geometry()[2]) + 1
+    else:
+        physics = amd.generate_physics(wl.testem3_physics_spec())
+        make_problem = wl.testem3_problem
+        problem = wl.testem3_problem(physics=physics, init_capacity=1 << 26,
+                                     max_events=max(args.primaries, 1),
+                                     track_order=args.track_order)
+        primaries = wl.testem3_primaries(args.primaries, seed=args.seed)
+        ndet = max(problem["volume_to_detector"]) + 1
     params = amd.CoreParams(problem)
 
     stepper = amd.Stepper(params, args.slots)
     log(f"state: {args.slots} slots")
     for w in range(args.warmup):
-        r = stepper.run(primaries)
+        r = stepper.run(primaries, max_steps=20_000)
         log(f"warmup {w}: {r}")
 
     stepper.state.synchronize()
     t0 = time.perf_counter()
     totals = []
     for k in range(args.steps):
-        totals.append(stepper.run(primaries))
+        totals.append(stepper.run(primaries, max_steps=20_000))
         log(f"step {k}: {totals[-1]} at {time.perf_counter() - t0:.2f}s")
     stepper.state.synchronize()
     elapsed = time.perf_counter() - t0
@@ -140,7 +162,22 @@ def main():
     track_steps = sum(t["track_steps"] for t in totals)
     value = track_steps / elapsed
     events_per_s = args.steps * args.primaries / elapsed
-    edep = stepper.state.get("calo_edep", max(problem["volume_to_detector"]) + 1)
+    edep = stepper.state.get("calo_edep", ndet)
+    if args.workload == "cylinder":
+        workload = ("Cylindrical detector (Si tracker 10 x 0.3 mm, PbWO4 ECAL 23 cm, "
+                    "HCAL 20 x (5 cm brass + 4 mm scintillator)) in a uniform 3.8 T "
+                    f"field, {args.primaries} primaries e-/e+/gamma 40/20/40, "
+                    "log-uniform 0.5-50 GeV, isotropic, 2^20 slots, FP64, "
+                    "Dormand-Prince propagation")
+        physics_label = ("Compton, Livermore PE, Bethe-Heitler, e+ annihilation, "
+                         "Moller-Bhabha ionisation, Seltzer-Berger and relativistic "
+                         "brems, Urban MSC, Urban energy-loss fluctuations")
+    else:
+        workload = ("TestEm3 50 x (2.3 mm Pb + 5.7 mm lAr), 10000 primaries e-/gamma "
+                    "50/50, log-uniform 1-100 GeV, 5 deg cone, 2^20 slots, FP64")
+        physics_label = ("Compton, Livermore PE, Bethe-Heitler, e+ annihilation, "
+                         "Moller-Bhabha ionisation, SB/relativistic brems, Urban MSC, "
+                         "Urban energy-loss fluctuations")
 
     line = {
         "metric": "steps/s and events/s, FP32-mixed vs FP64 on 1 GPU; p50/p99 event latency",
@@ -159,23 +196,110 @@ def main():
         "track_steps_per_event": track_steps / (args.steps * args.primaries),
         "iterations_per_step": totals[-1]["iterations"],
         "config": {
-            "workload": "TestEm3 50 x (2.3 mm Pb + 5.7 mm lAr), 10000 primaries e-/gamma "
-                        "50/50, log-uniform 1-100 GeV, 5 deg cone, 2^20 slots, FP64",
+            "workload": workload,
             "primaries": args.primaries,
             "slots": args.slots,
             "track_order": args.track_order,
-            "physics": "Compton, Livermore PE, Bethe-Heitler, e+ annihilation, "
-                       "Moller-Bhabha ionisation, SB/relativistic brems, Urban MSC, "
-                       "Urban energy-loss fluctuations",
+            "physics": physics_label,
             "edep_total_mev": float(edep.sum()),
         },
     }
 
+    if not args.no_mixed:
+        line["mixed"] = mixed_run(amd, make_problem, physics, primaries, args, value,
+                                  events_per_s)
+    if args.serve_events > 0 and args.workload == "testem3":
+        line["serve"] = serve_run(amd, wl, physics, args)
     if not args.no_roofline:
         line["roofline"] = roofline(amd, params, primaries, args.slots)
     if args.cpu_primaries > 0:
         line["cpu_baseline"] = cpu_baseline(amd, wl, problem, args)
     print(json.dumps(line), flush=True)
+
+
+def mixed_run(amd, make_problem, physics, primaries, args, fp64_value, fp64_events):
+    """Same workload with the physics kernels in the mixed-precision mode."""
+    problem = make_problem(physics=physics, init_capacity=1 << 26,
+                           max_events=max(args.primaries, 1), track_order=args.track_order,
+                           precision="mixed")
+    params = amd.CoreParams(problem)
+    stepper = amd.Stepper(params, args.slots)
+    for w in range(args.warmup):
+        log(f"mixed warmup {w}: {stepper.run(primaries, max_steps=20_000)}")
+    stepper.state.synchronize()
+    t0 = time.perf_counter()
+    steps = 0
+    for k in range(args.steps):
+        r = stepper.run(primaries, max_steps=20_000)
+        steps += r["track_steps"]
+        log(f"mixed step {k}: {r} at {time.perf_counter() - t0:.2f}s")
+    stepper.state.synchronize()
+    dt = time.perf_counter() - t0
+    value = steps / dt
+    events = args.steps * args.primaries / dt
+    log(f"mixed precision: {value:.4g} track-steps/s, {events:.4g} events/s")
+    return {"value": value, "unit": "track-steps/s", "events_per_s": events,
+            "ms_per_step": 1e3 * dt / args.steps, "speedup_events_vs_fp64": events / fp64_events,
+            "dtype": "f32 transcendentals and draws in sampling, interpolation and MSC; "
+                     "f64 positions, safety, field, energies and tallies"}
+
+
+def serve_run(amd, wl, physics, args):
+    """Served TestEm3 stream: mixed precision, action-sorted, continuous
+    batching on 2^21 slots, against the same events one at a time through
+    the plain FP64 loop (unsorted, 2^20 slots)."""
+    from celeritas_amd import serving
+
+    events = wl.served_events(args.serve_events, seed=args.seed)
+    energy = float(sum(e["energy"].sum() for e in events))
+    served = amd.CoreParams(wl.testem3_problem(
+        physics=physics, init_capacity=1 << 26, max_events=1 << 16,
+        track_order="reindex_both_action", precision="mixed"))
+    server = serving.Server(served, 1 << 21)
+    server.run(events[:2], [0.0, 0.0])  # warm-up
+    sat = server.run(events, serving.poisson_arrivals(len(events), float("inf")))
+    log(f"served (saturated): {sat['events_per_s']:.3g} events/s, "
+        f"{sat['track_steps'] / sat['wall_time']:.3g} track-steps/s")
+    rate = 0.5 * sat["events_per_s"]
+    half = server.run(events, serving.poisson_arrivals(len(events), rate, seed=args.seed))
+    log(f"served at {rate:.3g} Hz: p50 {half['p50_latency']:.3g} s, "
+        f"p99 {half['p99_latency']:.3g} s")
+
+    # Plain FP64 loop, one event at a time (LocalTransporter-style flushes)
+    plain = amd.CoreParams(wl.testem3_problem(physics=physics, init_capacity=1 << 26,
+                                              max_events=1 << 16))
+    stepper = amd.Stepper(plain, 1 << 20)
+    nb = min(args.serve_baseline_events, len(events))
+    stepper.run(events[0][:4])  # warm-up
+    stepper.state.synchronize()
+    t0 = time.perf_counter()
+    for ev in events[:nb]:
+        stepper.run(ev)
+    stepper.state.synchronize()
+    dt = time.perf_counter() - t0
+    plain_rate = nb / dt
+    plain_energy = float(sum(e["energy"].sum() for e in events[:nb]))
+    log(f"plain FP64 loop: {plain_rate:.3g} events/s over {nb} events")
+    # Events differ in size: compare energy throughput as well
+    served_gev = energy / 1e3 / sat["wall_time"]
+    plain_gev = plain_energy / 1e3 / dt
+    return {
+        "workload": f"{len(events)} TestEm3 events, primaries per event log-uniform "
+                    "10-2000, energies log-uniform 0.1-50 GeV, e-/e+/gamma 40/20/40, "
+                    "mixed precision, reindex_both_action, 2^21 slots",
+        "events_per_s": sat["events_per_s"],
+        "track_steps_per_s": sat["track_steps"] / sat["wall_time"],
+        "primary_gev_per_s": served_gev,
+        "saturated_p50_latency_s": sat["p50_latency"],
+        "saturated_p99_latency_s": sat["p99_latency"],
+        "half_load_rate_hz": rate,
+        "half_load_p50_latency_s": half["p50_latency"],
+        "half_load_p99_latency_s": half["p99_latency"],
+        "plain_fp64_events_per_s": plain_rate,
+        "plain_fp64_primary_gev_per_s": plain_gev,
+        "plain_fp64_sample": f"first {nb} events, one at a time, FP64, unsorted, 2^20 slots",
+        "speedup_gev_per_s_vs_plain_fp64": served_gev / plain_gev,
+    }
 
 
 def roofline(amd, params, primaries, slots):
@@ -220,7 +344,10 @@ def cpu_baseline(amd, wl, problem, args):
     """The oracle (C++ restatement of the loop) on a sample of the primaries."""
     from tests import oracle_bind as ob
 
-    sample = wl.testem3_primaries(args.cpu_primaries, seed=args.seed + 1)
+    if args.workload == "cylinder":
+        sample = wl.cylinder_primaries(args.cpu_primaries, seed=args.seed + 1)
+    else:
+        sample = wl.testem3_primaries(args.cpu_primaries, seed=args.seed + 1)
     for i in range(len(sample)):
         sample["event_id"][i] = i
     oprob = ob.Problem(problem)

@@ -352,6 +352,13 @@ class CoreState:
     def reseed(self, event_id: int):
         _check(_load().celer_state_reseed(self._h, event_id))
 
+    def set_block_size(self, block_size: int):
+        """Workgroup size of the step kernels (64, 128 or 256)."""
+        f = _load().celer_state_set_block_size
+        f.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+        f.restype = ctypes.c_int
+        _check(f(self._h, block_size))
+
     def rng_initialize(self, subsequence: Sequence[int], offset: Sequence[int]):
         sub = np.ascontiguousarray(subsequence, dtype=np.uint64)
         off = np.ascontiguousarray(offset, dtype=np.uint64)
@@ -412,6 +419,13 @@ class Stepper:
     def warm_up(self):
         _check(_load().celer_stepper_warm_up(self._h))
 
+    def capture_step(self):
+        """Replay later steps without primaries from a captured hipGraph."""
+        f = _load().celer_stepper_capture_step
+        f.argtypes = [ctypes.c_void_p]
+        f.restype = ctypes.c_int
+        _check(f(self._h))
+
     def kill_active(self):
         _check(_load().celer_stepper_kill_active(self._h))
 
@@ -454,7 +468,7 @@ class Stepper:
         _check(f(self._h, secs.ctypes.data, counts.ctypes.data, n, int(reset)))
         return {lab: (float(s), int(c)) for lab, s, c in zip(labels, secs, counts)}
 
-    def run(self, primaries: np.ndarray, max_steps: int = 1_000_000) -> dict:
+    def run(self, primaries: np.ndarray, max_steps: int = 100_000) -> dict:
         """Transport primaries to completion (celer-sim's Transporter loop:
         Runner.cc/Transporter.cc); returns step iterations and the sum of
         active tracks over iterations (track-steps)."""

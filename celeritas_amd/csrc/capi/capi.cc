@@ -10,6 +10,7 @@
 #include <string>
 
 #include "accel/LocalTransporter.hh"
+#include "accel/ServingLoop.hh"
 #include "global/CoreParams.hh"
 #include "global/CoreState.hh"
 #include "phys/TableGen.hh"
@@ -47,6 +48,13 @@ struct CelerTransporter
 {
     std::shared_ptr<CoreParams> params;
     std::unique_ptr<LocalTransporter> transporter;
+    CelerState state_handle;
+};
+
+struct CelerServer
+{
+    std::shared_ptr<CoreParams> params;
+    std::unique_ptr<ServingLoop> loop;
     CelerState state_handle;
 };
 
@@ -469,6 +477,76 @@ int celer_transporter_push(CelerTransporter* t, const CelerPrimary* tracks, uint
         for (uint32_t i = 0; i < count; ++i)
             t->transporter->Push(reinterpret_cast<Primary const&>(tracks[i]));
     });
+}
+
+CelerServer* celer_server_create(CelerParams* params, uint32_t num_track_slots,
+                                 uint32_t stream_id)
+{
+    return guarded_new<CelerServer>([&] {
+        auto* result = new CelerServer;
+        result->params = params->params;
+        result->loop = std::make_unique<ServingLoop>(params->params, num_track_slots,
+                                                     stream_id);
+        result->state_handle.params = params->params;
+        result->state_handle.state = result->loop->stepper().sp_state();
+        return result;
+    });
+}
+
+void celer_server_destroy(CelerServer* s)
+{
+    delete s;
+}
+
+CelerState* celer_server_state(CelerServer* s)
+{
+    return &s->state_handle;
+}
+
+int celer_server_run(CelerServer* s,
+                     const CelerPrimary* primaries,
+                     const uint32_t* event_offsets,
+                     const double* arrivals,
+                     uint32_t num_events,
+                     uint32_t max_queued,
+                     uint32_t max_steps,
+                     double* admit_times,
+                     double* done_times,
+                     double* edep,
+                     double* totals)
+{
+    return guarded([&] {
+        std::vector<ServeEvent> events(num_events);
+        for (uint32_t e = 0; e < num_events; ++e)
+        {
+            CELER_VALIDATE(event_offsets[e + 1] >= event_offsets[e],
+                           << "event offsets must not decrease");
+            events[e].first_primary = event_offsets[e];
+            events[e].num_primaries = event_offsets[e + 1] - event_offsets[e];
+            events[e].arrival = arrivals[e];
+        }
+        ServeOptions opts;
+        opts.max_queued = max_queued;
+        opts.max_steps = max_steps;
+        ServeResult r = s->loop->run(reinterpret_cast<Primary const*>(primaries),
+                                     events.data(), num_events, opts);
+        std::copy(r.admit_time.begin(), r.admit_time.end(), admit_times);
+        std::copy(r.done_time.begin(), r.done_time.end(), done_times);
+        std::copy(r.edep.begin(), r.edep.end(), edep);
+        totals[0] = r.wall_time;
+        totals[1] = static_cast<double>(r.steps);
+        totals[2] = static_cast<double>(r.track_steps);
+    });
+}
+
+int celer_state_set_block_size(CelerState* s, uint32_t block_size)
+{
+    return guarded([&] { s->state->block_size(block_size); });
+}
+
+int celer_stepper_capture_step(CelerStepper* s)
+{
+    return guarded([&] { s->stepper->capture_step(); });
 }
 
 int celer_transporter_flush(CelerTransporter* t)

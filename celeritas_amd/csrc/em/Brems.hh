@@ -19,9 +19,12 @@
 //---------------------------------------------------------------------------//
 #pragma once
 
+#include "corecel/Precision.hh"
 #include "StandardEm.hh"
 
 namespace celeritas_amd
+{
+inline namespace CELER_PREC_NS
 {
 namespace brems
 {
@@ -78,7 +81,7 @@ CELER_FORCEINLINE_FUNCTION double
 positron_invbeta(double inc_energy, double mass, double gamma_energy)
 {
     double const e = inc_energy - gamma_energy;
-    return (e + mass) / std::sqrt(e * (e + 2 * mass));
+    return (e + mass) / mth::sqrt(e * (e + 2 * mass));
 }
 
 //! Tsai's DCS without LPM (RBDiffXsCalculator::dxsec_per_atom, Z >= 5)
@@ -100,11 +103,11 @@ rb_dxsec(ElementRecord const& el, double total_energy, double gamma_energy)
     double const eps = term1 * el.rb_epsilon_factor;
     double const gam2 = gam * gam;
     double const eps2 = eps * eps;
-    double const phi1 = 16.863 - 2 * std::log(1 + 0.311877 * gam2)
-                        + 2.4 * std::exp(-0.9 * gam) + 1.6 * std::exp(-1.5 * gam);
+    double const phi1 = 16.863 - 2 * mth::log(1 + 0.311877 * gam2)
+                        + 2.4 * mth::exp(-0.9 * gam) + 1.6 * mth::exp(-1.5 * gam);
     double const phi2 = 2 / (3 + 19.5 * gam + 18 * gam2);
-    double const psi1 = 24.34 - 2 * std::log(1 + 13.111641 * eps2)
-                        + 2.8 * std::exp(-8 * eps) + 1.2 * std::exp(-29.2 * eps);
+    double const psi1 = 24.34 - 2 * mth::log(1 + 13.111641 * eps2)
+                        + 2.8 * mth::exp(-8 * eps) + 1.2 * mth::exp(-29.2 * eps);
     double const psi2 = 2 / (3 + 120 * eps + 1200 * eps2);
     double const dxsec
         = term0
@@ -127,14 +130,18 @@ CELER_FUNCTION double sample_rb_energy(ElementRecord const& el,
     double const tmin = kc < inc_energy ? kc : inc_energy;
     double const tmax = high_energy_limit < inc_energy ? high_energy_limit : inc_energy;
     double const a = tmin * tmin + dens;
-    double const logratio = std::log((1 / a) * (tmax * tmax + dens));
+    double const logratio = mth::log((1 / a) * (tmax * tmax + dens));
     double const max_xs = el.rb_factor1 + el.rb_factor2;
     double k, dsigma;
     do
     {
-        k = std::sqrt(a * std::exp(logratio * generate_canonical<double>(rng)) - dens);
+        k = mth::sqrt(a * mth::exp(logratio * draw_canonical(rng)) - dens);
+#if CELER_PREC_IS_MIXED
+        // FP32 exp/log round trips may overshoot the upper edge by ~1e-7
+        k = k < tmax ? k : tmax;
+#endif
         dsigma = rb_dxsec(el, total, k);
-    } while (dsigma < max_xs * generate_canonical<double>(rng));
+    } while (dsigma < max_xs * draw_canonical(rng));
     return k;
 }
 
@@ -157,21 +164,25 @@ CELER_FUNCTION double sample_sb_energy(PhysicsParamsRef const& p,
     std::uint32_t const ny = el.sb_ny;
     double const total = inc_energy + mass;
     double const dens = electron_density * migdal_constant() * (total * total);
-    GridInterp const xl = find_interp(xg, nx, std::log(inc_energy));
+    GridInterp const xl = find_interp(xg, nx, mth::log(inc_energy));
     double const* row0 = vals + std::size_t(xl.index) * ny;
     double const* row1 = row0 + ny;
     double const max_xs = (1 - xl.fraction) * row0[argmax[xl.index]]
                           + xl.fraction * row1[argmax[xl.index + 1]];
     double const inv_e = 1 / inc_energy;
     double const a = kc * kc + dens;
-    double const logratio = std::log((1 / a) * (inc_energy * inc_energy + dens));
+    double const logratio = mth::log((1 / a) * (inc_energy * inc_energy + dens));
     double const alpha_z = 2 * phys::pi * alpha_fine * el.z;
     double const cutoff_invbeta
         = is_electron ? 0 : positron_invbeta(inc_energy, mass, kc);
     double exit, xs;
     do
     {
-        exit = std::sqrt(a * std::exp(logratio * generate_canonical<double>(rng)) - dens);
+        exit = mth::sqrt(a * mth::exp(logratio * draw_canonical(rng)) - dens);
+#if CELER_PREC_IS_MIXED
+        // FP32 exp/log round trips may overshoot the upper edge by ~1e-7
+        exit = exit < inc_energy ? exit : inc_energy;
+#endif
         GridInterp const yl = find_interp(yg, ny, exit * inv_e);
         std::uint32_t const j = yl.index;
         double const v = (1 - xl.fraction)
@@ -182,10 +193,10 @@ CELER_FUNCTION double sample_sb_energy(PhysicsParamsRef const& p,
         if (!is_electron)
         {
             double const delta = cutoff_invbeta - positron_invbeta(inc_energy, mass, exit);
-            scale = std::exp(alpha_z * (delta < 0 ? delta : 0.0));
+            scale = mth::exp(alpha_z * (delta < 0 ? delta : 0.0));
         }
         xs = v * scale;
-    } while (xs < max_xs * generate_canonical<double>(rng));
+    } while (xs < max_xs * draw_canonical(rng));
     return exit;
 }
 
@@ -225,7 +236,7 @@ CELER_FUNCTION Interaction sample_brems(PhysicsParamsRef const& p,
                                            particle == p.electron_id, rng);
     double const costheta = sample_tsai_urban(energy, mass, rng);
     // BremFinalStateHelper
-    double const inc_momentum = std::sqrt(energy * energy + 2 * mass * energy);
+    double const inc_momentum = mth::sqrt(energy * energy + 2 * mass * energy);
     Secondary& sec = s.secondaries.storage[start];
     sample_exiting_direction(costheta, dir, rng, sec.dir);
     sec.particle_id = p.gamma_id;
@@ -238,4 +249,5 @@ CELER_FUNCTION Interaction sample_brems(PhysicsParamsRef const& p,
 }
 
 //---------------------------------------------------------------------------//
+}  // namespace CELER_PREC_NS
 }  // namespace celeritas_amd

@@ -12,9 +12,12 @@
 
 #include <cmath>
 
+#include "corecel/Precision.hh"
 #include "global/CoreData.hh"
 
 namespace celeritas_amd
+{
+inline namespace CELER_PREC_NS
 {
 //---------------------------------------------------------------------------//
 //! Result of sampling an interaction (celeritas/phys/Interaction.hh)
@@ -45,7 +48,7 @@ CELER_FORCEINLINE_FUNCTION double dot3(double const* a, double const* b)
 
 CELER_FORCEINLINE_FUNCTION void make_unit(double* v)
 {
-    double const scale = 1 / std::sqrt(dot3(v, v));
+    double const scale = 1 / mth::sqrt(dot3(v, v));
     v[0] *= scale;
     v[1] *= scale;
     v[2] *= scale;
@@ -54,13 +57,13 @@ CELER_FORCEINLINE_FUNCTION void make_unit(double* v)
 CELER_FORCEINLINE_FUNCTION void
 from_spherical(double costheta, double phi, double* out)
 {
-    double const sintheta = std::sqrt(1 - costheta * costheta);
+    double const sintheta = mth::sqrt(1 - costheta * costheta);
     double s, c;
 #if CELER_DEVICE_COMPILE
     sincos(phi, &s, &c);
 #else
-    s = std::sin(phi);
-    c = std::cos(phi);
+    s = mth::sin(phi);
+    c = mth::cos(phi);
 #endif
     out[0] = sintheta * c;
     out[1] = sintheta * s;
@@ -70,7 +73,7 @@ from_spherical(double costheta, double phi, double* out)
 //! Rotate a direction given in the frame whose z axis is rot
 CELER_FUNCTION void rotate(double const* dir, double const* rot, double* out)
 {
-    double const sintheta = std::sqrt(1 - rot[2] * rot[2]);
+    double const sintheta = mth::sqrt(1 - rot[2] * rot[2]);
     double cosphi, sinphi;
     if (sintheta >= 0.005)
     {
@@ -80,8 +83,8 @@ CELER_FUNCTION void rotate(double const* dir, double const* rot, double* out)
     }
     else if (sintheta > 0)
     {
-        cosphi = rot[0] / std::sqrt(rot[0] * rot[0] + rot[1] * rot[1]);
-        sinphi = std::sqrt(1 - cosphi * cosphi);
+        cosphi = rot[0] / mth::sqrt(rot[0] * rot[0] + rot[1] * rot[1]);
+        sinphi = mth::sqrt(1 - cosphi * cosphi);
     }
     else
     {
@@ -129,4 +132,5 @@ CELER_FUNCTION std::uint32_t allocate_secondaries(CoreStateRef const& s,
 }
 
 //---------------------------------------------------------------------------//
+}  // namespace CELER_PREC_NS
 }  // namespace celeritas_amd

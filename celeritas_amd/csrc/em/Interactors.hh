@@ -6,6 +6,7 @@
 //---------------------------------------------------------------------------//
 #pragma once
 
+#include "corecel/Precision.hh"
 #include "geo/OrangeLiteTracker.hh"
 #include "Brems.hh"
 #include "InteractorCommon.hh"
@@ -14,6 +15,8 @@
 #include "StandardEm.hh"
 
 namespace celeritas_amd
+{
+inline namespace CELER_PREC_NS
 {
 //---------------------------------------------------------------------------//
 //! Everything an interactor may need besides the RNG
@@ -40,16 +43,25 @@ CELER_FUNCTION std::uint32_t sample_element(PhysicsParamsRef const& p,
                                             double energy,
                                             Engine& rng)
 {
-    (void)proc;
-    (void)model_index;
-    (void)energy;
-    (void)rng;
     if (p.num_materials == 0 || p.materials == nullptr)
         return 0;
     MaterialRecord const& m = p.materials[mat];
-    if (m.num_elements <= 1)
+    std::uint32_t const table = proc.models[model_index].elem_cdf;
+    if (m.num_elements <= 1 || table == invalid_id)
         return 0;
-    return 0;
+    std::uint32_t const first = p.elem_cdf_index[table + mat];
+    if (first == invalid_id)
+        return 0;
+    // TabulatedElementSelector::operator() (TabulatedElementSelector.hh:
+    // 75-95): the first element whose normalised cumulative XS exceeds u
+    double const u = draw_canonical(rng);
+    std::uint32_t i = 0;
+    for (; i + 1 < m.num_elements; ++i)
+    {
+        if (phys::calc_grid(p.grids[first + i], p.reals, energy) > u)
+            break;
+    }
+    return i;
 }
 
 //! Dispatch to the model's interactor
@@ -124,6 +136,15 @@ CELER_FUNCTION void apply_interaction(CoreParamsRef const& params,
     }
     if (r.action == Interaction::unchanged)
         return;
+#if CELER_PREC_IS_MIXED
+    // Reduced-precision sampling must not leave a non-physical energy
+    if (CELER_UNLIKELY(!(r.energy >= 0 && r.energy < HUGE_VAL)))
+    {
+        state.sim.status[slot] = static_cast<std::uint8_t>(TrackStatus::errored);
+        state.sim.post_step_action[slot] = params.scalars.tracking_cut_action;
+        return;
+    }
+#endif
     state.particle.energy[slot] = r.energy;
     if (r.action != Interaction::absorbed)
     {
@@ -158,4 +179,5 @@ CELER_FUNCTION void apply_interaction(CoreParamsRef const& params,
 }
 
 //---------------------------------------------------------------------------//
+}  // namespace CELER_PREC_NS
 }  // namespace celeritas_amd

@@ -14,11 +14,14 @@
 //---------------------------------------------------------------------------//
 #pragma once
 
+#include "corecel/Precision.hh"
 #include "phys/PhysicsImpl.hh"
 #include "random/XorwowEngine.hh"
 #include "InteractorCommon.hh"
 
 namespace celeritas_amd
+{
+inline namespace CELER_PREC_NS
 {
 //---------------------------------------------------------------------------//
 //! Electron kinetic energy below which the recoil is deposited locally
@@ -40,34 +43,34 @@ CELER_FUNCTION Interaction sample_klein_nishina(PhysicsParamsRef const& p,
     }
     double const k = energy * p.model.inv_electron_mass;
     double const eps0 = 1 / (1 + 2 * k);
-    double const a = -std::log(eps0);
+    double const a = -mth::log(eps0);
     double const b = 0.5 * (1 - eps0 * eps0);
     double const p_f1 = a / (a + b);
-    double const logratio = std::log(eps0);
+    double const logratio = mth::log(eps0);
     double const f2_lo = eps0 * eps0;
     double const f2_delta = 1 - f2_lo;
     double eps, omc;
     while (true)
     {
         double eps_sq;
-        if (generate_canonical<double>(rng) < p_f1)
+        if (draw_canonical(rng) < p_f1)
         {
-            eps = std::exp(logratio * generate_canonical<double>(rng));
+            eps = mth::exp(logratio * draw_canonical(rng));
             eps_sq = eps * eps;
         }
         else
         {
-            eps_sq = std::fma(f2_delta, generate_canonical<double>(rng), f2_lo);
-            eps = std::sqrt(eps_sq);
+            eps_sq = std::fma(f2_delta, draw_canonical(rng), f2_lo);
+            eps = mth::sqrt(eps_sq);
         }
         omc = (1 - eps) / (eps * k);
         double const sin2 = omc * (2 - omc);
         double const reject = eps * sin2 / (1 + eps_sq);
-        if (!(generate_canonical<double>(rng) < reject))
+        if (!(draw_canonical(rng) < reject))
             break;
     }
     r.energy = eps * energy;
-    double const phi = std::fma(2 * phys::pi, generate_canonical<double>(rng), 0.0);
+    double const phi = std::fma(2 * phys::pi, draw_canonical(rng), 0.0);
     double local[3];
     from_spherical(1 - omc, phi, local);
     rotate(local, dir, r.dir);
@@ -92,4 +95,5 @@ CELER_FUNCTION Interaction sample_klein_nishina(PhysicsParamsRef const& p,
 }
 
 //---------------------------------------------------------------------------//
+}  // namespace CELER_PREC_NS
 }  // namespace celeritas_amd

@@ -14,10 +14,13 @@
 //---------------------------------------------------------------------------//
 #pragma once
 
+#include "corecel/Precision.hh"
 #include "phys/LivermoreXs.hh"
 #include "StandardEm.hh"
 
 namespace celeritas_amd
+{
+inline namespace CELER_PREC_NS
 {
 //---------------------------------------------------------------------------//
 //! Element component by micro cross section (ElementSelector::operator())
@@ -31,7 +34,7 @@ select_pe_element(PhysicsParamsRef const& p, std::uint32_t mat, double energy, E
     for (std::uint32_t i = 0; i < m.num_elements; ++i)
         material_xs += comps[i].fraction
                        * phys::livermore_pe_micro_xs(p, comps[i].element, energy);
-    double accum = -material_xs * generate_canonical<double>(rng);
+    double accum = -material_xs * draw_canonical(rng);
     std::uint32_t i = 0;
     std::uint32_t const imax = m.num_elements - 1;
     for (; i != imax; ++i)
@@ -60,17 +63,17 @@ sample_sauter_gavrila(double inv_electron_mass, double inc_energy,
     double const energy_per_mecsq = (inc_energy > 1e-6 ? inc_energy : 1e-6)
                                     * inv_electron_mass;
     double const gamma = energy_per_mecsq + 1;
-    double const beta = std::sqrt(energy_per_mecsq * (gamma + 1)) / gamma;
+    double const beta = mth::sqrt(energy_per_mecsq * (gamma + 1)) / gamma;
     double const a = (1 - beta) / beta;
     double const b = 0.5 * beta * gamma * energy_per_mecsq * (gamma - 2);
     double const g_max = 2 * (1 / a + b);
     double g, nu;
     do
     {
-        double const u = generate_canonical<double>(rng);
-        nu = 2 * a * (2 * u + (a + 2) * std::sqrt(u)) / ((a + 2) * (a + 2) - 4 * u);
+        double const u = draw_canonical(rng);
+        nu = 2 * a * (2 * u + (a + 2) * mth::sqrt(u)) / ((a + 2) * (a + 2) - 4 * u);
         g = (2 - nu) * (1 / (a + nu) + b);
-    } while (g < g_max * generate_canonical<double>(rng));
+    } while (g < g_max * draw_canonical(rng));
     sample_exiting_direction(1 - nu, inc_dir, rng, out);
 }
 
@@ -97,7 +100,7 @@ CELER_FUNCTION Interaction sample_livermore_pe(PhysicsParamsRef const& p,
 
     // Subshell (sample_subshell)
     std::uint32_t shell_id = 0;
-    double const cutoff = generate_canonical<double>(rng)
+    double const cutoff = draw_canonical(rng)
                           * phys::livermore_pe_micro_xs(p, element, inc_energy);
     if (inc_energy < el.thresh_lo)
     {
@@ -145,4 +148,5 @@ CELER_FUNCTION Interaction sample_livermore_pe(PhysicsParamsRef const& p,
 }
 
 //---------------------------------------------------------------------------//
+}  // namespace CELER_PREC_NS
 }  // namespace celeritas_amd

@@ -21,11 +21,14 @@
 //---------------------------------------------------------------------------//
 #pragma once
 
+#include "corecel/Precision.hh"
 #include "phys/PhysicsImpl.hh"
 #include "random/XorwowEngine.hh"
 #include "InteractorCommon.hh"
 
 namespace celeritas_amd
+{
+inline namespace CELER_PREC_NS
 {
 //---------------------------------------------------------------------------//
 
@@ -37,10 +40,10 @@ CELER_FUNCTION double sample_tsai_urban(double energy, double mass, Engine& rng)
     double u;
     do
     {
-        double const a = generate_canonical<double>(rng);
-        double const b = generate_canonical<double>(rng);
-        double const uu = -std::log(a * b);
-        u = uu * (generate_canonical<double>(rng) < 0.25 ? 1.6 : 1.6 / 3);
+        double const a = draw_canonical(rng);
+        double const b = draw_canonical(rng);
+        double const uu = -mth::log(a * b);
+        u = uu * (draw_canonical(rng) < 0.25 ? 1.6 : 1.6 / 3);
     } while (u > umax);
     double const r = u / umax;
     return 1 - 2 * r * r;
@@ -51,7 +54,7 @@ template<class Engine>
 CELER_FUNCTION void
 sample_exiting_direction(double mu, double const* dir, Engine& rng, double* out)
 {
-    double const phi = std::fma(2 * phys::pi, generate_canonical<double>(rng), 0.0);
+    double const phi = std::fma(2 * phys::pi, draw_canonical(rng), 0.0);
     double local[3];
     from_spherical(mu, phi, local);
     rotate(local, dir, out);
@@ -74,12 +77,12 @@ CELER_FUNCTION void momentum_direction(double p_in,
 //---------------------------------------------------------------------------//
 CELER_FORCEINLINE_FUNCTION double bh_screening_f1(double d)
 {
-    return d > 1.4 ? 42.038 - 8.29 * std::log(d + 0.958)
+    return d > 1.4 ? 42.038 - 8.29 * mth::log(d + 0.958)
                    : 42.184 - d * (7.444 - 1.623 * d);
 }
 CELER_FORCEINLINE_FUNCTION double bh_screening_f2(double d)
 {
-    return d > 1.4 ? 42.038 - 8.29 * std::log(d + 0.958)
+    return d > 1.4 ? 42.038 - 8.29 * mth::log(d + 0.958)
                    : 41.326 - d * (5.848 - 0.902 * d);
 }
 
@@ -104,7 +107,7 @@ CELER_FUNCTION Interaction sample_bethe_heitler(PhysicsParamsRef const& p,
     double eps;
     if (energy < 2)
     {
-        eps = std::fma(half - eps0, generate_canonical<double>(rng), eps0);
+        eps = std::fma(half - eps0, draw_canonical(rng), eps0);
     }
     else
     {
@@ -112,8 +115,8 @@ CELER_FUNCTION Interaction sample_bethe_heitler(PhysicsParamsRef const& p,
         double f_z = 8.0 / 3.0 * el.log_z;
         if (energy > 50)
             f_z += 8 * el.coulomb_correction;
-        double const delta_max = std::exp((42.038 - f_z) / 8.29) - 0.958;
-        double const eps1 = half - half * std::sqrt(1 - delta_min / delta_max);
+        double const delta_max = mth::exp((42.038 - f_z) / 8.29) - 0.958;
+        double const eps1 = half - half * mth::sqrt(1 - delta_min / delta_max);
         double const eps_min = eps0 > eps1 ? eps0 : eps1;
         double const f10 = bh_screening_f1(delta_min) - f_z;
         double const f20 = bh_screening_f2(delta_min) - f_z;
@@ -123,19 +126,19 @@ CELER_FUNCTION Interaction sample_bethe_heitler(PhysicsParamsRef const& p,
         double g;
         do
         {
-            if (generate_canonical<double>(rng) < p_f1)
+            if (draw_canonical(rng) < p_f1)
             {
-                eps = half - (half - eps_min) * std::cbrt(generate_canonical<double>(rng));
+                eps = half - (half - eps_min) * mth::cbrt(draw_canonical(rng));
                 double const delta = 136 / el.cbrt_z * eps0 / (eps * (1 - eps));
                 g = (bh_screening_f1(delta) - f_z) / f10;
             }
             else
             {
-                eps = eps_min + (half - eps_min) * generate_canonical<double>(rng);
+                eps = eps_min + (half - eps_min) * draw_canonical(rng);
                 double const delta = 136 / el.cbrt_z * eps0 / (eps * (1 - eps));
                 g = (bh_screening_f2(delta) - f_z) / f20;
             }
-        } while (g < generate_canonical<double>(rng));
+        } while (g < draw_canonical(rng));
     }
     r.action = Interaction::absorbed;
     r.energy = 0;
@@ -144,7 +147,7 @@ CELER_FUNCTION Interaction sample_bethe_heitler(PhysicsParamsRef const& p,
     Secondary* sec = s.secondaries.storage + start;
     double e0 = (1 - eps) * energy - me;
     double e1 = eps * energy - me;
-    if (generate_canonical<double>(rng) < half)
+    if (draw_canonical(rng) < half)
     {
         double const t = e0;
         e0 = e1;
@@ -154,7 +157,7 @@ CELER_FUNCTION Interaction sample_bethe_heitler(PhysicsParamsRef const& p,
     sec[1].particle_id = p.positron_id;
     sec[0].energy = e0;
     sec[1].energy = e1;
-    double const phi = std::fma(2 * phys::pi, generate_canonical<double>(rng), 0.0);
+    double const phi = std::fma(2 * phys::pi, draw_canonical(rng), 0.0);
     double local[3];
     double cost = sample_tsai_urban(e0, me, rng);
     from_spherical(cost, phi, local);
@@ -217,8 +220,8 @@ CELER_FUNCTION Interaction sample_moller_bhabha(PhysicsParamsRef const& p,
         double const delta = 1 / min_frac - a;
         do
         {
-            eps = 1 / std::fma(delta, generate_canonical<double>(rng), a);
-        } while (g_den * generate_canonical<double>(rng) > moller_g(gamma, eps));
+            eps = 1 / std::fma(delta, draw_canonical(rng), a);
+        } while (g_den * draw_canonical(rng) > moller_g(gamma, eps));
     }
     else
     {
@@ -227,12 +230,12 @@ CELER_FUNCTION Interaction sample_moller_bhabha(PhysicsParamsRef const& p,
         double const delta = 1 / min_frac - a;
         do
         {
-            eps = 1 / std::fma(delta, generate_canonical<double>(rng), a);
-        } while (g_den * generate_canonical<double>(rng) > bhabha_g(gamma, eps, eps));
+            eps = 1 / std::fma(delta, draw_canonical(rng), a);
+        } while (g_den * draw_canonical(rng) > bhabha_g(gamma, eps, eps));
     }
     double const e_sec = energy * eps;
-    double const inc_momentum = std::sqrt(energy * energy + 2 * me * energy);
-    double const momentum = std::sqrt(e_sec * (e_sec + 2 * me));
+    double const inc_momentum = mth::sqrt(energy * energy + 2 * me * energy);
+    double const momentum = mth::sqrt(e_sec * (e_sec + 2 * me));
     double const costheta = e_sec * (energy + me + me) / (momentum * inc_momentum);
     Secondary& sec = s.secondaries.storage[start];
     sec.energy = e_sec;
@@ -272,8 +275,8 @@ CELER_FUNCTION Interaction sample_eplusgg(PhysicsParamsRef const& p,
     if (energy == 0)
     {
         sec[0].energy = sec[1].energy = me;
-        double const costheta = std::fma(2.0, generate_canonical<double>(rng), -1.0);
-        double const phi = std::fma(2 * phys::pi, generate_canonical<double>(rng), 0.0);
+        double const costheta = std::fma(2.0, draw_canonical(rng), -1.0);
+        double const phi = std::fma(2 * phys::pi, draw_canonical(rng), 0.0);
         from_spherical(costheta, phi, sec[0].dir);
         for (int i = 0; i < 3; ++i)
             sec[1].dir[i] = -sec[0].dir[i];
@@ -282,19 +285,19 @@ CELER_FUNCTION Interaction sample_eplusgg(PhysicsParamsRef const& p,
     constexpr double half = 0.5;
     double const tau = energy / me;
     double const tau2 = tau + 2;
-    double const sqgrate = std::sqrt(tau / tau2) * half;
+    double const sqgrate = mth::sqrt(tau / tau2) * half;
     double const lo = half - sqgrate;
-    double const logratio = std::log((1 / lo) * (half + sqgrate));
+    double const logratio = mth::log((1 / lo) * (half + sqgrate));
     double epsil;
     do
     {
-        epsil = lo * std::exp(logratio * generate_canonical<double>(rng));
-    } while (generate_canonical<double>(rng)
+        epsil = lo * mth::exp(logratio * draw_canonical(rng));
+    } while (draw_canonical(rng)
              < epsil - (2 * (tau + 1) * epsil - 1) / (epsil * tau2 * tau2));
-    double const cost = (epsil * tau2 - 1) / (epsil * std::sqrt(tau * tau2));
+    double const cost = (epsil * tau2 - 1) / (epsil * mth::sqrt(tau * tau2));
     double const total = energy + 2 * me;
     double const eg = epsil * total;
-    double const eplus_moment = std::sqrt(energy * total);
+    double const eplus_moment = mth::sqrt(energy * total);
     sec[0].energy = eg;
     sample_exiting_direction(cost, dir, rng, sec[0].dir);
     sec[1].energy = total - eg;
@@ -303,4 +306,5 @@ CELER_FUNCTION Interaction sample_eplusgg(PhysicsParamsRef const& p,
 }
 
 //---------------------------------------------------------------------------//
+}  // namespace CELER_PREC_NS
 }  // namespace celeritas_amd

@@ -15,6 +15,7 @@
 //! update (TrackUpdater.hh). The geometry, RNG and step state of a track are
 //! read once into registers and written back once.
 //---------------------------------------------------------------------------//
+#include "corecel/Precision.hh"
 #include "CoreActions.hh"
 #include "corecel/WaveOps.hh"
 #include "field/FieldPropagator.hh"
@@ -37,7 +38,7 @@ __global__ void __launch_bounds__(block_size)
                               ThreadSlots const ts)
 {
     size_type slot;
-    if (!thread_to_slot(ts, blockIdx.x * block_size + threadIdx.x, state.size, &slot))
+    if (!thread_to_slot(ts, blockIdx.x * blockDim.x + threadIdx.x, state.size, &slot))
         return;
     auto const status = static_cast<TrackStatus>(state.sim.status[slot]);
     if (status == TrackStatus::inactive || state.sim.along_step_action[slot] != action)
@@ -232,6 +233,10 @@ __global__ void __launch_bounds__(block_size)
         {
             ps.energy_deposition[slot] += deposited;
             energy -= deposited;
+#if CELER_PREC_IS_MIXED
+            if (CELER_UNLIKELY(!(energy >= 0 && energy < HUGE_VAL)))
+                energy = 0;
+#endif
             state.particle.energy[slot] = energy;
         }
         if (energy == 0)
@@ -261,17 +266,16 @@ __global__ void __launch_bounds__(block_size)
 //---------------------------------------------------------------------------//
 }  // namespace
 
-hipError_t launch_along_step_charged(CoreParamsRef const& p,
+hipError_t CELER_PREC_FN(launch_along_step_charged)(CoreParamsRef const& p,
                                      CoreStateRef const& s,
                                      std::uint8_t action,
                                      ThreadSlots const& ts,
                                      hipStream_t st)
 {
-    unsigned int const nb = (s.size + block_size - 1) / block_size;
     if (p.field.enabled)
-        along_step_charged_kernel<true><<<nb, block_size, 0, st>>>(p, s, action, ts);
+        along_step_charged_kernel<true><<<grid_of(s), s.block_size, 0, st>>>(p, s, action, ts);
     else
-        along_step_charged_kernel<false><<<nb, block_size, 0, st>>>(p, s, action, ts);
+        along_step_charged_kernel<false><<<grid_of(s), s.block_size, 0, st>>>(p, s, action, ts);
     return hipGetLastError();
 }
 

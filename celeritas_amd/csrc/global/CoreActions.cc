@@ -21,7 +21,10 @@ void AlongStepNeutralAction::step(CoreParams const& params, CoreState& state) co
 
 void AlongStepChargedAction::step(CoreParams const& params, CoreState& state) const
 {
-    CELER_HIP_CALL(launch_along_step_charged(
+    auto launch = params.precision() == PrecisionMode::mixed
+                      ? launch_along_step_charged_mixed
+                      : launch_along_step_charged;
+    CELER_HIP_CALL(launch(
         params.device_ref(), state.ref(),
         static_cast<std::uint8_t>(this->action_id()),
         make_thread_slots(params, state, *this), state.stream()));

@@ -60,6 +60,28 @@ hipError_t launch_calo_tally(CoreParamsRef const&,
                              hipStream_t);
 hipError_t launch_kill_active(CoreParamsRef const&, CoreStateRef const&, hipStream_t);
 
+// Mixed-precision builds of the physics kernels (FP32 transcendentals and
+// draws in sampling, interpolation and MSC)
+hipError_t launch_pre_step_mixed(CoreParamsRef const&,
+                                 CoreStateRef const&,
+                                 ThreadSlots const&,
+                                 hipStream_t);
+hipError_t launch_along_step_charged_mixed(CoreParamsRef const&,
+                                           CoreStateRef const&,
+                                           std::uint8_t action,
+                                           ThreadSlots const&,
+                                           hipStream_t);
+hipError_t launch_discrete_select_mixed(CoreParamsRef const&,
+                                        CoreStateRef const&,
+                                        ThreadSlots const&,
+                                        hipStream_t);
+hipError_t launch_interaction_mixed(CoreParamsRef const&,
+                                    CoreStateRef const&,
+                                    std::uint8_t action,
+                                    std::uint32_t model,
+                                    ThreadSlots const&,
+                                    hipStream_t);
+
 //---------------------------------------------------------------------------//
 //! Whether tracks are reindexed through track_slots (TrackOrder >= shuffle)
 inline bool is_reindexed(TrackOrder t)

@@ -119,6 +119,9 @@ struct InitStateRef
     std::uint32_t* parents{nullptr};
     std::uint32_t* indices{nullptr};
     std::uint32_t* track_counters{nullptr};  //!< per event
+    //! Per event: initializers plus live tracks (0 once the event is done)
+    std::int32_t* event_live{nullptr};
+    double* event_edep{nullptr};  //!< per event: deposited energy [MeV]
     TrackInitializer* initializers{nullptr};
     std::uint32_t* block_scratch{nullptr};  //!< compaction partials
     size_type capacity{0};
@@ -140,6 +143,8 @@ struct CoreStateRef
 {
     size_type size{0};
     std::uint32_t stream_id{0};
+    //! Workgroup size of the step kernels (64, 128 or 256; autotuned)
+    std::uint32_t block_size{256};
     SimStateRef sim;
     ParticleStateRef particle;
     GeoStateRef geo;
@@ -183,6 +188,12 @@ struct ThreadSlots
     std::uint32_t const* range{nullptr};  //!< &action_offsets[action], or null
     std::uint32_t const* slots{nullptr};  //!< thread -> slot, or null (identity)
 };
+
+//! Number of workgroups covering the state's slots
+inline unsigned int grid_of(CoreStateRef const& s)
+{
+    return (s.size + s.block_size - 1) / s.block_size;
+}
 
 CELER_FORCEINLINE_FUNCTION bool
 thread_to_slot(ThreadSlots const& ts, size_type tid, size_type size, size_type* slot)

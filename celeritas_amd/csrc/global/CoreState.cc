@@ -92,6 +92,8 @@ CoreState::CoreState(CoreParams const& params,
     s.init.capacity = params.init_capacity();
     s.init.max_events = params.max_events();
     add(&s.init.track_counters, s.init.max_events);
+    add(&s.init.event_live, s.init.max_events);
+    add(&s.init.event_edep, s.init.max_events);
     add(&s.init.initializers, s.init.capacity);
     add(&s.init.block_scratch, extend_block_scratch_size(num_slots) + 64);
     double const stack_factor
@@ -246,6 +248,14 @@ std::vector<std::uint64_t> CoreState::read_diagnostics(bool reset)
                                       stream_));
     result.assign(tmp.begin(), tmp.end());
     return result;
+}
+
+//---------------------------------------------------------------------------//
+void CoreState::block_size(size_type bs)
+{
+    CELER_VALIDATE(bs == 64 || bs == 128 || bs == 256,
+                   << "step kernel workgroup size must be 64, 128 or 256 (got " << bs << ")");
+    ref_.block_size = bs;
 }
 
 //---------------------------------------------------------------------------//

@@ -44,6 +44,9 @@ class CoreState
     CoreCounters const& sync_counters();
     //! Last synchronized counters
     CoreCounters const& counters() const { return *host_counters_; }
+    //! Workgroup size of the step kernels: 64, 128 or 256
+    void block_size(size_type bs);
+    size_type block_size() const { return ref_.block_size; }
     //! Enqueue an async copy of the counters to the pinned mirror
     void enqueue_counter_readback();
 

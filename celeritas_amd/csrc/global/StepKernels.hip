@@ -13,6 +13,7 @@
 //! early-exit for slots whose action does not match, exactly like the
 //! reference's ConditionalTrackExecutor.
 //---------------------------------------------------------------------------//
+#include "corecel/Precision.hh"
 #include "CoreActions.hh"
 
 #include "corecel/WaveOps.hh"
@@ -53,7 +54,7 @@ __global__ void __launch_bounds__(block_size)
                     CoreStateRef const state,
                     ThreadSlots const ts)
 {
-    size_type const tid = blockIdx.x * block_size + threadIdx.x;
+    size_type const tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid == 0)
         state.counters->secondary_stack_size = 0;
     size_type slot;
@@ -88,7 +89,7 @@ __global__ void __launch_bounds__(block_size)
     if (!(mfp > 0))
     {
         XorwowEngine rng = XorwowEngine::load(state.rng, slot);
-        mfp = -std::log(generate_canonical<double>(rng));
+        mfp = -mth::log(draw_canonical(rng));
         rng.store(state.rng, slot);
         ps.interaction_mfp[slot] = mfp;
     }
@@ -153,7 +154,7 @@ __global__ void __launch_bounds__(block_size)
                               ThreadSlots const ts)
 {
     size_type slot;
-    if (!thread_to_slot(ts, blockIdx.x * block_size + threadIdx.x, state.size, &slot))
+    if (!thread_to_slot(ts, blockIdx.x * blockDim.x + threadIdx.x, state.size, &slot))
         return;
     auto const status = status_of(state, slot);
     if (status == TrackStatus::inactive || state.sim.along_step_action[slot] != action)
@@ -207,7 +208,7 @@ __global__ void __launch_bounds__(block_size)
                            ThreadSlots const ts)
 {
     size_type slot;
-    if (!thread_to_slot(ts, blockIdx.x * block_size + threadIdx.x, state.size, &slot))
+    if (!thread_to_slot(ts, blockIdx.x * blockDim.x + threadIdx.x, state.size, &slot))
         return;
     PhysicsParamsRef const& pp = params.phys;
     if (state.sim.post_step_action[slot] != pp.scalars.discrete_action
@@ -219,7 +220,7 @@ __global__ void __launch_bounds__(block_size)
     std::uint32_t const mat = material_of(params, state, slot);
     double const energy = state.particle.energy[slot];
     XorwowEngine rng = XorwowEngine::load(state.rng, slot);
-    double accum = -state.phys.macro_xs[slot] * generate_canonical<double>(rng);
+    double accum = -state.phys.macro_xs[slot] * draw_canonical(rng);
     std::uint32_t pidx = part.num_processes - 1;
     for (std::uint32_t i = 0; i + 1 < part.num_processes; ++i)
     {
@@ -236,7 +237,7 @@ __global__ void __launch_bounds__(block_size)
     if (proc.integral)
     {
         double const xs = phys::calc_xs(pp, proc, mat, energy);
-        if (generate_canonical<double>(rng)
+        if (draw_canonical(rng)
                 * state.phys.per_process_xs[pidx * state.size + slot]
             > xs)
         {
@@ -276,7 +277,7 @@ __global__ void __launch_bounds__(block_size)
                        ThreadSlots const ts)
 {
     size_type slot;
-    if (!thread_to_slot(ts, blockIdx.x * block_size + threadIdx.x, state.size, &slot))
+    if (!thread_to_slot(ts, blockIdx.x * blockDim.x + threadIdx.x, state.size, &slot))
         return;
     if (state.sim.post_step_action[slot] != action
         || status_of(state, slot) == TrackStatus::inactive)
@@ -304,7 +305,7 @@ __global__ void __launch_bounds__(block_size)
                     ThreadSlots const ts)
 {
     size_type slot;
-    if (!thread_to_slot(ts, blockIdx.x * block_size + threadIdx.x, state.size, &slot))
+    if (!thread_to_slot(ts, blockIdx.x * blockDim.x + threadIdx.x, state.size, &slot))
         return;
     if (state.sim.post_step_action[slot] != params.scalars.boundary_action
         || status_of(state, slot) != TrackStatus::alive)
@@ -335,7 +336,7 @@ __global__ void __launch_bounds__(block_size)
                         ThreadSlots const ts)
 {
     size_type slot;
-    if (!thread_to_slot(ts, blockIdx.x * block_size + threadIdx.x, state.size, &slot))
+    if (!thread_to_slot(ts, blockIdx.x * blockDim.x + threadIdx.x, state.size, &slot))
         return;
     if (state.sim.post_step_action[slot] != params.scalars.tracking_cut_action
         || status_of(state, slot) == TrackStatus::inactive)
@@ -382,7 +383,7 @@ __global__ void __launch_bounds__(block_size)
         __syncthreads();
     }
     size_type slot;
-    if (thread_to_slot(ts, blockIdx.x * block_size + threadIdx.x, state.size, &slot)
+    if (thread_to_slot(ts, blockIdx.x * blockDim.x + threadIdx.x, state.size, &slot)
         && status_of(state, slot) != TrackStatus::inactive)
     {
         double const edep = state.phys.energy_deposition[slot];
@@ -415,47 +416,49 @@ __global__ void __launch_bounds__(block_size)
 //---------------------------------------------------------------------------//
 // LAUNCHERS
 //---------------------------------------------------------------------------//
-hipError_t launch_pre_step(CoreParamsRef const& p,
+hipError_t CELER_PREC_FN(launch_pre_step)(CoreParamsRef const& p,
                            CoreStateRef const& s,
                            ThreadSlots const& ts,
                            hipStream_t st)
 {
-    pre_step_kernel<<<num_blocks(s.size), block_size, 0, st>>>(p, s, ts);
+    pre_step_kernel<<<grid_of(s), s.block_size, 0, st>>>(p, s, ts);
     return hipGetLastError();
 }
 
+#if !CELER_PREC_IS_MIXED
 hipError_t launch_along_step_neutral(CoreParamsRef const& p,
                                      CoreStateRef const& s,
                                      std::uint8_t action,
                                      ThreadSlots const& ts,
                                      hipStream_t st)
 {
-    along_step_neutral_kernel<<<num_blocks(s.size), block_size, 0, st>>>(p, s, action, ts);
+    along_step_neutral_kernel<<<grid_of(s), s.block_size, 0, st>>>(p, s, action, ts);
     return hipGetLastError();
 }
 
-hipError_t launch_discrete_select(CoreParamsRef const& p,
+#endif
+
+hipError_t CELER_PREC_FN(launch_discrete_select)(CoreParamsRef const& p,
                                   CoreStateRef const& s,
                                   ThreadSlots const& ts,
                                   hipStream_t st)
 {
-    discrete_select_kernel<<<num_blocks(s.size), block_size, 0, st>>>(p, s, ts);
+    discrete_select_kernel<<<grid_of(s), s.block_size, 0, st>>>(p, s, ts);
     return hipGetLastError();
 }
 
-hipError_t launch_interaction(CoreParamsRef const& p,
+hipError_t CELER_PREC_FN(launch_interaction)(CoreParamsRef const& p,
                               CoreStateRef const& s,
                               std::uint8_t action,
                               std::uint32_t model,
                               ThreadSlots const& ts,
                               hipStream_t st)
 {
-    auto const nb = num_blocks(s.size);
     switch (static_cast<ModelClass>(model))
     {
 #define CELER_LAUNCH_MODEL(M)                                              \
     case ModelClass::M:                                                    \
-        interaction_kernel<ModelClass::M><<<nb, block_size, 0, st>>>(p, s, action, ts); \
+        interaction_kernel<ModelClass::M><<<grid_of(s), s.block_size, 0, st>>>(p, s, action, ts); \
         break;
         CELER_LAUNCH_MODEL(klein_nishina)
         CELER_LAUNCH_MODEL(livermore_pe)
@@ -472,12 +475,13 @@ hipError_t launch_interaction(CoreParamsRef const& p,
     return hipGetLastError();
 }
 
+#if !CELER_PREC_IS_MIXED
 hipError_t launch_boundary(CoreParamsRef const& p,
                            CoreStateRef const& s,
                            ThreadSlots const& ts,
                            hipStream_t st)
 {
-    boundary_kernel<<<num_blocks(s.size), block_size, 0, st>>>(p, s, ts);
+    boundary_kernel<<<grid_of(s), s.block_size, 0, st>>>(p, s, ts);
     return hipGetLastError();
 }
 
@@ -486,7 +490,7 @@ hipError_t launch_tracking_cut(CoreParamsRef const& p,
                                ThreadSlots const& ts,
                                hipStream_t st)
 {
-    tracking_cut_kernel<<<num_blocks(s.size), block_size, 0, st>>>(p, s, ts);
+    tracking_cut_kernel<<<grid_of(s), s.block_size, 0, st>>>(p, s, ts);
     return hipGetLastError();
 }
 
@@ -506,6 +510,8 @@ hipError_t launch_calo_tally(CoreParamsRef const& p,
         calo_tally_kernel<<<num_blocks(s.size), block_size, 0, st>>>(p, s, num_det, ts);
     return hipGetLastError();
 }
+
+#endif
 
 //---------------------------------------------------------------------------//
 }  // namespace celeritas_amd

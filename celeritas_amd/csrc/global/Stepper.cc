@@ -19,7 +19,9 @@ hipError_t launch_kill_active(CoreParamsRef const&, CoreStateRef const&, hipStre
 
 //---------------------------------------------------------------------------//
 Stepper::Stepper(StepperInput input)
-    : params_(std::move(input.params)), action_times_(input.action_times)
+    : params_(std::move(input.params))
+    , action_times_(input.action_times)
+    , use_graph_(input.use_graph)
 {
     CELER_VALIDATE(params_, << "stepper requires params");
     CELER_VALIDATE(input.num_track_slots > 0, << "stepper requires track slots");
@@ -55,6 +57,35 @@ Stepper::~Stepper()
 {
     for (auto e : events_)
         (void)hipEventDestroy(e);
+    if (graph_exec_)
+        (void)hipGraphExecDestroy(graph_exec_);
+}
+
+//---------------------------------------------------------------------------//
+/*!
+ * Capture one step of the action sequence into a graph.
+ *
+ * Every action enqueues only kernels whose arguments are fixed for the
+ * state (device pointers, action IDs, device-resident launch ranges and
+ * counters), and the primaries action launches nothing when none are
+ * pending, so the same graph is valid for every later step without new
+ * primaries. The per-step counter reset and readback stay outside it.
+ */
+void Stepper::capture_step()
+{
+    CELER_VALIDATE(!action_times_, << "per-action timing cannot be captured in a graph");
+    if (graph_exec_)
+        return;
+    hipStream_t const stream = state_->stream();
+    hipGraph_t graph = nullptr;
+    CELER_HIP_CALL(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    for (auto const& a : actions_)
+        a->step(*params_, *state_);
+    CELER_HIP_CALL(hipStreamEndCapture(stream, &graph));
+    hipError_t const err = hipGraphInstantiate(&graph_exec_, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    CELER_HIP_CALL(err);
+    use_graph_ = true;
 }
 
 void Stepper::reset_action_times()
@@ -124,10 +155,24 @@ StepperResult Stepper::make_result()
 
 StepperResult Stepper::operator()()
 {
+    return this->step_impl(false);
+}
+
+StepperResult Stepper::step_impl(bool new_primaries)
+{
     // Generated count is per step (Stepper.cc:125-139)
     CELER_HIP_CALL(hipMemsetAsync(&state_->ref().counters->num_generated, 0,
                                   sizeof(std::uint32_t), state_->stream()));
-    this->enqueue_step();
+    if (use_graph_ && !new_primaries && !action_times_ && !state_->warming_up())
+    {
+        if (!graph_exec_)
+            this->capture_step();
+        CELER_HIP_CALL(hipGraphLaunch(graph_exec_, state_->stream()));
+    }
+    else
+    {
+        this->enqueue_step();
+    }
     return this->make_result();
 }
 
@@ -141,7 +186,7 @@ StepperResult Stepper::operator()(Primary const* primaries, size_type count)
                    << "event number " << max_event
                    << " exceeds max_events=" << params_->max_events());
     primaries_action_->insert(*params_, *state_, primaries, count);
-    return (*this)();
+    return this->step_impl(true);
 }
 
 //---------------------------------------------------------------------------//

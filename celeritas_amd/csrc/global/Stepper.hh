@@ -25,6 +25,8 @@ struct StepperInput
     std::uint32_t stream_id{0};
     size_type num_track_slots{0};
     bool action_times{false};
+    //! Replay steps without new primaries from a captured hipGraph
+    bool use_graph{false};
 };
 
 //! Counters after one step iteration (celeritas/global/Stepper.hh:61-70)
@@ -69,6 +71,12 @@ class Stepper
     //! Enqueue one step without reading counters back
     void enqueue_step();
 
+    //! Capture the step sequence (no new primaries) as a hipGraph and
+    //! replay it for later steps without primaries
+    void capture_step();
+    //! Whether steps without primaries replay the captured graph
+    bool uses_graph() const { return graph_exec_ != nullptr; }
+
     std::vector<std::shared_ptr<StepActionInterface const>> const& actions() const
     {
         return actions_;
@@ -92,8 +100,11 @@ class Stepper
     std::vector<std::uint64_t> counts_;
     std::vector<hipEvent_t> events_;
     bool events_pending_{false};
+    bool use_graph_{false};
+    hipGraphExec_t graph_exec_{nullptr};
 
     StepperResult make_result();
+    StepperResult step_impl(bool new_primaries);
 };
 
 //---------------------------------------------------------------------------//

@@ -17,11 +17,14 @@
 
 #include <cmath>
 
+#include "corecel/Precision.hh"
 #include "em/InteractorCommon.hh"
 #include "PhysicsImpl.hh"
 #include "random/XorwowEngine.hh"
 
 namespace celeritas_amd
+{
+inline namespace CELER_PREC_NS
 {
 namespace alongstep
 {
@@ -44,11 +47,11 @@ struct Normal
             has_spare = false;
             return std::fma(spare, stddev, mean);
         }
-        double const theta = (2 * phys::pi) * generate_canonical<double>(rng);
-        double const r = std::sqrt(-2 * std::log(generate_canonical<double>(rng)));
-        spare = r * std::cos(theta);
+        double const theta = (2 * phys::pi) * draw_canonical(rng);
+        double const r = mth::sqrt(-2 * mth::log(draw_canonical(rng)));
+        spare = r * mth::cos(theta);
         has_spare = true;
-        return std::fma(r * std::sin(theta), stddev, mean);
+        return std::fma(r * mth::sin(theta), stddev, mean);
     }
 };
 
@@ -59,15 +62,15 @@ CELER_FUNCTION unsigned int sample_poisson(double lambda, Engine& rng)
     if (lambda <= 16)
     {
         int k = 0;
-        double p = std::exp(lambda);
+        double p = mth::exp(lambda);
         do
         {
             ++k;
-            p *= generate_canonical<double>(rng);
+            p *= draw_canonical(rng);
         } while (p > 1);
         return static_cast<unsigned int>(k - 1);
     }
-    Normal n{lambda, std::sqrt(lambda)};
+    Normal n{lambda, mth::sqrt(lambda)};
     double const x = n(rng) + 0.5;
     return x > 0 ? static_cast<unsigned int>(x) : 0u;
 }
@@ -76,7 +79,7 @@ CELER_FUNCTION unsigned int sample_poisson(double lambda, Engine& rng)
 template<class Engine>
 CELER_FORCEINLINE_FUNCTION double uniform(double a, double b, Engine& rng)
 {
-    return std::fma(b - a, generate_canonical<double>(rng), a);
+    return std::fma(b - a, draw_canonical(rng), a);
 }
 
 //---------------------------------------------------------------------------//
@@ -129,7 +132,7 @@ CELER_FUNCTION double sample_urban_fluct(UrbanFluctuation const& params,
     double xs_exc[2] = {0, 0};
     if (max_energy > mat.mean_exc_energy)
     {
-        double const w = std::log(two_mebsgs) - beta_sq;
+        double const w = mth::log(two_mebsgs) - beta_sq;
         double const w0 = mat.log_mean_exc_energy;
         if (w > w0)
         {
@@ -148,12 +151,12 @@ CELER_FUNCTION double sample_urban_fluct(UrbanFluctuation const& params,
             }
             double scaling = 4;
             if (xs_exc[0] < fluct_exc_thresh)
-                scaling = 0.5 + (scaling - 0.5) * std::sqrt(xs_exc[0] / fluct_exc_thresh);
+                scaling = 0.5 + (scaling - 0.5) * mth::sqrt(xs_exc[0] / fluct_exc_thresh);
             binding[0] *= scaling;
             xs_exc[0] /= scaling;
         }
     }
-    double xs_ion = mean_loss * (max_energy - e0) / (max_energy * e0 * std::log(max_energy / e0));
+    double xs_ion = mean_loss * (max_energy - e0) / (max_energy * e0 * mth::log(max_energy / e0));
     if (xs_exc[0] + xs_exc[1] > 0)
         xs_ion *= fluct_rate;
 
@@ -178,7 +181,7 @@ CELER_FUNCTION double sample_urban_fluct(UrbanFluctuation const& params,
             }
         }
         if (variance > 0)
-            exc += sample_fast_urban(mean, std::sqrt(variance), rng);
+            exc += sample_fast_urban(mean, mth::sqrt(variance), rng);
         result += exc;
     }
     // Ionisation
@@ -191,11 +194,11 @@ CELER_FUNCTION double sample_urban_fluct(UrbanFluctuation const& params,
         {
             alpha = (xs_ion + fluct_max_collisions) * energy_ratio
                     / (fluct_max_collisions * energy_ratio + xs_ion);
-            double const mean_loss_coll = alpha * std::log(alpha) / (alpha - 1);
+            double const mean_loss_coll = alpha * mth::log(alpha) / (alpha - 1);
             mean_num_coll = xs_ion * energy_ratio * (alpha - 1) / ((energy_ratio - 1) * alpha);
             double const mean = mean_num_coll * mean_loss_coll * e0;
             double const stddev
-                = e0 * std::sqrt(xs_ion * (alpha - mean_loss_coll * mean_loss_coll));
+                = e0 * mth::sqrt(xs_ion * (alpha - mean_loss_coll * mean_loss_coll));
             ion += sample_fast_urban(mean, stddev, rng);
         }
         if (xs_ion > 0 && energy_ratio > alpha)
@@ -283,7 +286,7 @@ inline constexpr double msc_min_scaling_energy = 5e-3;  //!< [MeV]
 
 CELER_FORCEINLINE_FUNCTION double fastpow(double a, double b)
 {
-    return std::exp(b * std::log(a));
+    return mth::exp(b * mth::log(a));
 }
 
 //! Per-track inputs of the MSC helpers (UrbanMscHelper)
@@ -334,7 +337,7 @@ CELER_FUNCTION double to_geo(MscTrack const& t, double tstep, double* alpha_out)
     }
     else if (tstep < t.range * msc_dtrl)
     {
-        step = -t.lambda * std::expm1(-tstep / t.lambda);
+        step = -t.lambda * mth::expm1(-tstep / t.lambda);
     }
     else
     {
@@ -369,7 +372,7 @@ from_geo(MscTrack const& t, double true_step, double alpha, double gstep)
     double tstep;
     if (alpha == 0)
     {
-        tstep = -t.lambda * std::log1p(-gstep / t.lambda);
+        tstep = -t.lambda * mth::log1p(-gstep / t.lambda);
         if (tstep < msc_min_step)
             tstep = gstep;
     }
@@ -457,7 +460,7 @@ CELER_FUNCTION double safety_step_limit(MscTrack const& t,
 CELER_FORCEINLINE_FUNCTION double calc_displacement(double geom_path, double true_path)
 {
     double const rmax2 = (true_path - geom_path) * (true_path + geom_path);
-    return 0.73 * std::sqrt(rmax2);
+    return 0.73 * mth::sqrt(rmax2);
 }
 
 //! Positron correction to the Highland width (UrbanPositronCorrector)
@@ -470,20 +473,20 @@ CELER_FUNCTION double positron_correction(double zeff, double y)
     double const d = 1.21e-3 * zeff;
     double const mult = std::fma(zeff, std::fma(zeff, 1.84035e-4, -1.86427e-2), 1.41125);
     double const xl = 0.6, xh = 0.9, e = 113;
-    double const x = std::sqrt(y * (y + 2) / ((y + 1) * (y + 1)));
+    double const x = mth::sqrt(y * (y + 2) / ((y + 1) * (y + 1)));
     double corr;
     if (x < xl)
     {
-        corr = a * (1 - std::exp(-b * x));
+        corr = a * (1 - mth::exp(-b * x));
     }
     else if (x > xh)
     {
-        corr = c + d * std::exp(e * (x - 1));
+        corr = c + d * mth::exp(e * (x - 1));
     }
     else
     {
-        double const yl = a * (1 - std::exp(-b * xl));
-        double const yh = c + d * std::exp(e * (xh - 1));
+        double const yl = a * (1 - mth::exp(-b * xl));
+        double const yh = c + d * mth::exp(e * (xh - 1));
         double const y0 = (yh - yl) / (xh - xl);
         double const y1 = yl - y0 * xl;
         corr = y0 * x + y1;
@@ -545,12 +548,12 @@ CELER_FUNCTION MscResult msc_scatter(MscTrack const& t,
         double const lambda_end = t.calc_msc_mfp(end_energy);
         double const denom = std::fabs(lambda - lambda_end) < lambda * 0.01
                                  ? lambda
-                                 : (lambda - lambda_end) / std::log(lambda / lambda_end);
+                                 : (lambda - lambda_end) / mth::log(lambda / lambda_end);
         tau = true_path / denom;
         if (tau < msc_tau_big)
         {
-            xmean = std::exp(-tau);
-            x2mean = (1 + 2 * std::exp(-2.5 * tau)) / 3;
+            xmean = mth::exp(-tau);
+            x2mean = (1 + 2 * mth::exp(-2.5 * tau)) / 3;
             if (CELER_UNLIKELY(limit_min == 0))
                 limit_min = msc_limit_min;
             limit_min = limit_min < p.msc.lambda_limit ? limit_min : p.msc.lambda_limit;
@@ -560,15 +563,15 @@ CELER_FUNCTION MscResult msc_scatter(MscTrack const& t,
             double y = tp / material.rad_length;
             if (is_positron)
                 y *= positron_correction(material.zeff,
-                                         std::sqrt(inc_energy * end_energy) / mass);
+                                         mth::sqrt(inc_energy * end_energy) / mass);
             double const invbetacp
-                = std::sqrt((inc_energy + mass) * (end_energy + mass)
+                = mth::sqrt((inc_energy + mass) * (end_energy + mass)
                             / (inc_energy * (inc_energy + 2 * mass) * end_energy
                                * (end_energy + 2 * mass)));
-            theta0 = 13.6 * std::sqrt(y) * invbetacp;
-            theta0 *= std::fma(std::log(y), msc.theta_coeff[1], msc.theta_coeff[0]);
+            theta0 = 13.6 * mth::sqrt(y) * invbetacp;
+            theta0 *= std::fma(mth::log(y), msc.theta_coeff[1], msc.theta_coeff[0]);
             if (true_path < limit_min)
-                theta0 *= std::sqrt(true_path / limit_min);
+                theta0 *= mth::sqrt(true_path / limit_min);
             theta0 = theta0 > 0 ? theta0 : 0;
             if (theta0 < 1e-8)
             {
@@ -589,8 +592,8 @@ CELER_FUNCTION MscResult msc_scatter(MscTrack const& t,
         double r;
         do
         {
-            double const rdm = generate_canonical<double>(rng);
-            bool const use_pow = generate_canonical<double>(rng) < p_pow;
+            double const rdm = draw_canonical(rng);
+            bool const use_pow = draw_canonical(rng) < p_pow;
             r = 2 * (use_pow ? fastpow(rdm, 1 / (a + 1)) : rdm) - 1;
         } while (std::fabs(r) > 1);
         return r;
@@ -616,10 +619,10 @@ CELER_FUNCTION MscResult msc_scatter(MscTrack const& t,
         double const radlen_mfp = true_path / (tau * material.rad_length);
         double xsi = std::fma(u, std::fma(u, msc.tail_coeff[2], msc.tail_coeff[1]),
                               msc.tail_coeff[0])
-                     + msc.tail_corr * std::log(radlen_mfp);
+                     + msc.tail_corr * mth::log(radlen_mfp);
         xsi = xsi > 1.9 ? xsi : 1.9;
-        double const ea = std::exp(-xsi);
-        double const sh = 2 * std::sin(0.5 * theta0);
+        double const ea = mth::exp(-xsi);
+        double const sh = 2 * mth::sin(0.5 * theta0);
         double const x = sh * sh;
         double const xmean_1 = 1 - x * (1 + (xsi * ea) / (1 - ea));
         if (xmean_1 <= 0.999 * xmean)
@@ -640,17 +643,17 @@ CELER_FUNCTION MscResult msc_scatter(MscTrack const& t,
             double const f2x0 = (c - 1) / (c * (1 - d));
             double const prob = f2x0 / (ea / (1 - ea) + f2x0);
             double const qprob = xmean / (prob * xmean_1 + (1 - prob) * xmean_2);
-            if (generate_canonical<double>(rng) >= qprob)
+            if (draw_canonical(rng) >= qprob)
             {
                 costheta = uniform(-1.0, 1.0, rng);
             }
-            else if (generate_canonical<double>(rng) < prob)
+            else if (draw_canonical(rng) < prob)
             {
-                costheta = 1 + std::log(uniform(ea, 1.0, rng)) * x;
+                costheta = 1 + mth::log(uniform(ea, 1.0, rng)) * x;
             }
             else
             {
-                double var = (1 - d) * generate_canonical<double>(rng);
+                double var = (1 - d) * draw_canonical(rng);
                 if (var < 0.01 * d)
                 {
                     var /= (d * (c - 1));
@@ -676,9 +679,9 @@ CELER_FUNCTION MscResult msc_scatter(MscTrack const& t,
             // sample_displacement_dir
             double const cbeta = 2.160;
             double const cbeta1 = 0.9988703417569197;
-            double const psi = -std::log(1 - generate_canonical<double>(rng) * cbeta1) / cbeta;
-            double const dphi = generate_canonical<double>(rng) < 0.5 ? phi + psi : phi - psi;
-            double const local[3] = {std::cos(dphi), std::sin(dphi), 0};
+            double const psi = -mth::log(1 - draw_canonical(rng) * cbeta1) / cbeta;
+            double const dphi = draw_canonical(rng) < 0.5 ? phi + psi : phi - psi;
+            double const local[3] = {mth::cos(dphi), mth::sin(dphi), 0};
             rotate(local, inc_dir, result.displacement);
             for (int i = 0; i < 3; ++i)
                 result.displacement[i] *= length;
@@ -693,4 +696,5 @@ CELER_FUNCTION MscResult msc_scatter(MscTrack const& t,
 
 //---------------------------------------------------------------------------//
 }  // namespace alongstep
+}  // namespace CELER_PREC_NS
 }  // namespace celeritas_amd

@@ -14,9 +14,12 @@
 
 #include <cmath>
 
+#include "corecel/Precision.hh"
 #include "PhysicsData.hh"
 
 namespace celeritas_amd
+{
+inline namespace CELER_PREC_NS
 {
 namespace phys
 {
@@ -103,4 +106,5 @@ livermore_pe_macro_xs(PhysicsParamsRef const& p, std::uint32_t mat, double energ
 
 //---------------------------------------------------------------------------//
 }  // namespace phys
+}  // namespace CELER_PREC_NS
 }  // namespace celeritas_amd

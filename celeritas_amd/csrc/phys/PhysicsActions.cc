@@ -14,26 +14,26 @@ namespace celeritas_amd
 //---------------------------------------------------------------------------//
 void PreStepAction::step(CoreParams const& params, CoreState& state) const
 {
-    CELER_HIP_CALL(launch_pre_step(params.device_ref(), state.ref(),
-                                   make_thread_slots(params, state, *this),
-                                   state.stream()));
+    bool const mixed = params.precision() == PrecisionMode::mixed;
+    CELER_HIP_CALL((mixed ? launch_pre_step_mixed : launch_pre_step)(
+        params.device_ref(), state.ref(), make_thread_slots(params, state, *this),
+        state.stream()));
 }
 
 void DiscreteSelectAction::step(CoreParams const& params, CoreState& state) const
 {
-    CELER_HIP_CALL(launch_discrete_select(params.device_ref(), state.ref(),
-                                          make_thread_slots(params, state, *this),
-                                          state.stream()));
+    bool const mixed = params.precision() == PrecisionMode::mixed;
+    CELER_HIP_CALL((mixed ? launch_discrete_select_mixed : launch_discrete_select)(
+        params.device_ref(), state.ref(), make_thread_slots(params, state, *this),
+        state.stream()));
 }
 
 void ModelAction::step(CoreParams const& params, CoreState& state) const
 {
-    CELER_HIP_CALL(launch_interaction(params.device_ref(),
-                                      state.ref(),
-                                      static_cast<std::uint8_t>(this->action_id()),
-                                      model_class_,
-                                      make_thread_slots(params, state, *this),
-                                      state.stream()));
+    bool const mixed = params.precision() == PrecisionMode::mixed;
+    CELER_HIP_CALL((mixed ? launch_interaction_mixed : launch_interaction)(
+        params.device_ref(), state.ref(), static_cast<std::uint8_t>(this->action_id()),
+        model_class_, make_thread_slots(params, state, *this), state.stream()));
 }
 
 //---------------------------------------------------------------------------//

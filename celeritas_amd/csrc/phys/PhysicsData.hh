@@ -67,6 +67,10 @@ struct ModelRange
     double energy_low;  //!< [MeV]
     std::uint32_t model;  //!< ModelClass
     std::uint32_t action;  //!< step action ID of the model
+    //! Offset into elem_cdf_index (one entry per material: first XsGrid of
+    //! the material's per-element CDFs, or invalid), or invalid
+    std::uint32_t elem_cdf;
+    std::uint32_t pad_;
 };
 
 //! One particle-process: tables indexed by material
@@ -77,7 +81,6 @@ struct ProcessRecord
     std::uint32_t integral;  //!< 1 if the integral approach is used
     std::uint32_t energy_max_xs;  //!< reals offset (+material)
     std::uint32_t num_models;
-    std::uint32_t elem_cdf;  //!< reals offset of element CDF grids, or invalid
     ModelRange models[max_models];
 };
 
@@ -263,6 +266,8 @@ struct PhysicsParamsRef
     // Materials
     MaterialRecord const* materials{nullptr};
     ElementComponent const* components{nullptr};
+    //! Per model and material: first element-CDF grid (ModelRange::elem_cdf)
+    std::uint32_t const* elem_cdf_index{nullptr};
     ElementRecord const* elements{nullptr};
     std::uint32_t const* sb_argmax{nullptr};  //!< per-x argmax of SB tables
     LivermoreElement const* pe_elements{nullptr};  //!< per element

@@ -8,10 +8,13 @@
 
 #include <cmath>
 
+#include "corecel/Precision.hh"
 #include "global/CoreData.hh"
 #include "LivermoreXs.hh"
 
 namespace celeritas_amd
+{
+inline namespace CELER_PREC_NS
 {
 namespace phys
 {
@@ -30,7 +33,7 @@ CELER_FUNCTION double
 calc_grid(XsGrid const& g, double const* __restrict__ reals, double energy)
 {
     double const* v = reals + g.offset;
-    double const loge = std::log(energy);
+    double const loge = mth::log(energy);
     if (!(loge > g.loge_front))  // also catches NaN: never index with it
     {
         double r = v[0];
@@ -43,11 +46,11 @@ calc_grid(XsGrid const& g, double const* __restrict__ reals, double energy)
         return (i >= g.prime_index) ? r / energy : r;
     }
     auto const lower = static_cast<std::uint32_t>((loge - g.loge_front) / g.loge_delta);
-    double const upper_e = std::exp(g.loge_front + g.loge_delta * (lower + 1));
+    double const upper_e = mth::exp(g.loge_front + g.loge_delta * (lower + 1));
     double upper_xs = v[lower + 1];
     if (lower + 1 == g.prime_index)
         upper_xs /= upper_e;
-    double const lower_e = std::exp(g.loge_front + g.loge_delta * lower);
+    double const lower_e = mth::exp(g.loge_front + g.loge_delta * lower);
     double const lower_xs = v[lower];
     double const slope = ((-lower_xs) + upper_xs) / ((-lower_e) + upper_e);
     double result = std::fma(slope, (-lower_e) + energy, lower_xs);
@@ -64,11 +67,11 @@ CELER_FUNCTION double
 eplusgg_macro_xs(double electron_mass, double electron_density, double energy)
 {
     double const gamma = (energy > 1e-6 ? energy : 1e-6) / electron_mass;
-    double const sqrt_gg2 = std::sqrt(gamma * (gamma + 2));
+    double const sqrt_gg2 = mth::sqrt(gamma * (gamma + 2));
     double const gp1 = gamma + 1;
     double const poly = 1 + gp1 * (4 + gp1 * 1);
     return pi * r_electron * r_electron * electron_density
-           * (poly * std::log(gamma + 1 + sqrt_gg2) - (gamma + 4) * sqrt_gg2)
+           * (poly * mth::log(gamma + 1 + sqrt_gg2) - (gamma + 4) * sqrt_gg2)
            / (gamma * (gamma + 2) * (gamma + 2));
 }
 
@@ -113,14 +116,14 @@ CELER_FUNCTION double
 calc_range(XsGrid const& g, double const* __restrict__ reals, double energy)
 {
     double const* v = reals + g.offset;
-    double const loge = std::log(energy);
+    double const loge = mth::log(energy);
     if (!(loge > g.loge_front))  // also catches NaN: never index with it
-        return v[0] * std::exp(0.5 * (loge - g.loge_front));
+        return v[0] * mth::exp(0.5 * (loge - g.loge_front));
     if (loge >= g.loge_back)
         return v[g.size - 1];
     auto const i = static_cast<std::uint32_t>((loge - g.loge_front) / g.loge_delta);
-    double const xl = std::exp(g.loge_front + g.loge_delta * i);
-    double const xr = std::exp(g.loge_front + g.loge_delta * (i + 1));
+    double const xl = mth::exp(g.loge_front + g.loge_delta * i);
+    double const xr = mth::exp(g.loge_front + g.loge_delta * (i + 1));
     double const slope = ((-v[i]) + v[i + 1]) / ((-xl) + xr);
     return std::fma(slope, (-xl) + energy, v[i]);
 }
@@ -137,10 +140,10 @@ calc_inverse_range(XsGrid const& g, double const* __restrict__ reals, double ran
     if (!(range >= r[0]))  // also catches NaN: never index with it
     {
         double const f = range / r[0];
-        return std::exp(g.loge_front) * (f * f);
+        return mth::exp(g.loge_front) * (f * f);
     }
     if (range >= r[n - 1])
-        return std::exp(g.loge_back);
+        return mth::exp(g.loge_back);
     std::uint32_t lo = 0;
     std::uint32_t len = n;
     while (len > 0)
@@ -159,8 +162,8 @@ calc_inverse_range(XsGrid const& g, double const* __restrict__ reals, double ran
     std::uint32_t i = lo;
     if (r[i] != range)
         --i;
-    double const el = std::exp(g.loge_front + g.loge_delta * i);
-    double const er = std::exp(g.loge_front + g.loge_delta * (i + 1));
+    double const el = mth::exp(g.loge_front + g.loge_delta * i);
+    double const er = mth::exp(g.loge_front + g.loge_delta * (i + 1));
     double const slope = ((-el) + er) / ((-r[i]) + r[i + 1]);
     return std::fma(slope, (-r[i]) + range, el);
 }
@@ -204,9 +207,10 @@ CELER_FUNCTION double speed(double energy, double mass)
 {
     double const inv_gamma = mass / (energy + mass);
     double const beta_sq = 1 - inv_gamma * inv_gamma;
-    return std::sqrt(beta_sq) * c_light;
+    return mth::sqrt(beta_sq) * c_light;
 }
 
 //---------------------------------------------------------------------------//
 }  // namespace phys
+}  // namespace CELER_PREC_NS
 }  // namespace celeritas_amd

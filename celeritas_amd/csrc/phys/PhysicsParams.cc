@@ -246,6 +246,8 @@ struct PhysicsParams::HostData
     DeviceVector<ElementRecord> d_elements;
     std::vector<std::uint32_t> sb_argmax;
     DeviceVector<std::uint32_t> d_sb_argmax;
+    std::vector<std::uint32_t> elem_cdf_index;
+    DeviceVector<std::uint32_t> d_elem_cdf_index;
     std::vector<LivermoreElement> pe_elements;
     std::vector<LivermoreShell> pe_shells;
     std::vector<UrbanFluctuation> fluct;
@@ -552,7 +554,6 @@ PhysicsParams::PhysicsParams(JsonValue const& inp, ActionRegistry* reg)
         rec.process_class = static_cast<std::uint32_t>(
             process_from_string(pj->get_str("class", "")));
         rec.integral = pj->get_bool("integral", false) ? 1 : 0;
-        rec.elem_cdf = invalid_id;
         if (rec.process_class == static_cast<std::uint32_t>(ProcessClass::annihilation))
         {
             // e+ annihilation applies at rest (EPlusAnnihilationProcess)
@@ -618,6 +619,49 @@ PhysicsParams::PhysicsParams(JsonValue const& inp, ActionRegistry* reg)
             }
             rec.models[k].energy_low = mj.get("low", 0.0);
             rec.models[k].model = static_cast<std::uint32_t>(mc);
+            rec.models[k].elem_cdf = invalid_id;
+            if (mc == ModelClass::combined_brem)
+            {
+                // PhysicsParams.cc:662-670: no element selection for it
+                for (std::size_t m = 0; m < num_materials_; ++m)
+                    CELER_VALIDATE(h.materials[m].num_elements <= 1,
+                                   << "model '" << mlabel
+                                   << "' cannot be used with materials of more than one "
+                                      "element");
+            }
+            if (mj.has("element_cdf"))
+            {
+                // Per material: one CDF grid per element for compounds
+                auto const& cj = mj.at("element_cdf");
+                CELER_VALIDATE(cj.size() == num_materials_,
+                               << "element_cdf of '" << mlabel << "' needs one entry per material");
+                rec.models[k].elem_cdf = static_cast<std::uint32_t>(h.elem_cdf_index.size());
+                for (std::size_t m = 0; m < num_materials_; ++m)
+                {
+                    std::uint32_t first = invalid_id;
+                    auto const& mjm = cj[m];
+                    if (mjm.type == JsonValue::Type::object)
+                    {
+                        auto const& rows = mjm.at("cdf");
+                        CELER_VALIDATE(rows.size() == h.materials[m].num_elements,
+                                       << "element_cdf needs one grid per element");
+                        std::vector<GridHost> gs;
+                        for (auto const& row : rows.arr)
+                        {
+                            GridHost gh;
+                            gh.values = row.as_doubles();
+                            CELER_VALIDATE(gh.values.size() >= 2, << "short CDF grid");
+                            gh.front = std::log(mjm.at("emin").as_double());
+                            gh.back = std::log(mjm.at("emax").as_double());
+                            gh.delta = (gh.back - gh.front)
+                                       / static_cast<double>(gh.values.size() - 1);
+                            gs.push_back(std::move(gh));
+                        }
+                        first = h.add_grids(gs);
+                    }
+                    h.elem_cdf_index.push_back(first);
+                }
+            }
             // One model (and action) may serve several particles, as the
             // reference's MollerBhabha and brems models serve e- and e+
             auto found = model_actions.find(mlabel);
@@ -752,6 +796,8 @@ PhysicsParams::PhysicsParams(JsonValue const& inp, ActionRegistry* reg)
     h.d_elements = to_device(h.elements);
     h.d_sb_argmax = to_device(h.sb_argmax);
     ref_.sb_argmax = h.d_sb_argmax.data();
+    h.d_elem_cdf_index = to_device(h.elem_cdf_index);
+    ref_.elem_cdf_index = h.d_elem_cdf_index.data();
     h.d_pe_elements = to_device(h.pe_elements);
     h.d_pe_shells = to_device(h.pe_shells);
     h.d_fluct = to_device(h.fluct);

@@ -755,6 +755,58 @@ JsonValue generate_physics(JsonValue const& spec)
             s += c.second * micro(elements[c.first]);
         return s * m.number_density;
     };
+    bool const any_compound = std::any_of(materials.begin(), materials.end(),
+                                          [](Material const& m) { return m.comps.size() > 1; });
+    /*
+     * Element-selection CDFs of a model (PhysicsParams::build_model_xs,
+     * PhysicsParams.cc:638-765): for each material with several elements,
+     * per element i the cumulative sum over j <= i of fraction_j * sigma_j(E)
+     * normalised by the total, on the grid of the material's process table
+     * (threshold(mi) is its lowest energy). Single-element materials get
+     * null.
+     */
+    auto element_cdf = [&](auto&& threshold, auto&& micro) {
+        JsonValue arr = JsonValue::array();
+        for (std::size_t mi = 0; mi < nmat; ++mi)
+        {
+            Material const& m = materials[mi];
+            if (m.comps.size() <= 1)
+            {
+                arr.arr.push_back(JsonValue());
+                continue;
+            }
+            double const e0 = std::max(threshold(mi), g.emin);
+            int const n = std::max(
+                2, static_cast<int>(std::ceil(bpd * std::log10(g.emax / e0) - 1e-9)) + 1);
+            LogGrid const gm{e0, g.emax, n};
+            std::vector<std::vector<double>> cdf(m.comps.size(), std::vector<double>(n));
+            for (int i = 0; i < n; ++i)
+            {
+                double const e = gm.energy(i);
+                double cum = 0;
+                for (std::size_t c = 0; c < m.comps.size(); ++c)
+                {
+                    cum += m.comps[c].second * micro(elements[m.comps[c].first], m, mi, e);
+                    cdf[c][i] = cum;
+                }
+                if (cum > 0)
+                {
+                    for (auto& v : cdf)
+                        v[i] /= cum;
+                }
+            }
+            JsonValue jm = JsonValue::object();
+            jm.obj["emin"] = JsonValue::number(gm.emin);
+            jm.obj["emax"] = JsonValue::number(gm.emax);
+            JsonValue jc = JsonValue::array();
+            for (auto const& v : cdf)
+                jc.arr.push_back(JsonValue::array(v));
+            jm.obj["cdf"] = jc;
+            arr.arr.push_back(jm);
+        }
+        return arr;
+    };
+    auto no_threshold = [](std::size_t) { return 0.0; };
 
     JsonValue processes = JsonValue::array();
     auto add_process = [&](char const* label, char const* cls, char const* particle,
@@ -778,11 +830,18 @@ JsonValue generate_physics(JsonValue const& spec)
     });
     add_process("phot", "photoelectric", "gamma", false,
                 model_json("photoel-livermore", "livermore_pe"));
-    add_process("conv", "conversion", "gamma", false,
-                model_json("conv-bethe-heitler", "bethe_heitler"))
-        ->obj["xs"] = per_material([&](Material const& m, std::size_t, double e) {
-        return atom_sum(m, [e](Element const& el) { return pair_xs(el, e); });
-    });
+    {
+        JsonValue models = model_json("conv-bethe-heitler", "bethe_heitler");
+        models.arr[0].obj["element_cdf"] = element_cdf(
+            no_threshold,
+            [](Element const& el, Material const&, std::size_t, double e) {
+                return pair_xs(el, e);
+            });
+        add_process("conv", "conversion", "gamma", false, std::move(models))
+            ->obj["xs"] = per_material([&](Material const& m, std::size_t, double e) {
+            return atom_sum(m, [e](Element const& el) { return pair_xs(el, e); });
+        });
+    }
     // Positron annihilation (cross section computed on the fly)
     add_process("annihil", "annihilation", "positron", true,
                 model_json("annihil-2-gamma", "eplusgg"));
@@ -837,8 +896,41 @@ JsonValue generate_physics(JsonValue const& spec)
         ioni->obj["dedx_grid"] = dg;
         ioni->obj["range_grid"] = rg;
 
+        // One combined model for single-element materials; with compounds,
+        // Seltzer-Berger below 1 GeV and relativistic above as two models
+        // with element selection (the combined model rejects compounds,
+        // PhysicsParams.cc:662-670)
+        auto brems_micro = [&](Element const& el, Material const& m, std::size_t mi,
+                               double e) {
+            double const kc = cut_gamma[mi];
+            if (e <= kc)
+                return 0.0;
+            return integrate_log([&](double k) { return brems_k_dsigma(el, m, e, k) / k; },
+                                 kc, e, 8);
+        };
+        JsonValue brem_models;
+        if (any_compound)
+        {
+            JsonValue const cdf = element_cdf(
+                [&](std::size_t mi) { return cut_gamma[mi]; }, brems_micro);
+            brem_models = JsonValue::array();
+            for (int k = 0; k < 2; ++k)
+            {
+                JsonValue mj = JsonValue::object();
+                mj.obj["label"] = JsonValue::string(k == 0 ? "brems-sb" : "brems-rel");
+                mj.obj["model"] = JsonValue::string(k == 0 ? "seltzer_berger"
+                                                           : "relativistic_brem");
+                mj.obj["low"] = JsonValue::number(k == 0 ? 0.0 : 1e3);
+                mj.obj["element_cdf"] = cdf;
+                brem_models.arr.push_back(std::move(mj));
+            }
+        }
+        else
+        {
+            brem_models = model_json("brems-combined", "combined_brem");
+        }
         JsonValue* brems = add_process(electron ? "eBrem" : "eBrem+", "e_brems", pname, true,
-                                       model_json("brems-combined", "combined_brem"));
+                                       std::move(brem_models));
         brems->obj["xs"] = per_material_from(
             [&](std::size_t mi) { return cut_gamma[mi]; },
             [&](Material const& m, std::size_t mi, double e) {

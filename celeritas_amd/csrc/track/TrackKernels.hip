@@ -69,6 +69,7 @@ __global__ void __launch_bounds__(block_size)
     {
         ti.track_id = ranks ? state.init.track_counters[p.event_id] + ranks[tid]
                             : atomicAdd(&state.init.track_counters[p.event_id], 1u);
+        atomicAdd(&state.init.event_live[p.event_id], 1);
     }
     else
     {
@@ -385,6 +386,9 @@ __global__ void __launch_bounds__(block_size)
     bool initialized = false;
     size_type const start = state.phys.sec_start[slot];
     size_type const count = state.phys.sec_count[slot];
+    // Event bookkeeping: +1 per secondary made, -1 for a dead parent
+    std::uint32_t const event = state.sim.event_id[slot];
+    int live_delta = status == TrackStatus::alive ? 0 : -1;
     if (count > 0)
     {
         std::uint32_t const parent_id = state.sim.track_id[slot];
@@ -398,6 +402,7 @@ __global__ void __launch_bounds__(block_size)
             Secondary const sec = state.secondaries.storage[start + i];
             if (sec.particle_id == invalid_id)
                 continue;
+            ++live_delta;
             TrackInitializer ti;
             ti.track_id = atomicAdd(&state.init.track_counters[event_id], 1u);
             ti.parent_id = parent_id;
@@ -433,6 +438,14 @@ __global__ void __launch_bounds__(block_size)
                 ++next_idx;
             }
         }
+    }
+    if (event < state.init.max_events)
+    {
+        if (live_delta != 0)
+            atomicAdd(&state.init.event_live[event], live_delta);
+        double const edep = state.phys.energy_deposition[slot];
+        if (edep != 0)
+            atomicAdd(&state.init.event_edep[event], edep);
     }
     if (!initialized && status == TrackStatus::killed)
     {

@@ -65,7 +65,8 @@ def testem3_physics_spec() -> dict:
 
 def testem3_problem(physics: dict | None = None, seed: int = 20220511,
                     along_step: str = "general_linear", init_capacity: int = 1 << 20,
-                    max_events: int = 1 << 16, track_order: str = "none") -> dict:
+                    max_events: int = 1 << 16, track_order: str = "none",
+                    precision: str = "fp64") -> dict:
     """TestEm3 problem; pass expanded ``physics`` to share tables."""
     geo = load_orange("testem3-flat.org.json")
     names = geo["universes"][0]["cell_names"]
@@ -90,6 +91,7 @@ def testem3_problem(physics: dict | None = None, seed: int = 20220511,
         "physics": physics if physics is not None else {"generate": testem3_physics_spec()},
         "along_step": along_step,
         "seed": seed,
+        "precision": precision,
         "track_init": {"capacity": init_capacity, "max_events": max_events,
                        "track_order": track_order},
     }
@@ -117,3 +119,253 @@ def testem3_primaries(n: int, seed: int = 12345, emin: float = 1e3, emax: float 
     out["dir"][:, 2] = st * np.sin(phi)
     out["pos"][:, 0] = -22.0
     return out
+
+
+#---------------------------------------------------------------------------#
+# CYLINDRICAL DETECTOR IN A SOLENOID FIELD
+#---------------------------------------------------------------------------#
+#: Barrel layout along z (lengths in cm): silicon tracker of 10 layers of
+#: 0.3 mm, PbWO4 electromagnetic calorimeter of 23 cm, hadronic calorimeter of
+#: 20 layers of 5 cm brass + 4 mm plastic scintillator, |z| < 300 cm, in a
+#: 800 x 800 x 1000 cm vacuum world. Gaps are vacuum.
+CYL_HALF_LENGTH = 300.0
+CYL_PIPE_RADIUS = 3.0
+CYL_TRACKER = [(4.0 + 10.0 * i, 4.03 + 10.0 * i) for i in range(10)]
+CYL_ECAL = (129.0, 152.0)
+CYL_HCAL = [(180.0 + 5.4 * j, 185.0 + 5.4 * j, 185.4 + 5.4 * j) for j in range(20)]
+CYL_WORLD_HALF = (400.0, 400.0, 500.0)
+SI, PBWO4, BRASS, SCINT, VACUUM = range(5)
+
+
+def cylinder_physics_spec() -> dict:
+    """Elements, materials and production cuts of the cylindrical detector.
+
+    Shell binding energies are published values; the photoelectric shell
+    weights and normalisation points are a synthetic model (as for the
+    TestEm3 elements). Cuts approximate Geant4's energies for a 0.7 mm range
+    cut in each material.
+    """
+    el = [
+        {"z": 1, "atomic_mass": 1.00794, "mean_exc_energy": 19.2e-6,
+         "shells": [[13.6e-6, 1.0]], "pe_reference": [0.01, 1.0e-2]},
+        {"z": 6, "atomic_mass": 12.011, "mean_exc_energy": 81e-6,
+         "shells": [[288e-6, 0.9], [11.3e-6, 0.1]], "pe_reference": [0.01, 39.3]},
+        {"z": 8, "atomic_mass": 15.999, "mean_exc_energy": 95e-6,
+         "shells": [[538e-6, 0.9], [17e-6, 0.1]], "pe_reference": [0.01, 154.0]},
+        {"z": 14, "atomic_mass": 28.0855, "mean_exc_energy": 173e-6,
+         "shells": [[1.839e-3, 0.85], [149.7e-6, 0.06], [99.8e-6, 0.07], [8.2e-6, 0.02]],
+         "pe_reference": [0.01, 1581.0]},
+        {"z": 29, "atomic_mass": 63.546, "mean_exc_energy": 322e-6,
+         "shells": [[8.979e-3, 0.84], [1.096e-3, 0.05], [0.9327e-3, 0.08],
+                    [0.1198e-3, 0.02], [0.0077e-3, 0.01]],
+         "pe_reference": [0.1, 31.7]},
+        {"z": 30, "atomic_mass": 65.38, "mean_exc_energy": 330e-6,
+         "shells": [[9.659e-3, 0.84], [1.196e-3, 0.05], [1.0216e-3, 0.08],
+                    [0.1395e-3, 0.02], [0.0094e-3, 0.01]],
+         "pe_reference": [0.1, 36.9]},
+        {"z": 74, "atomic_mass": 183.84, "mean_exc_energy": 727e-6,
+         "shells": [[69.525e-3, 0.79], [12.100e-3, 0.05], [11.544e-3, 0.04],
+                    [10.207e-3, 0.06], [2.820e-3, 0.045], [0.595e-3, 0.012],
+                    [0.0361e-3, 0.003]],
+         "pe_reference": [0.1, 1310.0]},
+        {"z": 82, "atomic_mass": 207.2, "mean_exc_energy": 823e-6,
+         "shells": [[88.0045e-3, 0.79], [15.861e-3, 0.05], [15.200e-3, 0.04],
+                    [13.0352e-3, 0.06], [3.0e-3, 0.045], [0.5e-3, 0.012], [0.05e-3, 0.003]],
+         "pe_reference": [0.1, 1790.0]},
+    ]
+    H, C, O, Si, Cu, Zn, W, Pb = range(8)
+    materials = [
+        {"label": "Si", "density": 2.33, "elements": [{"element": Si, "fraction": 1.0}]},
+        {"label": "PbWO4", "density": 8.28,
+         "elements": [{"element": Pb, "fraction": 1.0}, {"element": W, "fraction": 1.0},
+                      {"element": O, "fraction": 4.0}]},
+        # 70/30 brass by mass as atom fractions
+        {"label": "brass", "density": 8.53,
+         "elements": [{"element": Cu, "fraction": 0.7 / 63.546},
+                      {"element": Zn, "fraction": 0.3 / 65.38}]},
+        # polyvinyltoluene-like (CH)n scintillator
+        {"label": "scintillator", "density": 1.032,
+         "elements": [{"element": C, "fraction": 1.0}, {"element": H, "fraction": 1.1}]},
+        {"label": "vacuum", "density": 1e-25, "gas": True,
+         "elements": [{"element": H, "fraction": 1.0}]},
+    ]
+    cutoffs = {
+        "gamma": [6.95e-3, 0.0995, 0.0206, 2.6e-3, 990e-6],
+        "electron": [0.548, 0.905, 0.987, 0.358, 990e-6],
+        "positron": [0.526, 0.860, 0.938, 0.350, 990e-6],
+    }
+    return {"elements": el, "materials": materials, "cutoffs": cutoffs,
+            "options": {"bins_per_decade": 7, "emin": 1e-4, "emax": 1e8}}
+
+
+def cylinder_geometry() -> tuple[dict, list[int], list[int]]:
+    """ORANGE-format geometry of the barrel detector with the volume to
+    material and volume to detector maps (one detector per layer)."""
+    surfaces, types, names = [], [], []
+
+    def add(kind, value, name):
+        types.append(kind)
+        surfaces.append(value)
+        names.append(name)
+        return len(types) - 1
+
+    wx, wy, wz = CYL_WORLD_HALF
+    box = [add("px", -wx, "world.mx"), add("px", wx, "world.px"),
+           add("py", -wy, "world.my"), add("py", wy, "world.py"),
+           add("pz", -wz, "world.mz"), add("pz", wz, "world.pz")]
+    zlo = add("pz", -CYL_HALF_LENGTH, "barrel.mz")
+    zhi = add("pz", CYL_HALF_LENGTH, "barrel.pz")
+
+    # Radial shells tiling [0, R]: (r_in, r_out, material, label, is_detector)
+    shells = [(0.0, CYL_PIPE_RADIUS, VACUUM, "beam_pipe", False)]
+    r = CYL_PIPE_RADIUS
+    for i, (a, b) in enumerate(CYL_TRACKER):
+        shells.append((r, a, VACUUM, f"tracker_gap_{i}", False))
+        shells.append((a, b, SI, f"tracker_{i}", True))
+        r = b
+    shells.append((r, CYL_ECAL[0], VACUUM, "ecal_gap", False))
+    shells.append((CYL_ECAL[0], CYL_ECAL[1], PBWO4, "ecal", True))
+    r = CYL_ECAL[1]
+    shells.append((r, CYL_HCAL[0][0], VACUUM, "hcal_gap", False))
+    for j, (a, b, c) in enumerate(CYL_HCAL):
+        shells.append((a, b, BRASS, f"hcal_absorber_{j}", True))
+        shells.append((b, c, SCINT, f"hcal_scint_{j}", True))
+    outer_r = CYL_HCAL[-1][2]
+
+    cyl = {}
+
+    def cyl_surface(radius):
+        key = round(radius, 9)
+        if key not in cyl:
+            cyl[key] = add("czc", radius * radius, f"cyl_{radius:g}")
+        return cyl[key]
+
+    cells = [{"faces": box, "flags": 1, "logic": "0 1 ~ & 2 & 3 ~ & 4 & 5 ~ & ~",
+              "num_intersections": 6, "zorder": 2}]
+    cell_names = ["[EXTERIOR]"]
+    vol_mat, vol_det = [-1], [-1]
+    ndet = 0
+    for (a, b, mat, label, is_det) in shells:
+        outer = cyl_surface(b)
+        if a == 0:
+            faces = sorted([zlo, zhi, outer])
+            idx = {f: k for k, f in enumerate(faces)}
+            logic = f"{idx[zlo]} {idx[zhi]} ~ & {idx[outer]} ~ &"
+            nint = 4
+        else:
+            inner = cyl_surface(a)
+            faces = sorted([zlo, zhi, inner, outer])
+            idx = {f: k for k, f in enumerate(faces)}
+            logic = f"{idx[zlo]} {idx[zhi]} ~ & {idx[inner]} & {idx[outer]} ~ &"
+            nint = 6
+        cells.append({"faces": faces, "logic": logic, "num_intersections": nint,
+                      "zorder": 2})
+        cell_names.append(label)
+        vol_mat.append(mat)
+        if is_det:
+            vol_det.append(ndet)
+            ndet += 1
+        else:
+            vol_det.append(-1)
+    # World: inside the box, outside the barrel envelope
+    env = cyl_surface(outer_r)
+    faces = box + [zlo, zhi, env]
+    cells.append({"faces": faces, "flags": 1,
+                  "logic": "0 1 ~ & 2 & 3 ~ & 4 & 5 ~ & 6 7 ~ & 8 ~ & ~ &",
+                  "num_intersections": 10, "zorder": 2})
+    cell_names.append("world")
+    vol_mat.append(VACUUM)
+    vol_det.append(-1)
+    geo = {
+        "_format": "SCALE ORANGE",
+        "_version": 0,
+        "universes": [{
+            "_type": "simple unit",
+            "bbox": [[-wx, -wy, -wz], [wx, wy, wz]],
+            "cell_names": cell_names,
+            "cells": cells,
+            "md": {"name": "world", "provenance": "celeritas_amd.workloads"},
+            "surface_names": names,
+            "surfaces": {"data": surfaces, "sizes": [1] * len(types), "types": types},
+        }],
+    }
+    return geo, vol_mat, vol_det
+
+
+def cylinder_problem(physics: dict | None = None, seed: int = 20240101,
+                     field_tesla: float = 3.8, init_capacity: int = 1 << 22,
+                     max_events: int = 1 << 16, track_order: str = "none",
+                     precision: str = "fp64") -> dict:
+    """Cylindrical detector in a uniform field along z (Dormand-Prince
+    propagation with the default driver options)."""
+    geo, vol_mat, vol_det = cylinder_geometry()
+    return {
+        "geometry": geo,
+        "volume_to_material": vol_mat,
+        "volume_to_detector": vol_det,
+        "physics": physics if physics is not None else {"generate": cylinder_physics_spec()},
+        "along_step": "uniform_field",
+        "field": {"value": [0.0, 0.0, field_tesla]},
+        "seed": seed,
+        "precision": precision,
+        "track_init": {"capacity": init_capacity, "max_events": max_events,
+                       "track_order": track_order},
+    }
+
+
+def cylinder_primaries(n: int, seed: int = 12345, emin: float = 500.0,
+                       emax: float = 5e4, fractions=(0.4, 0.2, 0.4),
+                       first_event: int = 0, per_event: int = 1) -> np.ndarray:
+    """Isotropic primaries from the origin: e-/e+/gamma in the given
+    fractions, energies log-uniform in [emin, emax] MeV."""
+    from . import PRIMARY_DTYPE
+
+    rng = np.random.default_rng(seed)
+    out = np.zeros(n, dtype=PRIMARY_DTYPE)
+    u = rng.random(n)
+    out["particle_id"] = np.where(u < fractions[0], ELECTRON,
+                                  np.where(u < fractions[0] + fractions[1], POSITRON, GAMMA))
+    out["event_id"] = first_event + np.arange(n) // per_event
+    out["energy"] = np.exp(rng.uniform(math.log(emin), math.log(emax), n))
+    mu = rng.uniform(-1.0, 1.0, n)
+    phi = rng.uniform(0, 2 * math.pi, n)
+    st = np.sqrt(1 - mu * mu)
+    out["dir"][:, 0] = st * np.cos(phi)
+    out["dir"][:, 1] = st * np.sin(phi)
+    out["dir"][:, 2] = mu
+    return out
+
+
+#---------------------------------------------------------------------------#
+# STREAMED EVENTS (serving)
+#---------------------------------------------------------------------------#
+def served_events(n_events: int, seed: int = 2024, nmin: int = 10, nmax: int = 2000,
+                  emin: float = 100.0, emax: float = 5e4, fractions=(0.4, 0.2, 0.4),
+                  cone_deg: float = 5.0) -> list[np.ndarray]:
+    """Events for the served TestEm3 workload: per event a log-uniform
+    number of primaries in [nmin, nmax], each with a log-uniform energy in
+    [emin, emax] MeV, e-/e+/gamma in the given fractions, from (-22, 0, 0)
+    cm in a cone about +x."""
+    from . import PRIMARY_DTYPE
+
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(math.log(nmin), math.log(nmax + 1), n_events)).astype(int)
+    counts = np.clip(counts, nmin, nmax)
+    events = []
+    cos_max = math.cos(math.radians(cone_deg))
+    for e, n in enumerate(counts):
+        p = np.zeros(n, dtype=PRIMARY_DTYPE)
+        u = rng.random(n)
+        p["particle_id"] = np.where(u < fractions[0], ELECTRON,
+                                    np.where(u < fractions[0] + fractions[1], POSITRON, GAMMA))
+        p["event_id"] = e
+        p["energy"] = np.exp(rng.uniform(math.log(emin), math.log(emax), n))
+        mu = rng.uniform(cos_max, 1.0, n)
+        phi = rng.uniform(0, 2 * math.pi, n)
+        st = np.sqrt(1 - mu * mu)
+        p["dir"][:, 0] = mu
+        p["dir"][:, 1] = st * np.cos(phi)
+        p["dir"][:, 2] = st * np.sin(phi)
+        p["pos"][:, 0] = -22.0
+        events.append(p)
+    return events

@@ -143,6 +143,7 @@ typedef struct CelerParams CelerParams;
 typedef struct CelerState CelerState;
 typedef struct CelerStepper CelerStepper;
 typedef struct CelerTransporter CelerTransporter;
+typedef struct CelerServer CelerServer;
 
 /*---------------------------------------------------------------------------*
  * LIBRARY
@@ -336,6 +337,49 @@ int celer_hip_reseed(const void* params_ref,
                      const void* state_ref,
                      uint64_t event_id,
                      void* stream);
+
+/*---------------------------------------------------------------------------*
+ * SERVING: continuous batching of events that arrive over time. No
+ * counterpart in the reference, whose offload front end transports one
+ * event per Flush (accel/LocalTransporter.cc:247-300); this one admits
+ * events in arrival order while the initializer queue (max_queued, 0: the
+ * slot count) has room, so events of uneven size overlap on the device,
+ * and detects each event's completion from device-resident per-event
+ * live-track counts. The server owns a stepper and its state.
+ *---------------------------------------------------------------------------*/
+/* Workgroup size of the step kernels (64, 128 or 256; default 256), the
+ * autotuner's launch parameter. Set it before capturing a graph. */
+int celer_state_set_block_size(CelerState* state, uint32_t block_size);
+
+/* Capture the step sequence (steps without new primaries) as a hipGraph;
+ * later celer_stepper_step calls without primaries replay it with one
+ * launch. Not with per-action timing. (The reference has no graph path; the
+ * capture point is the Stepper's action loop, global/Stepper.cc:125-139.) */
+int celer_stepper_capture_step(CelerStepper* stepper);
+
+CelerServer* celer_server_create(CelerParams* params,
+                                 uint32_t num_track_slots,
+                                 uint32_t stream_id);
+void celer_server_destroy(CelerServer* server);
+/* The server's state (for field access); owned by the server */
+CelerState* celer_server_state(CelerServer* server);
+/* Serve num_events events: event e is primaries[event_offsets[e] ..
+ * event_offsets[e+1]) (their event_id is replaced by e) arriving
+ * arrivals[e] seconds after the call starts (non-decreasing). Writes per
+ * event the admission and completion times [s] and the deposited energy
+ * [MeV], and totals = {wall time [s], step iterations, track-steps}.
+ * max_steps > 0 aborts with an error after that many steps. */
+int celer_server_run(CelerServer* server,
+                     const CelerPrimary* primaries,
+                     const uint32_t* event_offsets,
+                     const double* arrivals,
+                     uint32_t num_events,
+                     uint32_t max_queued,
+                     uint32_t max_steps,
+                     double* admit_times,
+                     double* done_times,
+                     double* edep,
+                     double* totals);
 
 /*---------------------------------------------------------------------------*
  * VERIFICATION: run one interactor model (ModelClass) on the device with

@@ -906,16 +906,21 @@ std::uint32_t sample_element(State& s, Track& t, ProcessDef const& p,
     PhysicsData const& pd = *s.problem->physics;
     if (mat >= pd.materials.size() || pd.materials[mat].elements.size() <= 1)
         return 0;
-    // Element selection by the per-element micro XS tables
-    auto const& cdf = p.elem_cdf.at(model_index).at(mat);
+    // Element selection by the per-element normalised cumulative XS
+    // (TabulatedElementSelector.hh:75-95); models without tables select none
+    if (model_index >= p.elem_cdf.size() || p.elem_cdf[model_index].size() <= mat
+        || p.elem_cdf[model_index][mat].empty())
+        return 0;
+    auto const& cdf = p.elem_cdf[model_index][mat];
     XorwowUniform rng{t.rng};
     double u = rng();
-    for (std::size_t i = 0; i + 1 < cdf.size(); ++i)
+    std::size_t i = 0;
+    for (; i + 1 < cdf.size(); ++i)
     {
-        if (u < cdf[i].calc(t.energy))
-            return static_cast<std::uint32_t>(i);
+        if (cdf[i].calc(t.energy) > u)
+            break;
     }
-    return static_cast<std::uint32_t>(cdf.size());
+    return static_cast<std::uint32_t>(i);
 }
 
 //---------------------------------------------------------------------------//

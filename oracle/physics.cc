@@ -378,6 +378,28 @@ void build_physics(Json const& inp, PhysicsData& out)
                 m.low = mj.num("low", 0);
                 m.high = mj.num("high", 0);
                 p.models.push_back(m);
+                // Element-selection CDFs (PhysicsParams::build_model_xs)
+                std::vector<std::vector<XsGridO>> cdf(num_mats);
+                if (Json const* cj = mj.find("element_cdf"))
+                {
+                    for (std::size_t mat = 0; mat < num_mats && mat < cj->a.size(); ++mat)
+                    {
+                        Json const& jm = cj->a[mat];
+                        if (!jm.find("cdf"))
+                            continue;
+                        for (auto const& row : jm("cdf").a)
+                        {
+                            XsGridO g;
+                            g.values = row.nums();
+                            g.size = static_cast<std::uint32_t>(g.values.size());
+                            g.front = std::log(jm.num("emin", 0));
+                            g.back = std::log(jm.num("emax", 0));
+                            g.delta = (g.back - g.front) / (g.size - 1);
+                            cdf[mat].push_back(std::move(g));
+                        }
+                    }
+                }
+                p.elem_cdf.push_back(std::move(cdf));
                 // A model label names one action shared by all its particles
                 if (std::find(out.model_labels.begin(), out.model_labels.end(), m.label)
                     == out.model_labels.end())
