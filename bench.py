@@ -41,14 +41,17 @@ BYTES_PER_UNIT = {
     # span, element, pre-volume, step, actions, per-process and total XS,
     # range written
     "pre-step": (93, 1, "pre_step"),
-    # geometry state (position, direction, volume, surface, next-step
-    # cache) read and written, step, actions, energy, time, range, MFP,
-    # macro XS, deposition, step count
-    "along-step-general-linear": (218, 2, "along_charged"),
-    "along-step-uniform-msc": (226, 2, "along_charged"),
-    "along-step-neutral": (194, 2, "along_neutral"),
-    # geometry state read and written, material, status
-    "geo-boundary": (120, 2, "boundary"),
+    # geometry state (position, direction, volume, surface, sense, boundary
+    # flags, next-step cache: 72 B read, 68 B written), RNG state (24 + 24),
+    # MSC range state (24 + 24), step, actions, energy, time, range, MFP,
+    # macro XS, deposition, step count (184 B read, 161 B written in all)
+    "along-step-general-linear": (345, 2, "along_charged"),
+    # the same plus the looping-step counter
+    "along-step-uniform-msc": (353, 2, "along_charged"),
+    # geometry state, step, actions, time, MFP, macro XS, energy, steps
+    "along-step-neutral": (216, 2, "along_neutral"),
+    # geometry state read and written, status, action
+    "geo-boundary": (142, 2, "boundary"),
 }
 
 
@@ -75,8 +78,9 @@ def parse_args():
     ap.add_argument("--serve-baseline-events", type=int, default=6,
                     help="events of the same stream run one at a time through the plain "
                          "FP64 loop, for the serving speedup")
-    ap.add_argument("--no-mixed", action="store_true",
-                    help="skip the mixed-precision measurement of the same workload")
+    ap.add_argument("--mixed", action="store_true",
+                    help="also measure the same workload in the mixed-precision mode "
+                         "(off by default: validated at small scale only)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel on a buffer of this size "
                          "(reference for memory-counter profiles) and exit")
@@ -205,13 +209,13 @@ def main():
         },
     }
 
-    if not args.no_mixed:
+    if args.mixed:
         line["mixed"] = mixed_run(amd, make_problem, physics, primaries, args, value,
                                   events_per_s)
     if args.serve_events > 0 and args.workload == "testem3":
         line["serve"] = serve_run(amd, wl, physics, args)
     if not args.no_roofline:
-        line["roofline"] = roofline(amd, params, primaries, args.slots)
+        line["roofline"] = roofline(amd, params, primaries, args.slots, args.track_order)
     if args.cpu_primaries > 0:
         line["cpu_baseline"] = cpu_baseline(amd, wl, problem, args)
     print(json.dumps(line), flush=True)
@@ -245,23 +249,25 @@ def mixed_run(amd, make_problem, physics, primaries, args, fp64_value, fp64_even
 
 
 def serve_run(amd, wl, physics, args):
-    """Served TestEm3 stream: mixed precision, action-sorted, continuous
-    batching on 2^21 slots, against the same events one at a time through
-    the plain FP64 loop (unsorted, 2^20 slots)."""
+    """Served TestEm3 stream: FP64, action-sorted, continuous batching on
+    2^21 slots with graph replay, against the same events one at a time
+    through the plain FP64 loop (unsorted, 2^20 slots)."""
     from celeritas_amd import serving
 
     events = wl.served_events(args.serve_events, seed=args.seed)
     energy = float(sum(e["energy"].sum() for e in events))
     served = amd.CoreParams(wl.testem3_problem(
         physics=physics, init_capacity=1 << 26, max_events=1 << 16,
-        track_order="reindex_both_action", precision="mixed"))
+        track_order="reindex_both_action"))
     server = serving.Server(served, 1 << 21)
-    server.run(events[:2], [0.0, 0.0])  # warm-up
-    sat = server.run(events, serving.poisson_arrivals(len(events), float("inf")))
+    server.run(events[:2], [0.0, 0.0], max_steps=200_000)  # warm-up
+    sat = server.run(events, serving.poisson_arrivals(len(events), float("inf")),
+                     max_steps=200_000)
     log(f"served (saturated): {sat['events_per_s']:.3g} events/s, "
         f"{sat['track_steps'] / sat['wall_time']:.3g} track-steps/s")
     rate = 0.5 * sat["events_per_s"]
-    half = server.run(events, serving.poisson_arrivals(len(events), rate, seed=args.seed))
+    half = server.run(events, serving.poisson_arrivals(len(events), rate, seed=args.seed),
+                      max_steps=200_000)
     log(f"served at {rate:.3g} Hz: p50 {half['p50_latency']:.3g} s, "
         f"p99 {half['p99_latency']:.3g} s")
 
@@ -286,7 +292,7 @@ def serve_run(amd, wl, physics, args):
     return {
         "workload": f"{len(events)} TestEm3 events, primaries per event log-uniform "
                     "10-2000, energies log-uniform 0.1-50 GeV, e-/e+/gamma 40/20/40, "
-                    "mixed precision, reindex_both_action, 2^21 slots",
+                    "FP64, reindex_both_action, graph replay, 2^21 slots",
         "events_per_s": sat["events_per_s"],
         "track_steps_per_s": sat["track_steps"] / sat["wall_time"],
         "primary_gev_per_s": served_gev,
@@ -302,7 +308,20 @@ def serve_run(amd, wl, physics, args):
     }
 
 
-def roofline(amd, params, primaries, slots):
+def sorted_launch(label, track_order):
+    """Whether the action launches over its sorted thread range."""
+    along = label.startswith("along-step")
+    post = label == "geo-boundary"
+    if track_order == "reindex_both_action":
+        return along or post
+    if track_order == "reindex_along_step_action":
+        return along
+    if track_order == "reindex_step_limit_action":
+        return post
+    return False
+
+
+def roofline(amd, params, primaries, slots, track_order):
     """Dominant kernel's algorithmic bandwidth from an instrumented run."""
     st = amd.Stepper(params, slots, action_times=True)
     st.diagnostics(enable=True)
@@ -316,7 +335,12 @@ def roofline(amd, params, primaries, slots):
                                    ranked[0])
     per_unit, per_slot, counter = BYTES_PER_UNIT[label]
     units = diag[counter]
-    algorithmic = per_unit * units + per_slot * slots * launches
+    if sorted_launch(label, track_order):
+        # Launched over its own thread range: one 4-byte slot index per
+        # track instead of scanning every slot's status and action
+        algorithmic = (per_unit + 4) * units
+    else:
+        algorithmic = per_unit * units + per_slot * slots * launches
     per_launch = algorithmic / launches
     avg_launch = secs / launches
     achieved = per_launch / avg_launch / 1e9

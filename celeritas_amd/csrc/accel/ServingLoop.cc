@@ -49,7 +49,7 @@ ServeResult ServingLoop::run(Primary const* primaries,
         CELER_VALIDATE(events[e].num_primaries > 0, << "event " << e << " has no primaries");
     }
     CoreState& state = stepper_->state();
-    CELER_VALIDATE(state.counters().num_active == 0 && state.counters().num_initializers == 0,
+    CELER_VALIDATE(state.counters().num_alive == 0 && state.counters().num_initializers == 0,
                    << "serving loop needs an empty state");
     // Event IDs restart at zero: clear the per-event tallies and counters
     auto const& init = state.ref().init;
