@@ -547,6 +547,25 @@ def verify_field_propagate(params: CoreParams, tracks, steps,
     return out
 
 
+def verify_geo_locate(params: CoreParams, points) -> np.ndarray:
+    """Volume of each point (rows of pos[3], dir[3]) as found by track
+    initialization on the device; 0xffffffff where it fails."""
+    lib_ = _load()
+    f = lib_.celer_verify_geo_locate
+    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+                  ctypes.c_void_p]
+    f.restype = ctypes.c_int
+    lib_.celer_params_device_ref.restype = ctypes.c_void_p
+    lib_.celer_params_device_ref.argtypes = [ctypes.c_void_p]
+    p = np.ascontiguousarray(points, np.float64).reshape(-1, 6)
+    out = np.zeros(len(p), np.uint32)
+    rc = f(lib_.celer_params_device_ref(params.handle), p.ctypes.data, len(p),
+           out.ctypes.data, None)
+    if rc != 0:
+        raise CeleritasError(f"celer_verify_geo_locate failed with HIP error {rc}")
+    return out
+
+
 def generate_physics(spec: dict | str) -> dict:
     """Expand a compact physics description into full tables (host only).
 

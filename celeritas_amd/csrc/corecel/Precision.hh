@@ -70,32 +70,41 @@ CELER_FORCEINLINE_FUNCTION canonical_real draw_canonical(Engine& rng)
 
 namespace mth
 {
-#if CELER_PREC_IS_MIXED && CELER_DEVICE_COMPILE
-// FP32 evaluation of double arguments (fast hardware paths)
+#if CELER_PREC_IS_MIXED
+// FP32 evaluation of double arguments: the fast hardware paths on the
+// device; the host build (used only to exercise the mixed samplers on the
+// CPU) evaluates the same functions with the FP32 C library
+#    if CELER_DEVICE_COMPILE
+#        define CELER_MTH_F32(DEVICE, HOST) DEVICE
+#    else
+#        define CELER_MTH_F32(DEVICE, HOST) HOST
+#    endif
 CELER_FORCEINLINE_FUNCTION double log(double x)
 {
-    return static_cast<double>(__logf(static_cast<float>(x)));
+    return static_cast<double>(CELER_MTH_F32(__logf, ::logf)(static_cast<float>(x)));
 }
 CELER_FORCEINLINE_FUNCTION double exp(double x)
 {
-    return static_cast<double>(__expf(static_cast<float>(x)));
+    return static_cast<double>(CELER_MTH_F32(__expf, ::expf)(static_cast<float>(x)));
 }
 CELER_FORCEINLINE_FUNCTION double sqrt(double x)
 {
-    return static_cast<double>(__fsqrt_rn(static_cast<float>(x)));
+    return static_cast<double>(CELER_MTH_F32(__fsqrt_rn, ::sqrtf)(static_cast<float>(x)));
 }
 CELER_FORCEINLINE_FUNCTION double pow(double x, double y)
 {
-    return static_cast<double>(__powf(static_cast<float>(x), static_cast<float>(y)));
+    return static_cast<double>(
+        CELER_MTH_F32(__powf, ::powf)(static_cast<float>(x), static_cast<float>(y)));
 }
 CELER_FORCEINLINE_FUNCTION double sin(double x)
 {
-    return static_cast<double>(__sinf(static_cast<float>(x)));
+    return static_cast<double>(CELER_MTH_F32(__sinf, ::sinf)(static_cast<float>(x)));
 }
 CELER_FORCEINLINE_FUNCTION double cos(double x)
 {
-    return static_cast<double>(__cosf(static_cast<float>(x)));
+    return static_cast<double>(CELER_MTH_F32(__cosf, ::cosf)(static_cast<float>(x)));
 }
+#    undef CELER_MTH_F32
 CELER_FORCEINLINE_FUNCTION double cbrt(double x)
 {
     return static_cast<double>(::cbrtf(static_cast<float>(x)));

@@ -56,6 +56,32 @@ inline constexpr int max_intersections = 2 * max_faces;
 
 //---------------------------------------------------------------------------//
 /*!
+ * Node of the point-location tree (a bounding interval hierarchy, the
+ * structure of orange/detail/BIHData.hh, stored depth-first).
+ *
+ * Each node carries the one-plane test of the edge that leads into it: a
+ * point passes if its coordinate along \c axis is <= \c plane (left child:
+ * the plane bounds the left volumes from above) or >= \c plane (right
+ * child: it bounds the right volumes from below); axis 3 always passes
+ * (the root). A point that passes an inner node continues with the next
+ * node (its left child); otherwise, and after a leaf, it continues at
+ * \c skip, the first node after the subtree. Leaves list \c num_vols
+ * volumes in \c bih_vols from \c vol_begin, in ascending order.
+ */
+struct BihNode
+{
+    double plane;
+    std::uint32_t skip;
+    std::uint32_t vol_begin;
+    std::uint16_t num_vols;  //!< 0 for an inner node
+    std::uint8_t axis;
+    std::uint8_t upper;  //!< 1: pass if x >= plane; 0: pass if x <= plane
+    std::uint32_t pad_;
+};
+static_assert(sizeof(BihNode) == 24, "BihNode layout");
+
+//---------------------------------------------------------------------------//
+/*!
  * Immutable geometry data (device pointers).
  */
 struct GeoParamsRef
@@ -83,6 +109,14 @@ struct GeoParamsRef
     std::uint32_t const* conn_begin{nullptr};
     std::uint32_t const* conn_size{nullptr};
     std::uint32_t const* conn_vols{nullptr};
+
+    // Point location: tree over the finite bounding boxes, then the
+    // volumes with an unbounded box (bih_vols[inf_begin, +inf_count))
+    BihNode const* bih_nodes{nullptr};
+    std::uint32_t const* bih_vols{nullptr};
+    std::uint32_t num_bih_nodes{0};
+    std::uint32_t bih_inf_begin{0};
+    std::uint32_t bih_inf_count{0};
 
     std::uint32_t background{invalid_id};  //!< Implicit fill volume
     std::uint32_t exterior{0};  //!< Volume that ends transport

@@ -92,6 +92,144 @@ void halfspace_boxes(SurfaceType t, std::vector<double> const& d, Box* neg, Box*
         }
     }
 }
+
+bool is_finite(Box const& b)
+{
+    for (int k = 0; k < 3; ++k)
+    {
+        if (!std::isfinite(b.lo[k]) || !std::isfinite(b.hi[k]))
+            return false;
+    }
+    return true;
+}
+
+bool is_empty(Box const& b)
+{
+    for (int k = 0; k < 3; ++k)
+    {
+        if (!(b.lo[k] <= b.hi[k]))
+            return true;
+    }
+    return false;
+}
+
+//! Half the surface area of a box
+double half_area(Box const& b)
+{
+    double d[3];
+    for (int k = 0; k < 3; ++k)
+        d[k] = std::max(b.hi[k] - b.lo[k], 0.0);
+    return d[0] * d[1] + d[1] * d[2] + d[2] * d[0];
+}
+
+/*!
+ * Depth-first builder of the point-location tree.
+ *
+ * Partitions as the reference does (orange/detail/BIHPartitioner.cc:
+ * 43-80): candidate planes lie halfway between consecutive distinct box
+ * centres along each axis (at most max_candidates per axis, equally
+ * spaced), a volume goes left when its centre is below the plane, and the
+ * cost of a split is the sum over both sides of the volume count times the
+ * surface area of their union box. A node becomes a leaf at max_leaf
+ * volumes or when no plane separates the centres.
+ */
+struct BihBuilder
+{
+    static constexpr std::size_t max_leaf = 2;
+    static constexpr std::size_t max_candidates = 64;
+
+    std::vector<Box> const& boxes;
+    std::vector<BihNode> nodes;
+    std::vector<std::uint32_t> vols;
+
+    double center(std::uint32_t v, int ax) const
+    {
+        return 0.5 * (boxes[v].lo[ax] + boxes[v].hi[ax]);
+    }
+
+    void make(std::vector<std::uint32_t> ids, int axis, bool upper, double plane)
+    {
+        auto const self = static_cast<std::uint32_t>(nodes.size());
+        BihNode node{};
+        node.plane = plane;
+        node.axis = static_cast<std::uint8_t>(axis);
+        node.upper = upper;
+        nodes.push_back(node);
+
+        double best_cost = HUGE_VAL;
+        int best_axis = -1;
+        double best_pos = 0;
+        if (ids.size() > max_leaf)
+        {
+            for (int ax = 0; ax < 3; ++ax)
+            {
+                std::vector<double> c;
+                for (auto v : ids)
+                    c.push_back(this->center(v, ax));
+                std::sort(c.begin(), c.end());
+                c.erase(std::unique(c.begin(), c.end()), c.end());
+                std::size_t const step
+                    = std::max<std::size_t>(c.size() / (max_candidates + 1), 1);
+                for (std::size_t i = step; i < c.size(); i += step)
+                {
+                    double const pos = 0.5 * (c[i - 1] + c[i]);
+                    Box l = Box::empty(), r = Box::empty();
+                    double nl = 0, nr = 0;
+                    for (auto v : ids)
+                    {
+                        if (this->center(v, ax) < pos)
+                        {
+                            l = unite(l, boxes[v]);
+                            nl += 1;
+                        }
+                        else
+                        {
+                            r = unite(r, boxes[v]);
+                            nr += 1;
+                        }
+                    }
+                    double const cost = nl * half_area(l) + nr * half_area(r);
+                    if (cost < best_cost)
+                    {
+                        best_cost = cost;
+                        best_axis = ax;
+                        best_pos = pos;
+                    }
+                }
+            }
+        }
+        if (best_axis < 0)
+        {
+            CELER_VALIDATE(ids.size() < 65536,
+                           << "too many volumes share one bounding-box centre");
+            std::sort(ids.begin(), ids.end());
+            nodes[self].vol_begin = static_cast<std::uint32_t>(vols.size());
+            nodes[self].num_vols = static_cast<std::uint16_t>(ids.size());
+            vols.insert(vols.end(), ids.begin(), ids.end());
+        }
+        else
+        {
+            std::vector<std::uint32_t> left, right;
+            Box lb = Box::empty(), rb = Box::empty();
+            for (auto v : ids)
+            {
+                if (this->center(v, best_axis) < best_pos)
+                {
+                    left.push_back(v);
+                    lb = unite(lb, boxes[v]);
+                }
+                else
+                {
+                    right.push_back(v);
+                    rb = unite(rb, boxes[v]);
+                }
+            }
+            this->make(std::move(left), best_axis, false, lb.hi[best_axis]);
+            this->make(std::move(right), best_axis, true, rb.lo[best_axis]);
+        }
+        nodes[self].skip = static_cast<std::uint32_t>(nodes.size());
+    }
+};
 //---------------------------------------------------------------------------//
 }  // namespace
 
@@ -258,6 +396,33 @@ GeoParams::GeoParams(GeoInput const& inp)
         for (int k = 0; k < 3; ++k)
             h_bbox.push_back(bb.hi[k]);
     }
+    // Point-location tree over the non-implicit volumes with a finite,
+    // non-empty box; unbounded ones are tested after it, empty ones never
+    // contain a point
+    {
+        std::vector<Box> boxes(num_volumes_);
+        std::vector<std::uint32_t> finite, unbounded;
+        for (size_type v = 0; v < num_volumes_; ++v)
+        {
+            for (int k = 0; k < 3; ++k)
+            {
+                boxes[v].lo[k] = h_bbox[6 * v + k];
+                boxes[v].hi[k] = h_bbox[6 * v + 3 + k];
+            }
+            if ((h_flags[v] & implicit_vol) || is_empty(boxes[v]))
+                continue;
+            (is_finite(boxes[v]) ? finite : unbounded).push_back(v);
+        }
+        BihBuilder b{boxes, {}, {}};
+        if (!finite.empty())
+            b.make(std::move(finite), 3, false, 0);
+        h_bih_nodes = std::move(b.nodes);
+        h_bih_vols = std::move(b.vols);
+        bih_inf_begin_ = static_cast<std::uint32_t>(h_bih_vols.size());
+        bih_inf_count_ = static_cast<std::uint32_t>(unbounded.size());
+        h_bih_vols.insert(h_bih_vols.end(), unbounded.begin(), unbounded.end());
+    }
+
     for (size_type s = 0; s < num_surfaces_; ++s)
     {
         h_conn_begin.push_back(static_cast<std::uint32_t>(h_conn_vols.size()));
@@ -282,6 +447,8 @@ GeoParams::GeoParams(GeoInput const& inp)
     d_conn_begin_ = to_device(h_conn_begin);
     d_conn_size_ = to_device(h_conn_size);
     d_conn_vols_ = to_device(h_conn_vols);
+    d_bih_nodes_ = to_device(h_bih_nodes);
+    d_bih_vols_ = to_device(h_bih_vols);
 
     ref_.num_surfaces = num_surfaces_;
     ref_.num_volumes = num_volumes_;
@@ -300,6 +467,11 @@ GeoParams::GeoParams(GeoInput const& inp)
     ref_.conn_begin = d_conn_begin_.data();
     ref_.conn_size = d_conn_size_.data();
     ref_.conn_vols = d_conn_vols_.data();
+    ref_.bih_nodes = d_bih_nodes_.data();
+    ref_.bih_vols = d_bih_vols_.data();
+    ref_.num_bih_nodes = static_cast<std::uint32_t>(h_bih_nodes.size());
+    ref_.bih_inf_begin = bih_inf_begin_;
+    ref_.bih_inf_count = bih_inf_count_;
     ref_.background = background;
     ref_.exterior = 0;
 }

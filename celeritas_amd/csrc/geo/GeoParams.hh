@@ -45,10 +45,12 @@ std::vector<std::int32_t> parse_logic_string(std::string const& s);
 /*!
  * Geometry params: validated host data plus device arrays.
  *
- * Replaces OrangeParams (orange/OrangeParams.hh) for one simple unit; the
- * BIH used by the reference for point location is replaced by per-volume
- * bounding boxes derived from the CSG logic (interval propagation through
- * the postfix expression), tested before the sense evaluation.
+ * Replaces OrangeParams (orange/OrangeParams.hh) for one simple unit.
+ * Per-volume bounding boxes come from the CSG logic (interval propagation
+ * through the postfix expression); point location walks a bounding
+ * interval hierarchy over them (the reference's BIH, orange/detail/
+ * BIHBuilder.cc, stored depth-first with escape indices so the device
+ * walk needs no stack) and tests a volume's box before its senses.
  */
 class GeoParams
 {
@@ -78,9 +80,13 @@ class GeoParams
     std::vector<std::uint32_t> h_conn_begin;
     std::vector<std::uint32_t> h_conn_size;
     std::vector<std::uint32_t> h_conn_vols;
+    std::vector<BihNode> h_bih_nodes;
+    std::vector<std::uint32_t> h_bih_vols;
     std::uint32_t background{invalid_id};
 
   private:
+    std::uint32_t bih_inf_begin_{0};
+    std::uint32_t bih_inf_count_{0};
     size_type num_volumes_{0};
     size_type num_surfaces_{0};
     std::vector<std::string> labels_;
@@ -100,6 +106,8 @@ class GeoParams
     DeviceVector<std::uint32_t> d_conn_begin_;
     DeviceVector<std::uint32_t> d_conn_size_;
     DeviceVector<std::uint32_t> d_conn_vols_;
+    DeviceVector<BihNode> d_bih_nodes_;
+    DeviceVector<std::uint32_t> d_bih_vols_;
     GeoParamsRef ref_;
 };
 

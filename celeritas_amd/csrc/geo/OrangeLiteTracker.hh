@@ -415,11 +415,46 @@ in_bbox(GeoParamsRef const& p, std::uint32_t vol, double const* x)
            && x[1] <= b[4] && x[2] <= b[5];
 }
 
+/*!
+ * First volume of bih_vols[begin, begin + count) whose box and logic
+ * contain the point (the "is_inside" test of orange/detail/
+ * BIHTraverser.hh:254-266 with SimpleUnitTracker's sense evaluation).
+ */
+CELER_FUNCTION bool find_in_list(GeoParamsRef const& p,
+                                 std::uint32_t begin,
+                                 std::uint32_t count,
+                                 double const* x,
+                                 std::uint32_t* found,
+                                 bool* on_surface)
+{
+    for (std::uint32_t k = 0; k < count; ++k)
+    {
+        std::uint32_t const v = p.bih_vols[begin + k];
+        if (!in_bbox(p, v, x))
+            continue;
+        int on_face;
+        std::uint32_t const mask = calc_sense_mask(p, v, x, -1, false, &on_face);
+        if (eval_logic(p, v, mask))
+        {
+            *found = v;
+            *on_surface = on_face >= 0;
+            return true;
+        }
+    }
+    return false;
+}
+
 //---------------------------------------------------------------------------//
 // TRACKING
 //---------------------------------------------------------------------------//
 /*!
  * Initialize the volume from the position; "failed" if on a surface.
+ *
+ * The candidate volumes come from the point-location tree (a node's plane
+ * test prunes its subtree; a leaf's volumes are tested in order), then the
+ * volumes with an unbounded box. Volumes of a simple unit do not overlap,
+ * so the first match is the volume (or, for a point on a face shared by
+ * two volumes, either one, and the point is rejected as on a surface).
  */
 CELER_FUNCTION void initialize(GeoParamsRef const& p,
                                GeoTrack& g,
@@ -434,21 +469,30 @@ CELER_FUNCTION void initialize(GeoParamsRef const& p,
     g.failed = 0;
     std::uint32_t found = invalid_id;
     bool on_surface = false;
-    for (std::uint32_t v = 0; v < p.num_volumes; ++v)
+    std::uint32_t i = 0;
+    while (i < p.num_bih_nodes)
     {
-        if (p.vol_flags[v] & implicit_vol)
-            continue;
-        if (!in_bbox(p, v, pos))
-            continue;
-        int on_face;
-        std::uint32_t const mask = calc_sense_mask(p, v, pos, -1, false, &on_face);
-        if (eval_logic(p, v, mask))
+        BihNode const node = p.bih_nodes[i];
+        bool pass = true;
+        if (node.axis < 3)
         {
-            found = v;
-            on_surface = on_face >= 0;
+            double const xa = component(pos, node.axis);
+            pass = node.upper ? xa >= node.plane : xa <= node.plane;
+        }
+        if (pass && node.num_vols == 0)
+        {
+            ++i;
+            continue;
+        }
+        if (pass
+            && find_in_list(p, node.vol_begin, node.num_vols, pos, &found, &on_surface))
+        {
             break;
         }
+        i = node.skip;
     }
+    if (found == invalid_id)
+        find_in_list(p, p.bih_inf_begin, p.bih_inf_count, pos, &found, &on_surface);
     if (on_surface)
     {
         found = invalid_id;

@@ -46,9 +46,10 @@ void TrackingCutAction::step(CoreParams const& params, CoreState& state) const
 
 void CaloTallyAction::step(CoreParams const& params, CoreState& state) const
 {
+    // Every slot is visited once in any order, so the tally reads the slots
+    // in place (coalesced) rather than through the sorted permutation
     CELER_HIP_CALL(launch_calo_tally(params.device_ref(), state.ref(),
-                                     params.num_detectors(),
-                                     make_thread_slots(params, state, *this),
+                                     params.num_detectors(), ThreadSlots{},
                                      state.stream()));
 }
 

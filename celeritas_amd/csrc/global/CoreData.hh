@@ -158,6 +158,8 @@ struct CoreStateRef
     std::uint32_t* action_offsets{nullptr};  //!< sorted ranges per key
     std::uint32_t* sort_counts{nullptr};  //!< key histogram per sort tile
     double* calo_edep{nullptr};  //!< per-volume energy deposition tally
+    //! Per-workgroup partial tallies [calo_tally_blocks][detectors]
+    double* calo_partial{nullptr};
     //! Optional work counters (null unless diagnostics are enabled), see
     //! DiagCounter
     unsigned long long* diag{nullptr};
@@ -165,6 +167,11 @@ struct CoreStateRef
 
 //! Largest number of sort keys (actions + 1 for inactive slots)
 inline constexpr unsigned int max_sort_keys = 64;
+
+//! Workgroups of the calorimeter tally's first pass
+inline constexpr unsigned int calo_tally_blocks = 1024;
+//! Detector bins that the tally reduces in LDS and in two passes
+inline constexpr unsigned int max_lds_detectors = 512;
 
 //! Per-slot key of a track reindexing
 enum class SortKey : std::uint8_t

@@ -107,6 +107,10 @@ CoreState::CoreState(CoreParams const& params,
     add(&s.action_offsets, max_sort_keys + 1);
     add(&s.sort_counts, sort_counts_size(num_slots));
     add(&s.calo_edep, std::max<size_type>(num_detectors_, 1));
+    add(&s.calo_partial,
+        num_detectors_ <= max_lds_detectors
+            ? calo_tally_blocks * std::max<size_type>(num_detectors_, 1)
+            : 1);
     a.allocate();
     for (auto& p : plan)
         p.second(p.first);

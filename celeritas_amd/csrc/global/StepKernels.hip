@@ -13,6 +13,8 @@
 //! early-exit for slots whose action does not match, exactly like the
 //! reference's ConditionalTrackExecutor.
 //---------------------------------------------------------------------------//
+#include <algorithm>
+
 #include "corecel/Precision.hh"
 #include "CoreActions.hh"
 
@@ -28,6 +30,7 @@ namespace
 {
 //---------------------------------------------------------------------------//
 constexpr int block_size = 256;
+constexpr int num_waves = block_size / wave::size;
 
 inline unsigned int num_blocks(size_type n)
 {
@@ -364,50 +367,94 @@ __global__ void __launch_bounds__(block_size)
 }
 
 //---------------------------------------------------------------------------//
-// CALORIMETER TALLY: per-workgroup reduction in LDS, one atomic per bin
+// CALORIMETER TALLY (the SimpleCalo user action's step callback)
+//
+// First pass: at most calo_tally_blocks workgroups stride over the slots
+// and sum their deposits into LDS bins, then write the bins to their row of
+// calo_partial. Second pass: one workgroup per bin adds the rows and the
+// tally. Compared with one global atomic per bin per workgroup of 256
+// slots, the same-address atomics that serialize in L2 are gone.
 //---------------------------------------------------------------------------//
-constexpr int max_lds_detectors = 512;
+//! Detector bin and deposit of a slot (false if nothing to tally)
+__device__ __forceinline__ bool calo_deposit(CoreParamsRef const& params,
+                                             CoreStateRef const& state,
+                                             size_type slot,
+                                             std::uint32_t* det,
+                                             double* edep)
+{
+    if (status_of(state, slot) == TrackStatus::inactive)
+        return false;
+    *edep = state.phys.energy_deposition[slot];
+    if (*edep == 0)
+        return false;
+    *det = params.volume_to_detector[state.pre_volume[slot]];
+    return *det != invalid_id;
+}
 
 __global__ void __launch_bounds__(block_size)
-    calo_tally_kernel(CoreParamsRef const params,
-                      CoreStateRef const state,
-                      size_type const num_det,
-                      ThreadSlots const ts)
+    calo_partial_kernel(CoreParamsRef const params,
+                        CoreStateRef const state,
+                        size_type const num_det,
+                        ThreadSlots const ts)
 {
     __shared__ double bins[max_lds_detectors];
-    bool const use_lds = num_det <= max_lds_detectors;
-    if (use_lds)
-    {
-        for (size_type i = threadIdx.x; i < num_det; i += block_size)
-            bins[i] = 0;
-        __syncthreads();
-    }
+    for (size_type i = threadIdx.x; i < num_det; i += block_size)
+        bins[i] = 0;
+    __syncthreads();
+    size_type const stride = gridDim.x * block_size;
     size_type slot;
-    if (thread_to_slot(ts, blockIdx.x * blockDim.x + threadIdx.x, state.size, &slot)
-        && status_of(state, slot) != TrackStatus::inactive)
+    for (size_type tid = blockIdx.x * block_size + threadIdx.x;
+         thread_to_slot(ts, tid, state.size, &slot);
+         tid += stride)
     {
-        double const edep = state.phys.energy_deposition[slot];
-        if (edep != 0)
-        {
-            std::uint32_t const det = params.volume_to_detector[state.pre_volume[slot]];
-            if (det != invalid_id)
-            {
-                if (use_lds)
-                    atomicAdd(&bins[det], edep);
-                else
-                    atomicAdd(&state.calo_edep[det], edep);
-            }
-        }
+        std::uint32_t det;
+        double edep;
+        if (calo_deposit(params, state, slot, &det, &edep))
+            atomicAdd(&bins[det], edep);
     }
-    if (use_lds)
+    __syncthreads();
+    double* row = state.calo_partial + std::size_t(blockIdx.x) * num_det;
+    for (size_type i = threadIdx.x; i < num_det; i += block_size)
+        row[i] = bins[i];
+}
+
+//! Sum of the partial rows of one bin (workgroup blockIdx.x = bin)
+__global__ void __launch_bounds__(block_size)
+    calo_reduce_kernel(CoreStateRef const state,
+                       size_type const num_det,
+                       unsigned int const num_rows)
+{
+    __shared__ double lds[num_waves];
+    size_type const det = blockIdx.x;
+    double sum = 0;
+    for (unsigned int r = threadIdx.x; r < num_rows; r += block_size)
+        sum += state.calo_partial[std::size_t(r) * num_det + det];
+    sum = wave::reduce_sum(sum);
+    if (wave::lane_id() == 0)
+        lds[threadIdx.x / wave::size] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0)
     {
-        __syncthreads();
-        for (size_type i = threadIdx.x; i < num_det; i += block_size)
-        {
-            if (bins[i] != 0)
-                atomicAdd(&state.calo_edep[i], bins[i]);
-        }
+        double total = 0;
+        for (int w = 0; w < num_waves; ++w)
+            total += lds[w];
+        state.calo_edep[det] += total;
     }
+}
+
+//! Direct global atomics, for more bins than LDS holds
+__global__ void __launch_bounds__(block_size)
+    calo_atomic_kernel(CoreParamsRef const params,
+                       CoreStateRef const state,
+                       ThreadSlots const ts)
+{
+    size_type slot;
+    if (!thread_to_slot(ts, blockIdx.x * blockDim.x + threadIdx.x, state.size, &slot))
+        return;
+    std::uint32_t det;
+    double edep;
+    if (calo_deposit(params, state, slot, &det, &edep))
+        atomicAdd(&state.calo_edep[det], edep);
 }
 
 //---------------------------------------------------------------------------//
@@ -506,8 +553,16 @@ hipError_t launch_calo_tally(CoreParamsRef const& p,
                              ThreadSlots const& ts,
                              hipStream_t st)
 {
-    if (num_det > 0)
-        calo_tally_kernel<<<num_blocks(s.size), block_size, 0, st>>>(p, s, num_det, ts);
+    if (num_det == 0)
+        return hipSuccess;
+    if (num_det > max_lds_detectors)
+    {
+        calo_atomic_kernel<<<num_blocks(s.size), block_size, 0, st>>>(p, s, ts);
+        return hipGetLastError();
+    }
+    unsigned int const rows = std::min<unsigned int>(num_blocks(s.size), calo_tally_blocks);
+    calo_partial_kernel<<<rows, block_size, 0, st>>>(p, s, num_det, ts);
+    calo_reduce_kernel<<<num_det, block_size, 0, st>>>(s, num_det, rows);
     return hipGetLastError();
 }
 

@@ -421,6 +421,19 @@ int celer_verify_field_propagate(const void* params_ref,
                                  double* out,
                                  void* stream);
 
+/*---------------------------------------------------------------------------*
+ * VERIFICATION: point location as in track initialization
+ * (orange/univ/SimpleUnitTracker.hh:161-190, candidates from the BIH of
+ * orange/detail/BIHTraverser.hh). Each of n points is 6 doubles (pos [cm],
+ * dir); writes its volume ID, or 0xffffffff where initialization fails
+ * (the point lies on a face, or in no volume and there is no background).
+ *---------------------------------------------------------------------------*/
+int celer_verify_geo_locate(const void* params_ref,
+                            const double* points,
+                            uint32_t n,
+                            uint32_t* out,
+                            void* stream);
+
 #ifdef __cplusplus
 }
 #endif

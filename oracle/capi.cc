@@ -321,6 +321,20 @@ std::uint32_t orc_geo_trace(void* problem, double const* pos, double const* dir,
     return n;
 }
 
+//! Volume of each point (pos[3], dir[3]) by exhaustive search, or
+//! 0xffffffff where initialization fails
+void orc_geo_locate(void* problem, double const* points, std::uint32_t n,
+                    std::uint32_t* out)
+{
+    Geometry const& g = static_cast<OrcProblem*>(problem)->p->geo;
+    for (std::uint32_t i = 0; i < n; ++i)
+    {
+        GeoState st;
+        geo_initialize(g, st, points + 6 * i, points + 6 * i + 3);
+        out[i] = st.failed ? 0xffffffffu : st.vol;
+    }
+}
+
 }  // extern "C"
 
 extern "C" {

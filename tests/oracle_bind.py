@@ -138,6 +138,17 @@ class Problem:
             return vols[:n], dists[:n], safe[:n]
         return vols[:n], dists[:n]
 
+    def locate(self, points):
+        """Volume of each point (rows of pos[3], dir[3]); 0xffffffff where
+        initialization fails."""
+        p = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 6)
+        out = np.zeros(len(p), dtype=np.uint32)
+        f = load().orc_geo_locate
+        f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]
+        f.restype = None
+        f(self._h, p.ctypes.data, len(p), out.ctypes.data)
+        return out
+
 
 class State:
     def __init__(self, problem: Problem, num_slots: int, stream_id: int = 0):
