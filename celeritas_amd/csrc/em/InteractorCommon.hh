@@ -54,10 +54,28 @@ CELER_FORCEINLINE_FUNCTION void make_unit(double* v)
     v[2] *= scale;
 }
 
+/*!
+ * sqrt(1 - c^2) for a cosine c. The mixed build evaluates c from FP32
+ * transcendentals, so |c| can exceed 1 by an ulp; it returns 0 there (and
+ * for a NaN) instead of a NaN that would spread into the track state.
+ */
+CELER_FORCEINLINE_FUNCTION double sin_of_cos(double c)
+{
+    double const s2 = 1 - c * c;
+#if CELER_PREC_IS_MIXED
+    return s2 > 0 ? mth::sqrt(s2) : 0.0;
+#else
+    return mth::sqrt(s2);
+#endif
+}
+
 CELER_FORCEINLINE_FUNCTION void
 from_spherical(double costheta, double phi, double* out)
 {
-    double const sintheta = mth::sqrt(1 - costheta * costheta);
+#if CELER_PREC_IS_MIXED
+    costheta = costheta > 1 ? 1.0 : (costheta < -1 ? -1.0 : costheta);
+#endif
+    double const sintheta = sin_of_cos(costheta);
     double s, c;
 #if CELER_DEVICE_COMPILE
     sincos(phi, &s, &c);
@@ -73,7 +91,7 @@ from_spherical(double costheta, double phi, double* out)
 //! Rotate a direction given in the frame whose z axis is rot
 CELER_FUNCTION void rotate(double const* dir, double const* rot, double* out)
 {
-    double const sintheta = mth::sqrt(1 - rot[2] * rot[2]);
+    double const sintheta = sin_of_cos(rot[2]);
     double cosphi, sinphi;
     if (sintheta >= 0.005)
     {
@@ -84,7 +102,7 @@ CELER_FUNCTION void rotate(double const* dir, double const* rot, double* out)
     else if (sintheta > 0)
     {
         cosphi = rot[0] / mth::sqrt(rot[0] * rot[0] + rot[1] * rot[1]);
-        sinphi = mth::sqrt(1 - cosphi * cosphi);
+        sinphi = sin_of_cos(cosphi);
     }
     else
     {
@@ -116,6 +134,13 @@ CELER_FUNCTION std::uint32_t allocate_secondaries(CoreStateRef const& s,
         atomicSub(&s.counters->secondary_stack_size, n);
         return invalid_id;
     }
+#else
+    // Host (single-threaded) build of the samplers
+    std::uint32_t const start = s.counters->secondary_stack_size;
+    if (start + n > s.secondaries.capacity)
+        return invalid_id;
+    s.counters->secondary_stack_size = start + n;
+#endif
     for (std::uint32_t i = 0; i < n; ++i)
     {
         Secondary& sec = s.secondaries.storage[start + i];
@@ -124,11 +149,6 @@ CELER_FUNCTION std::uint32_t allocate_secondaries(CoreStateRef const& s,
         sec.dir[0] = sec.dir[1] = sec.dir[2] = 0;
     }
     return start;
-#else
-    (void)s;
-    (void)n;
-    return invalid_id;
-#endif
 }
 
 //---------------------------------------------------------------------------//

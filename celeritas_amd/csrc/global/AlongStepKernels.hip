@@ -141,8 +141,12 @@ __global__ void __launch_bounds__(block_size)
             else
                 geo::move_internal(g, step);
         }
-        if (CELER_UNLIKELY(g.pos[0] == orig[0] && g.pos[1] == orig[1]
-                           && g.pos[2] == orig[2]))
+        // A move that leaves the position unchanged, or not finite, cannot
+        // make progress: the track is failed (a finite FP64 state never
+        // reaches the second test)
+        if (CELER_UNLIKELY((g.pos[0] == orig[0] && g.pos[1] == orig[1]
+                            && g.pos[2] == orig[2])
+                           || !std::isfinite(g.pos[0] + g.pos[1] + g.pos[2])))
         {
             g.store(state.geo, slot);
             rng.store(state.rng, slot);

@@ -176,6 +176,12 @@ __global__ void __launch_bounds__(block_size)
         else
             geo::move_internal(g, step);
         g.store(state.geo, slot);
+        if (CELER_UNLIKELY(!std::isfinite(g.pos[0] + g.pos[1] + g.pos[2])))
+        {
+            // Not reachable from a finite FP64 state
+            apply_errored(params, state, slot);
+            return;
+        }
         if (prop.boundary)
         {
             step = prop.distance;
