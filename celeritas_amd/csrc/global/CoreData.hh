@@ -158,7 +158,7 @@ struct CoreStateRef
     std::uint32_t* action_offsets{nullptr};  //!< sorted ranges per key
     std::uint32_t* sort_counts{nullptr};  //!< key histogram per sort tile
     double* calo_edep{nullptr};  //!< per-volume energy deposition tally
-    //! Per-workgroup partial tallies [calo_tally_blocks][detectors]
+    //! Per-workgroup partial tallies [detectors][workgroups of the pass]
     double* calo_partial{nullptr};
     //! Optional work counters (null unless diagnostics are enabled), see
     //! DiagCounter

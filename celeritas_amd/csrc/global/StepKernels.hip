@@ -376,9 +376,9 @@ __global__ void __launch_bounds__(block_size)
 // CALORIMETER TALLY (the SimpleCalo user action's step callback)
 //
 // First pass: at most calo_tally_blocks workgroups stride over the slots
-// and sum their deposits into LDS bins, then write the bins to their row of
-// calo_partial. Second pass: one workgroup per bin adds the rows and the
-// tally. Compared with one global atomic per bin per workgroup of 256
+// and sum their deposits into LDS bins, then write the bins to their
+// column of calo_partial ([bin][workgroup]). Second pass: one workgroup per
+// bin adds its row of partials to the tally. Compared with one global atomic per bin per workgroup of 256
 // slots, the same-address atomics that serialize in L2 are gone.
 //---------------------------------------------------------------------------//
 //! Detector bin and deposit of a slot (false if nothing to tally)
@@ -419,9 +419,9 @@ __global__ void __launch_bounds__(block_size)
             atomicAdd(&bins[det], edep);
     }
     __syncthreads();
-    double* row = state.calo_partial + std::size_t(blockIdx.x) * num_det;
+    // Bin-major: the second pass reads each bin's partials contiguously
     for (size_type i = threadIdx.x; i < num_det; i += block_size)
-        row[i] = bins[i];
+        state.calo_partial[std::size_t(i) * gridDim.x + blockIdx.x] = bins[i];
 }
 
 //! Sum of the partial rows of one bin (workgroup blockIdx.x = bin)
@@ -433,8 +433,9 @@ __global__ void __launch_bounds__(block_size)
     __shared__ double lds[num_waves];
     size_type const det = blockIdx.x;
     double sum = 0;
+    double const* column = state.calo_partial + std::size_t(det) * num_rows;
     for (unsigned int r = threadIdx.x; r < num_rows; r += block_size)
-        sum += state.calo_partial[std::size_t(r) * num_det + det];
+        sum += column[r];
     sum = wave::reduce_sum(sum);
     if (wave::lane_id() == 0)
         lds[threadIdx.x / wave::size] = sum;

@@ -87,7 +87,10 @@ __global__ void __launch_bounds__(block_size)
 }
 
 //! Exclusive scan of the histogram table (one workgroup); the first tile
-//! of each key gives the key's thread offset, and offsets[num_keys] = size
+//! of each key gives the key's thread offset, and offsets[num_keys] = size.
+//! Each thread owns a contiguous chunk: it sums the chunk, one workgroup
+//! scan orders the chunk sums, and the thread rewrites its chunk from its
+//! base (one workgroup-wide scan instead of one per 1024 entries).
 __global__ void __launch_bounds__(1024)
     sort_scan_kernel(CoreStateRef const state,
                      unsigned int const num_keys,
@@ -95,29 +98,23 @@ __global__ void __launch_bounds__(1024)
 {
     constexpr int nw = 1024 / wave::size;
     __shared__ unsigned int lds[2 * nw];
-    __shared__ unsigned int carry;
-    if (threadIdx.x == 0)
-        carry = 0;
-    __syncthreads();
     unsigned int* counts = state.sort_counts;
     unsigned int const total = num_keys * ntiles;
-    for (unsigned int base = 0; base < total; base += 1024)
+    unsigned int const per = (total + 1023) / 1024;
+    unsigned int const begin = min(threadIdx.x * per, total);
+    unsigned int const end = min(begin + per, total);
+    unsigned int sum = 0;
+    for (unsigned int i = begin; i < end; ++i)
+        sum += counts[i];
+    wave::BlockScan2<nw> scan(sum, 0u, lds);
+    unsigned int running = scan.excl_a;
+    for (unsigned int i = begin; i < end; ++i)
     {
-        unsigned int const i = base + threadIdx.x;
-        unsigned int const a = i < total ? counts[i] : 0u;
-        wave::BlockScan2<nw> scan(a, 0u, lds);
-        unsigned int const c = carry;
-        if (i < total)
-        {
-            unsigned int const excl = c + scan.excl_a;
-            counts[i] = excl;
-            if (i % ntiles == 0)
-                state.action_offsets[i / ntiles] = excl;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0)
-            carry = c + scan.total_a;
-        __syncthreads();
+        unsigned int const c = counts[i];
+        counts[i] = running;
+        if (i % ntiles == 0)
+            state.action_offsets[i / ntiles] = running;
+        running += c;
     }
     if (threadIdx.x == 0)
         state.action_offsets[num_keys] = state.size;
