@@ -28,6 +28,8 @@ ServingLoop::ServingLoop(std::shared_ptr<CoreParams const> params,
     // Steps without newly admitted events replay one captured graph
     inp.use_graph = true;
     stepper_ = std::make_unique<Stepper>(std::move(inp));
+    // Completion is detected from the per-event tallies
+    stepper_->state().event_tally(true);
 }
 
 //---------------------------------------------------------------------------//

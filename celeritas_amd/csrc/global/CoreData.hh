@@ -126,6 +126,9 @@ struct InitStateRef
     std::uint32_t* block_scratch{nullptr};  //!< compaction partials
     size_type capacity{0};
     size_type max_events{0};
+    //! 1 if event_live and event_edep are maintained (per-event completion
+    //! for the serving loop); plain transport skips their atomics
+    std::uint32_t event_tally{0};
 };
 
 //! Secondary storage (corecel/data/StackAllocator.hh)

@@ -47,6 +47,10 @@ class CoreState
     //! Workgroup size of the step kernels: 64, 128 or 256
     void block_size(size_type bs);
     size_type block_size() const { return ref_.block_size; }
+    //! Maintain the per-event live-track and deposition tallies (set before
+    //! a step is captured in a graph: launches take the state by value)
+    void event_tally(bool on) { ref_.init.event_tally = on ? 1u : 0u; }
+    bool event_tally() const { return ref_.init.event_tally != 0; }
     //! Enqueue an async copy of the counters to the pinned mirror
     void enqueue_counter_readback();
 

@@ -69,7 +69,8 @@ __global__ void __launch_bounds__(block_size)
     {
         ti.track_id = ranks ? state.init.track_counters[p.event_id] + ranks[tid]
                             : atomicAdd(&state.init.track_counters[p.event_id], 1u);
-        atomicAdd(&state.init.event_live[p.event_id], 1);
+        if (state.init.event_tally)
+            atomicAdd(&state.init.event_live[p.event_id], 1);
     }
     else
     {
@@ -439,7 +440,7 @@ __global__ void __launch_bounds__(block_size)
             }
         }
     }
-    if (event < state.init.max_events)
+    if (state.init.event_tally && event < state.init.max_events)
     {
         if (live_delta != 0)
             atomicAdd(&state.init.event_live[event], live_delta);
