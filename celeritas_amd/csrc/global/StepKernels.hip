@@ -99,11 +99,13 @@ __global__ void __launch_bounds__(block_size)
     std::uint32_t const mat = material_of(params, state, slot);
     double const energy = state.particle.energy[slot];
     double total = 0;
+    // One log and grid bracket of the energy, shared by the processes
+    phys::GridPoint gp(energy);
     for (std::uint32_t i = 0; i < part.num_processes; ++i)
     {
         ProcessRecord const& proc = part.process[i];
-        double const xs = proc.integral ? phys::calc_max_xs(pp, proc, mat, energy)
-                                        : phys::calc_xs(pp, proc, mat, energy);
+        double const xs = proc.integral ? phys::calc_max_xs(pp, proc, mat, gp)
+                                        : phys::calc_xs(pp, proc, mat, gp);
         total += xs;
         ps.per_process_xs[i * state.size + slot] = xs;
     }

@@ -23,6 +23,25 @@ inline constexpr double c_light = 29979245800.0;  //!< [cm/s]
 inline constexpr double pi = 3.14159265358979323846;
 
 /*!
+ * An energy and its bracket on a log-uniform grid: the log and the two grid
+ * energies (one FP64 log and two exps) are the costly part of a lookup, and
+ * grids with the same front and spacing share them. Reusing them repeats
+ * exactly the same operations, so results are bit-identical.
+ */
+struct GridPoint
+{
+    double energy;
+    double loge;
+    double front{HUGE_VAL};  //!< grid front/spacing of the cached bracket
+    double delta{HUGE_VAL};
+    std::uint32_t lower{0};
+    double lower_e{0};
+    double upper_e{0};
+
+    CELER_FUNCTION explicit GridPoint(double e) : energy(e), loge(mth::log(e)) {}
+};
+
+/*!
  * Interpolate a log-uniform grid at an energy (grid/XsCalculator.hh:107-150).
  *
  * Linear in E between the two bracketing points; values at and above the
@@ -30,10 +49,11 @@ inline constexpr double pi = 3.14159265358979323846;
  * used (with the 1/E scaling where applicable).
  */
 CELER_FUNCTION double
-calc_grid(XsGrid const& g, double const* __restrict__ reals, double energy)
+calc_grid(XsGrid const& g, double const* __restrict__ reals, GridPoint& gp)
 {
     double const* v = reals + g.offset;
-    double const loge = mth::log(energy);
+    double const energy = gp.energy;
+    double const loge = gp.loge;
     if (!(loge > g.loge_front))  // also catches NaN: never index with it
     {
         double r = v[0];
@@ -45,18 +65,33 @@ calc_grid(XsGrid const& g, double const* __restrict__ reals, double energy)
         double r = v[i];
         return (i >= g.prime_index) ? r / energy : r;
     }
-    auto const lower = static_cast<std::uint32_t>((loge - g.loge_front) / g.loge_delta);
-    double const upper_e = mth::exp(g.loge_front + g.loge_delta * (lower + 1));
+    if (!(gp.front == g.loge_front && gp.delta == g.loge_delta))
+    {
+        gp.front = g.loge_front;
+        gp.delta = g.loge_delta;
+        gp.lower = static_cast<std::uint32_t>((loge - g.loge_front) / g.loge_delta);
+        gp.upper_e = mth::exp(g.loge_front + g.loge_delta * (gp.lower + 1));
+        gp.lower_e = mth::exp(g.loge_front + g.loge_delta * gp.lower);
+    }
+    std::uint32_t const lower = gp.lower;
+    double const upper_e = gp.upper_e;
     double upper_xs = v[lower + 1];
     if (lower + 1 == g.prime_index)
         upper_xs /= upper_e;
-    double const lower_e = mth::exp(g.loge_front + g.loge_delta * lower);
+    double const lower_e = gp.lower_e;
     double const lower_xs = v[lower];
     double const slope = ((-lower_xs) + upper_xs) / ((-lower_e) + upper_e);
     double result = std::fma(slope, (-lower_e) + energy, lower_xs);
     if (lower >= g.prime_index)
         result /= energy;
     return result;
+}
+
+CELER_FUNCTION double
+calc_grid(XsGrid const& g, double const* __restrict__ reals, double energy)
+{
+    GridPoint gp(energy);
+    return calc_grid(g, reals, gp);
 }
 
 inline constexpr double r_electron = 2.8179403262e-13;  //!< [cm]
@@ -79,10 +114,11 @@ eplusgg_macro_xs(double electron_mass, double electron_density, double energy)
 CELER_FUNCTION double calc_xs(PhysicsParamsRef const& p,
                               ProcessRecord const& proc,
                               std::uint32_t mat,
-                              double energy)
+                              GridPoint& gp)
 {
+    double const energy = gp.energy;
     if (proc.xs_grid != invalid_id)
-        return calc_grid(p.grids[proc.xs_grid + mat], p.reals, energy);
+        return calc_grid(p.grids[proc.xs_grid + mat], p.reals, gp);
     if (proc.process_class == static_cast<std::uint32_t>(ProcessClass::annihilation))
     {
         return eplusgg_macro_xs(p.model.electron_mass,
@@ -93,17 +129,27 @@ CELER_FUNCTION double calc_xs(PhysicsParamsRef const& p,
     return 0;
 }
 
+CELER_FUNCTION double calc_xs(PhysicsParamsRef const& p,
+                              ProcessRecord const& proc,
+                              std::uint32_t mat,
+                              double energy)
+{
+    GridPoint gp(energy);
+    return calc_xs(p, proc, mat, gp);
+}
+
 //! Upper bound of the XS over the step (PhysicsTrackView.hh:446-478)
 CELER_FUNCTION double calc_max_xs(PhysicsParamsRef const& p,
                                   ProcessRecord const& proc,
                                   std::uint32_t mat,
-                                  double energy)
+                                  GridPoint& gp)
 {
+    double const energy = gp.energy;
     double const emax = p.reals[proc.energy_max_xs + mat];
     double const exi = energy * p.scalars.min_eprime_over_e;
     if (emax >= exi && emax < energy)
         return calc_xs(p, proc, mat, emax);
-    double const a = calc_xs(p, proc, mat, energy);
+    double const a = calc_xs(p, proc, mat, gp);
     double const b = calc_xs(p, proc, mat, exi);
     return a > b ? a : b;
 }
