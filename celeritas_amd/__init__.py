@@ -68,8 +68,8 @@ class CCounters(ctypes.Structure):
             "error_flags",
             "steps_taken",
             "num_new_neutral",
+            "num_sampling_failures",
             "pad0",
-            "pad1",
         )
     ]
 

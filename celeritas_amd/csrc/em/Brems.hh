@@ -133,6 +133,7 @@ CELER_FUNCTION double sample_rb_energy(ElementRecord const& el,
     double const logratio = mth::log((1 / a) * (tmax * tmax + dens));
     double const max_xs = el.rb_factor1 + el.rb_factor2;
     double k, dsigma;
+    unsigned int iter = 0;
     do
     {
         k = mth::sqrt(a * mth::exp(logratio * draw_canonical(rng)) - dens);
@@ -141,7 +142,7 @@ CELER_FUNCTION double sample_rb_energy(ElementRecord const& el,
         k = k < tmax ? k : tmax;
 #endif
         dsigma = rb_dxsec(el, total, k);
-    } while (dsigma < max_xs * draw_canonical(rng));
+    } while (dsigma < max_xs * draw_canonical(rng) && within_budget(iter, rng));
     return k;
 }
 
@@ -176,6 +177,7 @@ CELER_FUNCTION double sample_sb_energy(PhysicsParamsRef const& p,
     double const cutoff_invbeta
         = is_electron ? 0 : positron_invbeta(inc_energy, mass, kc);
     double exit, xs;
+    unsigned int iter = 0;
     do
     {
         exit = mth::sqrt(a * mth::exp(logratio * draw_canonical(rng)) - dens);
@@ -196,7 +198,7 @@ CELER_FUNCTION double sample_sb_energy(PhysicsParamsRef const& p,
             scale = mth::exp(alpha_z * (delta < 0 ? delta : 0.0));
         }
         xs = v * scale;
-    } while (xs < max_xs * draw_canonical(rng));
+    } while (xs < max_xs * draw_canonical(rng) && within_budget(iter, rng));
     return exit;
 }
 

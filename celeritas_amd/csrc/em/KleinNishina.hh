@@ -50,6 +50,7 @@ CELER_FUNCTION Interaction sample_klein_nishina(PhysicsParamsRef const& p,
     double const f2_lo = eps0 * eps0;
     double const f2_delta = 1 - f2_lo;
     double eps, omc;
+    unsigned int iter = 0;
     while (true)
     {
         double eps_sq;
@@ -66,7 +67,7 @@ CELER_FUNCTION Interaction sample_klein_nishina(PhysicsParamsRef const& p,
         omc = (1 - eps) / (eps * k);
         double const sin2 = omc * (2 - omc);
         double const reject = eps * sin2 / (1 + eps_sq);
-        if (!(draw_canonical(rng) < reject))
+        if (!(draw_canonical(rng) < reject) || !within_budget(iter, rng))
             break;
     }
     r.energy = eps * energy;

@@ -68,12 +68,13 @@ sample_sauter_gavrila(double inv_electron_mass, double inc_energy,
     double const b = 0.5 * beta * gamma * energy_per_mecsq * (gamma - 2);
     double const g_max = 2 * (1 / a + b);
     double g, nu;
+    unsigned int iter = 0;
     do
     {
         double const u = draw_canonical(rng);
         nu = 2 * a * (2 * u + (a + 2) * mth::sqrt(u)) / ((a + 2) * (a + 2) - 4 * u);
         g = (2 - nu) * (1 / (a + nu) + b);
-    } while (g < g_max * draw_canonical(rng));
+    } while (g < g_max * draw_canonical(rng) && within_budget(iter, rng));
     sample_exiting_direction(1 - nu, inc_dir, rng, out);
 }
 

@@ -38,13 +38,14 @@ CELER_FUNCTION double sample_tsai_urban(double energy, double mass, Engine& rng)
 {
     double const umax = 2 * (1 + energy / mass);
     double u;
+    unsigned int iter = 0;
     do
     {
         double const a = draw_canonical(rng);
         double const b = draw_canonical(rng);
         double const uu = -mth::log(a * b);
         u = uu * (draw_canonical(rng) < 0.25 ? 1.6 : 1.6 / 3);
-    } while (u > umax);
+    } while (u > umax && within_budget(iter, rng));
     double const r = u / umax;
     return 1 - 2 * r * r;
 }
@@ -124,6 +125,7 @@ CELER_FUNCTION Interaction sample_bethe_heitler(PhysicsParamsRef const& p,
         double const sf = 1.5 * f20;
         double const p_f1 = st / (st + sf);
         double g;
+        unsigned int iter = 0;
         do
         {
             if (draw_canonical(rng) < p_f1)
@@ -138,7 +140,7 @@ CELER_FUNCTION Interaction sample_bethe_heitler(PhysicsParamsRef const& p,
                 double const delta = 136 / el.cbrt_z * eps0 / (eps * (1 - eps));
                 g = (bh_screening_f2(delta) - f_z) / f20;
             }
-        } while (g < draw_canonical(rng));
+        } while (g < draw_canonical(rng) && within_budget(iter, rng));
     }
     r.action = Interaction::absorbed;
     r.energy = 0;
@@ -213,6 +215,7 @@ CELER_FUNCTION Interaction sample_moller_bhabha(PhysicsParamsRef const& p,
     double const min_frac = cut / energy;
     double const gamma = 1 + energy / me;
     double eps;
+    unsigned int iter = 0;
     if (particle == p.electron_id)
     {
         double const g_den = moller_g(gamma, 0.5);
@@ -221,7 +224,8 @@ CELER_FUNCTION Interaction sample_moller_bhabha(PhysicsParamsRef const& p,
         do
         {
             eps = 1 / std::fma(delta, draw_canonical(rng), a);
-        } while (g_den * draw_canonical(rng) > moller_g(gamma, eps));
+        } while (g_den * draw_canonical(rng) > moller_g(gamma, eps)
+                 && within_budget(iter, rng));
     }
     else
     {
@@ -231,7 +235,8 @@ CELER_FUNCTION Interaction sample_moller_bhabha(PhysicsParamsRef const& p,
         do
         {
             eps = 1 / std::fma(delta, draw_canonical(rng), a);
-        } while (g_den * draw_canonical(rng) > bhabha_g(gamma, eps, eps));
+        } while (g_den * draw_canonical(rng) > bhabha_g(gamma, eps, eps)
+                 && within_budget(iter, rng));
     }
     double const e_sec = energy * eps;
     double const inc_momentum = mth::sqrt(energy * energy + 2 * me * energy);
@@ -289,11 +294,13 @@ CELER_FUNCTION Interaction sample_eplusgg(PhysicsParamsRef const& p,
     double const lo = half - sqgrate;
     double const logratio = mth::log((1 / lo) * (half + sqgrate));
     double epsil;
+    unsigned int iter = 0;
     do
     {
         epsil = lo * mth::exp(logratio * draw_canonical(rng));
     } while (draw_canonical(rng)
-             < epsil - (2 * (tau + 1) * epsil - 1) / (epsil * tau2 * tau2));
+                 < epsil - (2 * (tau + 1) * epsil - 1) / (epsil * tau2 * tau2)
+             && within_budget(iter, rng));
     double const cost = (epsil * tau2 - 1) / (epsil * mth::sqrt(tau * tau2));
     double const total = energy + 2 * me;
     double const eg = epsil * total;

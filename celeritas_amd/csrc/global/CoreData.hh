@@ -78,7 +78,9 @@ struct CoreCounters
     std::uint32_t error_flags;  //!< bit 0: initializer overflow, bit 1: stack
     std::uint32_t steps_taken;  //!< number of step iterations on device
     std::uint32_t num_new_neutral;  //!< neutral initializers this launch
-    std::uint32_t pad_[2];
+    //! Tracks failed because a sampling loop ran out of its iteration budget
+    std::uint32_t num_sampling_failures;
+    std::uint32_t pad_;
 };
 
 enum CoreErrorFlags : std::uint32_t
@@ -86,6 +88,9 @@ enum CoreErrorFlags : std::uint32_t
     err_initializer_overflow = 1u,
     err_secondary_overflow = 2u,
     err_event_overflow = 4u,
+    //! A rejection loop exceeded its iteration budget (the track failed;
+    //! not fatal to the run)
+    err_sampling_budget = 8u,
 };
 
 //---------------------------------------------------------------------------//

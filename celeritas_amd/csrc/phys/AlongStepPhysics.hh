@@ -63,11 +63,12 @@ CELER_FUNCTION unsigned int sample_poisson(double lambda, Engine& rng)
     {
         int k = 0;
         double p = mth::exp(lambda);
+        unsigned int iter = 0;
         do
         {
             ++k;
             p *= draw_canonical(rng);
-        } while (p > 1);
+        } while (p > 1 && within_budget(iter, rng));
         return static_cast<unsigned int>(k - 1);
     }
     Normal n{lambda, mth::sqrt(lambda)};
@@ -101,10 +102,11 @@ CELER_FUNCTION double sample_fast_urban(double mean, double stddev, Engine& rng)
         Normal n{mean, stddev};
         double const max_loss = 2 * mean;
         double r;
+        unsigned int iter = 0;
         do
         {
             r = n(rng);
-        } while (r <= 0 || r > max_loss);
+        } while ((r <= 0 || r > max_loss) && within_budget(iter, rng));
         return r;
     }
     return uniform(0.0, 2 * mean, rng);
@@ -205,7 +207,8 @@ CELER_FUNCTION double sample_urban_fluct(UrbanFluctuation const& params,
         {
             unsigned int n = sample_poisson(xs_ion - mean_num_coll, rng);
             double const a = alpha / energy_ratio;
-            for (; n > 0; --n)
+            unsigned int iter = 0;
+            for (; n > 0 && within_budget(iter, rng); --n)
                 ion += alpha * e0 / uniform(a, 1.0, rng);
         }
         result += ion;
@@ -590,12 +593,13 @@ CELER_FUNCTION MscResult msc_scatter(MscTrack const& t,
         double const a = (2 * xmean + 9 * x2mean - 3) / (2 * xmean - 3 * x2mean + 1);
         double const p_pow = (a + 2) * xmean / a;
         double r;
+        unsigned int iter = 0;
         do
         {
             double const rdm = draw_canonical(rng);
             bool const use_pow = draw_canonical(rng) < p_pow;
             r = 2 * (use_pow ? fastpow(rdm, 1 / (a + 1)) : rdm) - 1;
-        } while (std::fabs(r) > 1);
+        } while (std::fabs(r) > 1 && within_budget(iter, rng));
         return r;
     };
     double costheta;

@@ -73,12 +73,20 @@ calc_grid(XsGrid const& g, double const* __restrict__ reals, GridPoint& gp)
         gp.upper_e = mth::exp(g.loge_front + g.loge_delta * (gp.lower + 1));
         gp.lower_e = mth::exp(g.loge_front + g.loge_delta * gp.lower);
     }
-    std::uint32_t const lower = gp.lower;
-    double const upper_e = gp.upper_e;
+    std::uint32_t lower = gp.lower;
+    double upper_e = gp.upper_e;
+    double lower_e = gp.lower_e;
+    if (CELER_UNLIKELY(lower + 2 > g.size))
+    {
+        // A log a few ulps below the stored back can round to the last
+        // point: use the last interval
+        lower = g.size - 2;
+        upper_e = mth::exp(g.loge_front + g.loge_delta * (lower + 1));
+        lower_e = mth::exp(g.loge_front + g.loge_delta * lower);
+    }
     double upper_xs = v[lower + 1];
     if (lower + 1 == g.prime_index)
         upper_xs /= upper_e;
-    double const lower_e = gp.lower_e;
     double const lower_xs = v[lower];
     double const slope = ((-lower_xs) + upper_xs) / ((-lower_e) + upper_e);
     double result = std::fma(slope, (-lower_e) + energy, lower_xs);
@@ -167,7 +175,8 @@ calc_range(XsGrid const& g, double const* __restrict__ reals, double energy)
         return v[0] * mth::exp(0.5 * (loge - g.loge_front));
     if (loge >= g.loge_back)
         return v[g.size - 1];
-    auto const i = static_cast<std::uint32_t>((loge - g.loge_front) / g.loge_delta);
+    auto i = static_cast<std::uint32_t>((loge - g.loge_front) / g.loge_delta);
+    i = i + 2 > g.size ? g.size - 2 : i;
     double const xl = mth::exp(g.loge_front + g.loge_delta * i);
     double const xr = mth::exp(g.loge_front + g.loge_delta * (i + 1));
     double const slope = ((-v[i]) + v[i + 1]) / ((-xl) + xr);

@@ -54,6 +54,9 @@ class XorwowEngine
   public:
     std::uint32_t v[5];
     std::uint32_t d;
+    //! Set by a sampling loop that ran out of its iteration budget (not part
+    //! of the stored state)
+    bool exhausted{false};
 
     //! Load from SoA state
     CELER_FORCEINLINE_FUNCTION static XorwowEngine
@@ -222,6 +225,7 @@ struct ReplayEngine
 {
     double const* u;
     std::uint32_t i;
+    bool exhausted{false};
     CELER_FORCEINLINE_FUNCTION double next() { return u[i++]; }
 };
 

@@ -49,6 +49,25 @@ CELER_FORCEINLINE_FUNCTION void apply_errored(CoreParamsRef const& params,
 }
 
 /*!
+ * Fail a track whose sampling loop ran out of its iteration budget and
+ * record it: the error flag (not fatal to the run) and a count of such
+ * tracks in the device counters.
+ */
+CELER_FUNCTION void apply_sampling_failure(CoreParamsRef const& params,
+                                           CoreStateRef const& state,
+                                           size_type slot)
+{
+    apply_errored(params, state, slot);
+#if CELER_DEVICE_COMPILE
+    atomicOr(&state.counters->error_flags, static_cast<std::uint32_t>(err_sampling_budget));
+    atomicAdd(&state.counters->num_sampling_failures, 1u);
+#else
+    state.counters->error_flags |= err_sampling_budget;
+    state.counters->num_sampling_failures += 1;
+#endif
+}
+
+/*!
  * Initialize a track in a vacant slot from an initializer.
  *
  * Follows InitTracksExecutor (celeritas/track/detail/InitTracksExecutor.hh:

@@ -17,10 +17,8 @@
 //!
 //! Pinned by: the Urban fluctuation golden values of
 //! EnergyLossHelper.test.cc:238-285 (mean loss and the mt19937 draw count of
-//! 10,000 samples, through orc_urban_fluct_sample). The MSC pieces have no
-//! golden values the container can read (the reference's MSC tests load a
-//! ROOT file): parity unpinned beyond the GPU-vs-oracle comparisons and the
-//! invariants of tests/test_oracle_msc.py.
+//! 10,000 samples, through orc_urban_fluct_sample). The MSC pieces are in
+//! msc.cc, with their pins listed in msc.hh.
 //---------------------------------------------------------------------------//
 #include <algorithm>
 #include <cmath>
@@ -29,6 +27,7 @@
 
 #include "alongstep.hh"
 #include "em.hh"
+#include "msc.hh"
 #include "physics.hh"
 #include "problem.hh"
 
@@ -88,10 +87,6 @@ double uniform_real(double a, double b, U& u)
     return std::fma(b - a, u(), a);
 }
 
-double fastpow(double a, double b)
-{
-    return std::exp(b * std::log(a));
-}
 
 //---------------------------------------------------------------------------//
 // Urban fluctuations
@@ -111,35 +106,6 @@ double fast_urban(double mean, double stddev, U& u)
     }
     return uniform_real(0.0, 2 * mean, u);
 }
-
-//---------------------------------------------------------------------------//
-// MSC helpers
-struct MscCtx
-{
-    PhysicsData const& pd;
-    ProcessDef const& eloss;
-    std::uint32_t mat;
-    int par;  // 0 e-, 1 e+
-    double energy;
-    double range;
-    double lambda = 0;
-
-    double mfp(double e) const
-    {
-        return 1 / (pd.msc.xs[par][mat].calc(e) / (e * e));
-    }
-    double inverse_range(double r) const { return calc_inverse_range(eloss.range[mat], r); }
-    double end_energy(double step) const
-    {
-        if (step <= range * 0.05)
-            return energy - step * eloss.dedx[mat].calc(energy);
-        return inverse_range(range - step);
-    }
-};
-
-constexpr double geom_limit = 5e-9;
-constexpr double limit_min_fix = 1e-9;
-constexpr double min_step = 1e-7;
 
 }  // namespace
 
@@ -246,7 +212,7 @@ void along_step_charged_impl(State& s)
         ProcessDef const* eloss = ep >= 0 ? &pd.processes[procs[ep]] : nullptr;
 
         // --- MscStepLimitApplier / UrbanMsc::limit_step
-        bool const applicable = pd.msc.enabled && t.step_length > geom_limit
+        bool const applicable = pd.msc.enabled && t.step_length > msc_geom_limit
                                 && t.status == alive
                                 && (t.particle == pd.electron || t.particle == pd.positron)
                                 && t.energy > pd.msc.low && t.energy < pd.msc.high;
@@ -259,7 +225,7 @@ void along_step_charged_impl(State& s)
             ctx.lambda = ctx.mfp(t.energy);
             double const phys_step = t.step_length;
             bool have_path = false;
-            if (phys_step <= limit_min_fix)
+            if (phys_step <= msc_limit_min_fix)
             {
                 true_path = phys_step;
                 have_path = true;
@@ -296,7 +262,7 @@ void along_step_charged_impl(State& s)
                         xm *= 0.5 + 0.5 * e / 5e-3;
                     t.msc_range_init = init;
                     t.msc_range_factor = factor;
-                    t.msc_limit_min = std::max(xm, limit_min_fix);
+                    t.msc_limit_min = std::max(xm, msc_limit_min_fix);
                 }
                 double const lmin = t.msc_limit_min;
                 double limit = ctx.range;
@@ -320,37 +286,8 @@ void along_step_charged_impl(State& s)
             }
             bool limited = true_path < phys_step;
             // MscStepToGeo
-            {
-                double step;
-                alpha = 0;
-                if (true_path < min_step)
-                {
-                    step = true_path;
-                }
-                else if (true_path < ctx.range * 0.05)
-                {
-                    step = -ctx.lambda * std::expm1(-true_path / ctx.lambda);
-                }
-                else
-                {
-                    double slope;
-                    if (t.energy < pd.particles[pd.electron].mass || true_path == ctx.range)
-                    {
-                        alpha = 1 / ctx.range;
-                        slope = std::max(1 - alpha * true_path, 0.0);
-                    }
-                    else
-                    {
-                        double const e1 = ctx.inverse_range(ctx.range - true_path);
-                        double const lambda1 = ctx.mfp(e1);
-                        alpha = (ctx.lambda - lambda1) / (ctx.lambda * true_path);
-                        slope = lambda1 / ctx.lambda;
-                    }
-                    double const w = 1 + 1 / (alpha * ctx.lambda);
-                    step = (1 - fastpow(slope, w)) / (alpha * w);
-                }
-                geom_path = std::min(step, true_path);
-            }
+            geom_path = msc_step_to_geo(ctx, pd.particles[pd.electron].mass, true_path,
+                                        &alpha);
             if (geom_path > ctx.lambda)
             {
                 geom_path = ctx.lambda;
@@ -428,28 +365,7 @@ void along_step_charged_impl(State& s)
             {
                 // MscStepFromGeo
                 geom_path = t.step_length;
-                double const gstep = geom_path;
-                double tstep;
-                if (gstep < min_step)
-                {
-                    tstep = gstep;
-                }
-                else
-                {
-                    if (alpha == 0)
-                    {
-                        tstep = -ctx.lambda * std::log1p(-gstep / ctx.lambda);
-                        if (tstep < min_step)
-                            tstep = gstep;
-                    }
-                    else
-                    {
-                        double const w = 1 + 1 / (alpha * ctx.lambda);
-                        double const x = std::min(alpha * w * gstep, 1.0);
-                        tstep = std::min((1 - fastpow(1 - x, 1 / w)) / alpha, ctx.range);
-                    }
-                    tstep = std::clamp(tstep, gstep, true_path);
-                }
+                double const tstep = msc_step_from_geo(ctx, true_path, alpha, geom_path);
                 true_path = tstep;
                 displaced = false;
             }
@@ -458,7 +374,7 @@ void along_step_charged_impl(State& s)
             if (displaced)
             {
                 double displ = 0.73 * std::sqrt((true_path - geom_path) * (true_path + geom_path));
-                displ = std::max(displ * (1 + 2 * 0.01), geom_limit);
+                displ = std::max(displ * (1 + 2 * 0.01), msc_geom_limit);
                 safety = std::min(geo_find_safety(prob.geo, t.geo), displ);
                 if (safety == 0)
                     displaced = false;
@@ -470,7 +386,7 @@ void along_step_charged_impl(State& s)
             double e_end = 0, tau = 0, xmean = 0, x2mean = 0, theta0 = 0;
             double lmin = t.msc_limit_min;
             bool skip = false;
-            if (true_path == ctx.range || true_path < geom_limit)
+            if (true_path == ctx.range || true_path < msc_geom_limit)
             {
                 skip = true;
             }
@@ -500,34 +416,9 @@ void along_step_charged_impl(State& s)
                     if (t.particle == pd.positron)
                     {
                         // UrbanPositronCorrector
-                        double const z = m.zeff;
-                        double const a = std::fma(z, -4.08e-3, 0.994);
-                        double const b = std::fma(1 / z, std::fma(1 / z, 365.0, 52.6), 7.16);
-                        double const c = std::fma(z, -4.47e-3, 1.0);
-                        double const d = 1.21e-3 * z;
-                        double const mult
-                            = std::fma(z, std::fma(z, 1.84035e-4, -1.86427e-2), 1.41125);
-                        double const yy = std::sqrt(e_in * e_end) / me;
-                        double const x = std::sqrt(yy * (yy + 2) / ((yy + 1) * (yy + 1)));
-                        double corr;
-                        if (x < 0.6)
-                            corr = a * (1 - std::exp(-b * x));
-                        else if (x > 0.9)
-                            corr = c + d * std::exp(113 * (x - 1));
-                        else
-                        {
-                            double const yl = a * (1 - std::exp(-b * 0.6));
-                            double const yh = c + d * std::exp(113 * (0.9 - 1));
-                            double const y0 = (yh - yl) / (0.9 - 0.6);
-                            corr = y0 * x + (yl - y0 * 0.6);
-                        }
-                        y *= corr * mult;
+                        y *= urban_positron_correction(m.zeff, std::sqrt(e_in * e_end) / me);
                     }
-                    double const invbetacp = std::sqrt(
-                        (e_in + me) * (e_end + me)
-                        / (e_in * (e_in + 2 * me) * e_end * (e_end + 2 * me)));
-                    theta0 = 13.6 * std::sqrt(y) * invbetacp;
-                    theta0 *= std::fma(std::log(y), md.theta[1], md.theta[0]);
+                    theta0 = msc_theta0(md, y, e_in, e_end, me);
                     if (true_path < lmin)
                         theta0 *= std::sqrt(true_path / lmin);
                     theta0 = std::max(theta0, 0.0);
@@ -542,17 +433,6 @@ void along_step_charged_impl(State& s)
             }
             if (!skip)
             {
-                auto simple = [&]() {
-                    double const a = (2 * xmean + 9 * x2mean - 3) / (2 * xmean - 3 * x2mean + 1);
-                    double const ppow = (a + 2) * xmean / a;
-                    double r;
-                    do
-                    {
-                        double const rdm = u();
-                        r = 2 * (u() < ppow ? fastpow(rdm, 1 / (a + 1)) : rdm) - 1;
-                    } while (std::fabs(r) > 1);
-                    return r;
-                };
                 double cost;
                 if (theta0 <= 0)
                 {
@@ -564,60 +444,13 @@ void along_step_charged_impl(State& s)
                 }
                 else if (2 * e_end < e_in || theta0 > pi / 6)
                 {
-                    cost = simple();
+                    cost = msc_simple_scattering(xmean, x2mean, u);
                 }
                 else
                 {
-                    double const maxtau = true_path < lmin ? lmin / ctx.lambda : tau;
-                    double const uu = fastpow(maxtau, 1.0 / 6);
-                    double const radlen_mfp = true_path / (tau * m.rad_length);
-                    double xsi = std::fma(uu, std::fma(uu, md.tail[2], md.tail[1]), md.tail[0])
-                                 + md.tail_corr * std::log(radlen_mfp);
-                    xsi = std::max(xsi, 1.9);
-                    double const ea = std::exp(-xsi);
-                    double const sh = 2 * std::sin(0.5 * theta0);
-                    double const x = sh * sh;
-                    double const xmean_1 = 1 - x * (1 + (xsi * ea) / (1 - ea));
-                    if (xmean_1 <= 0.999 * xmean)
-                    {
-                        cost = simple();
-                    }
-                    else
-                    {
-                        double c = xsi;
-                        if (std::fabs(xsi - 3) < 0.001)
-                            c = 3.001;
-                        else if (std::fabs(xsi - 2) < 0.001)
-                            c = 2.001;
-                        double const b1 = 2 + (c - xsi) * x;
-                        double const d = fastpow(c * x / b1, c - 1);
-                        double const x0 = 1 - xsi * x;
-                        double const xmean_2 = (x0 + d - (c * x - b1 * d) / (c - 2)) / (1 - d);
-                        double const f2x0 = (c - 1) / (c * (1 - d));
-                        double const prob = f2x0 / (ea / (1 - ea) + f2x0);
-                        double const qprob = xmean / (prob * xmean_1 + (1 - prob) * xmean_2);
-                        if (u() >= qprob)
-                        {
-                            cost = uniform_real(-1.0, 1.0, u);
-                        }
-                        else if (u() < prob)
-                        {
-                            cost = 1 + std::log(uniform_real(ea, 1.0, u)) * x;
-                        }
-                        else
-                        {
-                            double var = (1 - d) * u();
-                            if (var < 0.01 * d)
-                            {
-                                var /= (d * (c - 1));
-                                cost = -1 + var * (1 - 0.5 * var * c) * (2 + (c - xsi) * x);
-                            }
-                            else
-                            {
-                                cost = x * (c - xsi - c * fastpow(var + d, -1 / (c - 1))) + 1;
-                            }
-                        }
-                    }
+                    MscAngleInput const in{xmean, x2mean, tau, theta0, true_path,
+                                           lmin, ctx.lambda, m.rad_length, &md};
+                    cost = msc_sample_cos_theta(in, u);
                 }
                 double phi = uniform_real(0.0, 2 * pi, u);
                 bool moved = false;
@@ -626,7 +459,7 @@ void along_step_charged_impl(State& s)
                 {
                     double length = 0.73 * std::sqrt((true_path - geom_path) * (true_path + geom_path));
                     length = std::min(length, (1 - 0.01) * safety);
-                    if (length >= geom_limit)
+                    if (length >= msc_geom_limit)
                     {
                         double const psi = -std::log(1 - u() * 0.9988703417569197) / 2.160;
                         double const dphi = u() < 0.5 ? phi + psi : phi - psi;

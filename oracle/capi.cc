@@ -855,3 +855,170 @@ int orc_field_propagate(void* problem, double const* pos, double const* dir,
     });
 }
 }
+
+//---------------------------------------------------------------------------//
+// URBAN MSC PIECES (msc.hh)
+//---------------------------------------------------------------------------//
+#include "msc.hh"
+
+extern "C" {
+//! Material coefficients for an effective Z: out = {stepmin[2], theta[2],
+//! tail[3], tail_corr, scaled_zeff[2], d_over_r}
+void orc_msc_material_coeffs(double zeff, double* out)
+{
+    MscDataO::Material const d = msc_material_coeffs(zeff);
+    double const v[] = {d.stepmin[0], d.stepmin[1], d.theta[0], d.theta[1],
+                        d.tail[0],    d.tail[1],    d.tail[2],  d.tail_corr,
+                        d.scaled_zeff[0], d.scaled_zeff[1], d.d_over_r};
+    std::copy(std::begin(v), std::end(v), out);
+}
+
+double orc_urban_positron_correction(double zeff, double y)
+{
+    return urban_positron_correction(zeff, y);
+}
+
+//! MSC helper quantities and step conversions of an e- (particle 0) or e+
+//! (1) at an energy in a material: out_info = {range, lambda, geometric
+//! path of tstep, alpha, end energy of tstep}; true_out[i] = true path of
+//! the geometric path gsteps[i] (true path limited by tstep)
+int orc_msc_step_conversion(void* problem, std::uint32_t material, int par, double energy,
+                            double tstep, double const* gsteps, std::uint32_t n,
+                            double* out_info, double* true_out)
+{
+    return guard([&] {
+        auto const& pd = *static_cast<OrcProblem*>(problem)->p->physics;
+        if (!pd.msc.enabled)
+            throw std::runtime_error("MSC is off");
+        std::uint32_t const pid = par == 0 ? pd.electron : pd.positron;
+        int const ep = pd.eloss_process.at(pid);
+        if (ep < 0)
+            throw std::runtime_error("no energy-loss process");
+        ProcessDef const& eloss = pd.processes.at(pd.particle_processes.at(pid).at(ep));
+        MscCtx ctx{pd, eloss, material, par, energy, eloss.range.at(material).calc(energy)};
+        ctx.lambda = ctx.mfp(energy);
+        double alpha = 0;
+        double const me = pd.particles.at(pd.electron).mass;
+        double const gp = msc_step_to_geo(ctx, me, tstep, &alpha);
+        out_info[0] = ctx.range;
+        out_info[1] = ctx.lambda;
+        out_info[2] = gp;
+        out_info[3] = alpha;
+        out_info[4] = tstep < ctx.range ? ctx.end_energy(tstep) : 0;
+        for (std::uint32_t i = 0; i < n; ++i)
+            true_out[i] = msc_step_from_geo(ctx, tstep, alpha, gsteps[i]);
+    });
+}
+
+//! n samples of simple_scattering on a default-seeded mt19937
+void orc_msc_simple_scattering(double xmean, double x2mean, std::uint32_t n, double* out)
+{
+    std::mt19937 mt;
+    MtUniform u{mt};
+    for (std::uint32_t i = 0; i < n; ++i)
+        out[i] = msc_simple_scattering(xmean, x2mean, u);
+}
+
+//! n samples of sample_cos_theta on a default-seeded mt19937, for the
+//! coefficients of an effective Z; in = {tau, theta0, true_path,
+//! limit_min, lambda, rad_length}
+void orc_msc_sample_cos_theta(double zeff, double const* in, std::uint32_t n, double* out)
+{
+    MscDataO::Material const md = msc_material_coeffs(zeff);
+    double const tau = in[0];
+    MscAngleInput const a{std::exp(-tau), (1 + 2 * std::exp(-2.5 * tau)) / 3, tau, in[1],
+                          in[2], in[3], in[4], in[5], &md};
+    std::mt19937 mt;
+    MtUniform u{mt};
+    for (std::uint32_t i = 0; i < n; ++i)
+        out[i] = msc_sample_cos_theta(a, u);
+}
+
+double orc_msc_theta0(double zeff, double y, double e_in, double e_end, double me)
+{
+    MscDataO::Material const md = msc_material_coeffs(zeff);
+    return msc_theta0(md, y, e_in, e_end, me);
+}
+}
+
+extern "C" {
+//! Parameters of the tail model for the same inputs as
+//! orc_msc_sample_cos_theta: out = {xsi, x, c, d, prob, qprob, xmean_1,
+//! xmean_2, ea}
+void orc_msc_tail_parts(double zeff, double const* in, double* out)
+{
+    MscDataO::Material const md = msc_material_coeffs(zeff);
+    double const tau = in[0], theta0 = in[1], true_path = in[2], limit_min = in[3];
+    double const lambda = in[4], rad_length = in[5];
+    double const xmean = std::exp(-tau);
+    double const maxtau = true_path < limit_min ? limit_min / lambda : tau;
+    double const uu = detail::fastpow(maxtau, 1.0 / 6);
+    double const radlen_mfp = true_path / (tau * rad_length);
+    double xsi = std::fma(uu, std::fma(uu, md.tail[2], md.tail[1]), md.tail[0])
+                 + md.tail_corr * std::log(radlen_mfp);
+    xsi = std::max(xsi, 1.9);
+    double const ea = std::exp(-xsi);
+    double const sh = 2 * std::sin(0.5 * theta0);
+    double const x = sh * sh;
+    double const xmean_1 = 1 - x * (1 + (xsi * ea) / (1 - ea));
+    double c = xsi;
+    if (std::fabs(xsi - 3) < 0.001)
+        c = 3.001;
+    else if (std::fabs(xsi - 2) < 0.001)
+        c = 2.001;
+    double const b1 = 2 + (c - xsi) * x;
+    double const d = detail::fastpow(c * x / b1, c - 1);
+    double const x0 = 1 - xsi * x;
+    double const xmean_2 = (x0 + d - (c * x - b1 * d) / (c - 2)) / (1 - d);
+    double const f2x0 = (c - 1) / (c * (1 - d));
+    double const prob = f2x0 / (ea / (1 - ea) + f2x0);
+    double const qprob = xmean / (prob * xmean_1 + (1 - prob) * xmean_2);
+    double const v[] = {xsi, x, c, d, prob, qprob, xmean_1, xmean_2, ea};
+    std::copy(std::begin(v), std::end(v), out);
+}
+}
+
+extern "C" {
+//! Evaluate a log-uniform grid built from (emin, emax, size, prime index,
+//! values): kind 0 = cross section (XsCalculator), 1 = range
+//! (RangeCalculator), 2 = inverse range (InverseRangeCalculator, x = range)
+void orc_grid_eval(double emin, double emax, std::uint32_t size, std::uint32_t prime,
+                   double const* values, int kind, double const* x, std::uint32_t n,
+                   double* out)
+{
+    XsGridO g;
+    g.front = std::log(emin);
+    g.back = std::log(emax);
+    g.size = size;
+    g.delta = (g.back - g.front) / (size - 1);
+    g.prime = prime;
+    g.values.assign(values, values + size);
+    for (std::uint32_t i = 0; i < n; ++i)
+    {
+        out[i] = kind == 0   ? g.calc(x[i])
+                 : kind == 1 ? calc_range(g, x[i])
+                             : calc_inverse_range(g, x[i]);
+    }
+}
+
+//! Range and the inverse range of that range for a particle's energy-loss
+//! tables in a material
+int orc_range_roundtrip(void* problem, std::uint32_t particle, std::uint32_t material,
+                        double const* energies, std::uint32_t n, double* range_out,
+                        double* energy_out)
+{
+    return guard([&] {
+        auto const& pd = *static_cast<OrcProblem*>(problem)->p->physics;
+        int const ep = pd.eloss_process.at(particle);
+        if (ep < 0)
+            throw std::runtime_error("no energy-loss process");
+        ProcessDef const& eloss = pd.processes.at(pd.particle_processes.at(particle).at(ep));
+        XsGridO const& g = eloss.range.at(material);
+        for (std::uint32_t i = 0; i < n; ++i)
+        {
+            range_out[i] = calc_range(g, energies[i]);
+            energy_out[i] = calc_inverse_range(g, range_out[i]);
+        }
+    });
+}
+}

@@ -2,6 +2,7 @@
 // ORACLE — CPU restatement used only as a checker (see physics.hh).
 //---------------------------------------------------------------------------//
 #include "physics.hh"
+#include "msc.hh"
 
 #include <algorithm>
 #include <cmath>
@@ -183,6 +184,8 @@ double XsGridO::calc(double e) const
     if (loge >= back)
         return extrapolated(size - 1);
     auto lower = static_cast<std::uint32_t>((loge - front) / delta);
+    // A log a few ulps below the stored back can round to the last point
+    lower = std::min(lower, size - 2);
     double upper_e = std::exp(front + delta * (lower + 1));
     double upper_xs = values[lower + 1];
     if (lower + 1 == prime)
@@ -311,28 +314,9 @@ void build_physics(Json const& inp, PhysicsData& out)
         msc.xs[1] = grids_of(*mj, "positron", nullptr, nullptr, num_mats);
         msc.low = std::exp(msc.xs[0].at(0).front);
         msc.high = std::exp(msc.xs[0].at(0).back);
-        auto poly2 = [](double c0, double c1, double c2, double x) {
-            return std::fma(x, std::fma(x, c2, c1), c0);
-        };
-        auto fastpow = [](double a, double b) { return std::exp(b * std::log(a)); };
         for (auto const& m : out.materials)
         {
-            MscDataO::Material d;
-            double const zeff = m.zeff;
-            d.stepmin[0] = 1e3 * 27.725 / (1 + 0.203 * zeff);
-            d.stepmin[1] = 1e3 * 6.152 / (1 + 0.111 * zeff);
-            double const z16 = fastpow(zeff, 1.0 / 6.0);
-            double const fz = poly2(0.990395, -0.168386, 0.093286, z16);
-            d.theta[0] = fz * (1 - 8.7780e-2 / zeff);
-            d.theta[1] = fz * (4.0780e-2 + 1.7315e-4 * zeff);
-            double const z13 = z16 * z16;
-            d.tail[0] = poly2(2.3785, -4.1981e-1, 6.3100e-2, z13);
-            d.tail[1] = poly2(4.7526e-1, 1.7694, -3.3885e-1, z13);
-            d.tail[2] = poly2(2.3683e-1, -1.8111, 3.2774e-1, z13);
-            d.tail_corr = poly2(1.7888e-2, 1.9659e-2, -2.6664e-3, z13);
-            d.scaled_zeff[0] = 0.87 * fastpow(zeff, 2.0 / 3);
-            d.scaled_zeff[1] = 0.70 * fastpow(zeff, 1.0 / 2);
-            d.d_over_r = 9.6280e-1 - 8.4848e-2 * std::sqrt(zeff) + 4.3769e-3 * zeff;
+            MscDataO::Material const d = msc_material_coeffs(m.zeff);
             msc.mat.push_back(d);
         }
     }
@@ -775,7 +759,7 @@ double calc_range(XsGridO const& g, double energy)
         return g.values[0] * std::exp(0.5 * (loge - g.front));
     if (loge >= g.back)
         return g.values[g.size - 1];
-    auto const i = static_cast<std::uint32_t>((loge - g.front) / g.delta);
+    auto const i = std::min(static_cast<std::uint32_t>((loge - g.front) / g.delta), g.size - 2);
     double const xl = std::exp(g.front + g.delta * i);
     double const xr = std::exp(g.front + g.delta * (i + 1));
     double const slope = ((-g.values[i]) + g.values[i + 1]) / ((-xl) + xr);

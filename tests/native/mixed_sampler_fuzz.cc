@@ -40,6 +40,7 @@ struct CountingEngine
 {
     XorwowEngine e;
     unsigned long n{0}, limit{1000000};
+    bool exhausted{false};  // set by a loop that ran out of its iteration budget
     std::uint32_t operator()() { if (++n > limit) throw Limit("draw limit"); return e(); }
 };
 static char g_case[512];
@@ -111,6 +112,7 @@ int main(int argc, char** argv)
                         for (std::uint32_t j = 0; j < r.num_secondaries; ++j) { Secondary const& q = storage[r.sec_start + j]; std::printf("   sec %u: pid %u E %.9g dir %.9g %.9g %.9g |d|^2-1 %.3g\n", j, q.particle_id, q.energy, q.dir[0], q.dir[1], q.dir[2], q.dir[0]*q.dir[0]+q.dir[1]*q.dir[1]+q.dir[2]*q.dir[2]-1); } } }
                 }
                 catch (Limit const&) { ++fails; std::printf("LOOP: %s\n", g_case); }
+                if (rng.exhausted) { ++fails; rng.exhausted = false; std::printf("BUDGET: %s\n", g_case); }
             }
         }
         // Along-step physics of charged particles
@@ -161,6 +163,7 @@ int main(int argc, char** argv)
             if (nanmode >= 0.3 && !(dep >= 0 && dep <= energy)) { ++nbad; if (nbad < 20) std::printf("BAD ELOSS: %s -> %.9g\n", g_case, dep); }
         }
         catch (Limit const&) { ++fails; std::printf("LOOP: %s\n", g_case); }
+        if (rng.exhausted) { ++fails; rng.exhausted = false; std::printf("BUDGET: %s\n", g_case); }
     }
     std::printf("calls %ld loops %ld bad %ld failed %ld\n", calls, fails, nbad, nfailed);
     return fails != 0;

@@ -377,3 +377,76 @@ def field_propagate(problem: "Problem", pos, direction, momentum: float, charge:
     _check(f(problem.handle, p.ctypes.data, d.ctypes.data, momentum, charge, b.ctypes.data,
              max_substeps, s.ctypes.data, len(s), out.ctypes.data))
     return out
+
+
+# --------------------------------------------------------------------------- #
+# Urban MSC pieces (oracle/msc.hh)
+# --------------------------------------------------------------------------- #
+def msc_set_mutation(m: int) -> None:
+    """Plant a deliberate mistake in the oracle's MSC code (0: none)."""
+    f = load().orc_msc_set_mutation
+    f.argtypes = [ctypes.c_int]
+    f(m)
+
+
+def msc_material_coeffs(zeff: float) -> dict:
+    f = load().orc_msc_material_coeffs
+    f.argtypes = [ctypes.c_double, ctypes.c_void_p]
+    out = np.zeros(11)
+    f(zeff, out.ctypes.data)
+    keys = ("stepmin0", "stepmin1", "theta0", "theta1", "tail0", "tail1", "tail2",
+            "tail_corr", "scaled_zeff_e", "scaled_zeff_p", "d_over_r")
+    return dict(zip(keys, out.tolist()))
+
+
+def urban_positron_correction(zeff: float, y: float) -> float:
+    f = load().orc_urban_positron_correction
+    f.argtypes = [ctypes.c_double, ctypes.c_double]
+    f.restype = ctypes.c_double
+    return f(zeff, y)
+
+
+def msc_step_conversion(problem: "Problem", material: int, par: int, energy: float,
+                        tstep: float, gsteps) -> tuple:
+    """(range, lambda, geometric path, alpha, end energy), true paths of gsteps."""
+    f = load().orc_msc_step_conversion
+    f.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, ctypes.c_double,
+                  ctypes.c_double, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+                  ctypes.c_void_p]
+    f.restype = ctypes.c_int
+    g = np.ascontiguousarray(gsteps, dtype=np.float64)
+    info = np.zeros(5)
+    out = np.zeros(len(g))
+    _check(f(problem.handle, material, par, energy, tstep, g.ctypes.data, len(g),
+             info.ctypes.data, out.ctypes.data))
+    return tuple(info.tolist()), out
+
+
+def msc_simple_scattering(xmean: float, x2mean: float, n: int) -> np.ndarray:
+    f = load().orc_msc_simple_scattering
+    f.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_uint32, ctypes.c_void_p]
+    out = np.zeros(n)
+    f(xmean, x2mean, n, out.ctypes.data)
+    return out
+
+
+def msc_sample_cos_theta(zeff: float, tau: float, theta0: float, true_path: float,
+                         limit_min: float, lam: float, rad_length: float,
+                         n: int) -> np.ndarray:
+    f = load().orc_msc_sample_cos_theta
+    f.argtypes = [ctypes.c_double, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]
+    inp = np.array([tau, theta0, true_path, limit_min, lam, rad_length])
+    out = np.zeros(n)
+    f(zeff, inp.ctypes.data, n, out.ctypes.data)
+    return out
+
+
+def msc_tail_parts(zeff: float, tau: float, theta0: float, true_path: float,
+                   limit_min: float, lam: float, rad_length: float) -> dict:
+    f = load().orc_msc_tail_parts
+    f.argtypes = [ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
+    inp = np.array([tau, theta0, true_path, limit_min, lam, rad_length])
+    out = np.zeros(9)
+    f(zeff, inp.ctypes.data, out.ctypes.data)
+    keys = ("xsi", "x", "c", "d", "prob", "qprob", "xmean_1", "xmean_2", "ea")
+    return dict(zip(keys, out.tolist()))

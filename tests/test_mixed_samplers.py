@@ -2,13 +2,17 @@
 end and no sampler returns a non-finite or out-of-range result.
 
 The physics headers are compiled for the host in the mixed mode (FP32
-transcendentals and 24-bit draws, as on the device) together with the
+transcendentals and 23-bit draws, as on the device) together with the
 table builder, and driven over random inputs of the TestEm3 physics by
-``tests/native/mixed_sampler_fuzz.cc``. Longer runs of the same program
+``tests/native/mixed_sampler_fuzz.cc``; a loop that runs out of its
+iteration budget counts as a failure. Longer runs of the same program
 (13 million sampler calls over four seeds) found one defect of the FP32
 arithmetic, now fixed: a cosine rounded just outside [-1, 1] gave a NaN
 direction (in-flight annihilation at GeV energies, about once in 3 million
-calls); this test is the short form that runs with the CPU suite.
+calls); this test is the short form that runs with the CPU suite. The
+samplers are only driven inside their domains (a process is sampled only
+where its cross section is positive); the whole-loop check that also
+covers the selection is tests/test_mixed_host_loop.py.
 """
 
 import json
