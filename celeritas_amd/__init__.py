@@ -69,9 +69,9 @@ class CCounters(ctypes.Structure):
             "steps_taken",
             "num_new_neutral",
             "num_sampling_failures",
-            "pad0",
+            "num_completed",
         )
-    ]
+    ] + [("track_steps", ctypes.c_uint64)]
 
 
 # numpy dtypes matching the C structs

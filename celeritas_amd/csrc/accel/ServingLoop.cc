@@ -61,10 +61,13 @@ ServeResult ServingLoop::run(Primary const* primaries,
                                   init.max_events * sizeof(std::int32_t), state.stream()));
     CELER_HIP_CALL(hipMemsetAsync(init.track_counters, 0,
                                   init.max_events * sizeof(std::uint32_t), state.stream()));
+    CELER_HIP_CALL(hipMemsetAsync(&state.ref().counters->num_completed, 0,
+                                  sizeof(std::uint32_t), state.stream()));
     size_type const max_queued = options.max_queued > 0 ? options.max_queued
                                                         : state.size();
     CELER_VALIDATE(max_queued <= params_->init_capacity(),
                    << "max_queued exceeds the initializer capacity");
+    size_type const steps_per_sync = std::max<size_type>(options.steps_per_sync, 1);
 
     ServeResult result;
     result.admit_time.assign(num_events, -1);
@@ -78,10 +81,12 @@ ServeResult ServingLoop::run(Primary const* primaries,
     };
 
     std::vector<Primary> batch;
-    std::vector<std::int32_t> live;
+    std::vector<std::uint32_t> done_ids;
     size_type next = 0;  // next event to admit
-    size_type oldest = 0;  // oldest event not yet complete
     size_type completed = 0;
+    std::uint32_t seen = 0;  // completions already read
+    std::uint32_t const steps0 = state.counters().steps_taken;
+    unsigned long long const track_steps0 = state.counters().track_steps;
     StepperResult last;
     while (completed < num_events)
     {
@@ -121,36 +126,42 @@ ServeResult ServingLoop::run(Primary const* primaries,
                            << " events unfinished");
             break;
         }
-        last = batch.empty() ? (*stepper_)()
-                             : (*stepper_)(batch.data(), static_cast<size_type>(batch.size()));
-        ++result.steps;
-        result.track_steps += last.active;
+        if (!batch.empty())
+        {
+            last = (*stepper_)(batch.data(), static_cast<size_type>(batch.size()));
+        }
+        else
+        {
+            // Run ahead while no arrived event waits for admission
+            bool const waiting = next < num_events && events[next].arrival <= elapsed();
+            last = stepper_->run_steps(waiting ? 1 : steps_per_sync);
+        }
+        CoreCounters const& c = state.counters();
+        result.steps = c.steps_taken - steps0;
         CELER_VALIDATE(options.max_steps == 0 || result.steps < options.max_steps,
                        << "serving loop exceeded " << options.max_steps << " steps");
 
-        // Completion: live counts of the in-flight window
-        if (next > oldest)
+        // New completions recorded by the device
+        std::uint32_t const total = std::min<std::uint32_t>(c.num_completed, num_events);
+        if (total > seen)
         {
-            size_type const n = next - oldest;
-            live.resize(n);
-            CELER_HIP_CALL(hipMemcpyAsync(live.data(), state.ref().init.event_live + oldest,
-                                          n * sizeof(std::int32_t), hipMemcpyDeviceToHost,
-                                          state.stream()));
+            done_ids.resize(total - seen);
+            CELER_HIP_CALL(hipMemcpyAsync(done_ids.data(), state.ref().init.completed + seen,
+                                          done_ids.size() * sizeof(std::uint32_t),
+                                          hipMemcpyDeviceToHost, state.stream()));
             CELER_HIP_CALL(hipStreamSynchronize(state.stream()));
             double const t = elapsed();
-            for (size_type i = 0; i < n; ++i)
+            for (std::uint32_t e : done_ids)
             {
-                size_type const e = oldest + i;
-                if (result.done_time[e] < 0 && live[i] == 0)
-                {
-                    result.done_time[e] = t;
-                    ++completed;
-                }
+                CELER_VALIDATE(e < num_events && result.done_time[e] < 0,
+                               << "event " << e << " completed twice");
+                result.done_time[e] = t;
+                ++completed;
             }
-            while (oldest < next && result.done_time[oldest] >= 0)
-                ++oldest;
+            seen = total;
         }
     }
+    result.track_steps = state.counters().track_steps - track_steps0;
     result.wall_time = elapsed();
     if (num_events > 0)
     {

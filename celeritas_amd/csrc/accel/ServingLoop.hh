@@ -32,6 +32,9 @@ struct ServeOptions
     size_type max_queued{0};
     //! Abort after this many step iterations (0: no limit)
     size_type max_steps{0};
+    //! Step iterations launched per counter readback while no event is
+    //! waiting to be admitted (the device runs them back to back)
+    size_type steps_per_sync{4};
 };
 
 //! Per-event outcome and run totals
@@ -51,11 +54,14 @@ struct ServeResult
  * Events are admitted in arrival order once they have arrived (wall clock)
  * and while the initializer queue has room, so requests of uneven size
  * share the device with those already in flight: their primaries join the
- * next step, and their tracks fill slots as others free them. Each step
- * ends with the device-resident counters; the per-event live-track counts
- * (initializers plus tracks, kept by the track-management kernels) of the
- * in-flight window are read back with them, and an event whose count has
- * dropped to zero is complete. Latency is completion minus arrival.
+ * next step, and their tracks fill slots as others free them. Completion
+ * is detected on the device: the track-management kernels keep a live
+ * count per event (initializers plus tracks) and append an event to a
+ * completion list when its count reaches zero. While no arrived event is
+ * waiting, the loop launches several steps (graph replays) back to back
+ * and reads only the counters and the new completions afterwards; an
+ * event's completion time is that of the readback that first shows it.
+ * Latency is completion minus arrival.
  */
 class ServingLoop
 {

@@ -56,6 +56,7 @@ struct CelerServer
     std::shared_ptr<CoreParams> params;
     std::unique_ptr<ServingLoop> loop;
     CelerState state_handle;
+    uint32_t steps_per_sync{4};
 };
 
 namespace
@@ -503,6 +504,14 @@ CelerState* celer_server_state(CelerServer* s)
     return &s->state_handle;
 }
 
+int celer_server_set_steps_per_sync(CelerServer* s, uint32_t steps)
+{
+    return guarded([&] {
+        CELER_VALIDATE(steps > 0, << "steps per readback must be positive");
+        s->steps_per_sync = steps;
+    });
+}
+
 int celer_server_run(CelerServer* s,
                      const CelerPrimary* primaries,
                      const uint32_t* event_offsets,
@@ -528,6 +537,7 @@ int celer_server_run(CelerServer* s,
         ServeOptions opts;
         opts.max_queued = max_queued;
         opts.max_steps = max_steps;
+        opts.steps_per_sync = s->steps_per_sync;
         ServeResult r = s->loop->run(reinterpret_cast<Primary const*>(primaries),
                                      events.data(), num_events, opts);
         std::copy(r.admit_time.begin(), r.admit_time.end(), admit_times);

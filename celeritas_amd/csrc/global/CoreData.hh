@@ -76,11 +76,14 @@ struct CoreCounters
     std::uint32_t prev_secondaries;  //!< secondaries at the back of inits
     std::uint32_t secondary_stack_size;  //!< allocated secondaries
     std::uint32_t error_flags;  //!< bit 0: initializer overflow, bit 1: stack
-    std::uint32_t steps_taken;  //!< number of step iterations on device
+    std::uint32_t steps_taken;  //!< step iterations run on the device
     std::uint32_t num_new_neutral;  //!< neutral initializers this launch
     //! Tracks failed because a sampling loop ran out of its iteration budget
     std::uint32_t num_sampling_failures;
-    std::uint32_t pad_;
+    //! Events completed on the device (entries of InitStateRef::completed)
+    std::uint32_t num_completed;
+    //! Sum over step iterations of the active tracks (track-steps)
+    unsigned long long track_steps;
 };
 
 enum CoreErrorFlags : std::uint32_t
@@ -127,6 +130,8 @@ struct InitStateRef
     //! Per event: initializers plus live tracks (0 once the event is done)
     std::int32_t* event_live{nullptr};
     double* event_edep{nullptr};  //!< per event: deposited energy [MeV]
+    //! Events in the order their live count reached zero (num_completed)
+    std::uint32_t* completed{nullptr};
     TrackInitializer* initializers{nullptr};
     std::uint32_t* block_scratch{nullptr};  //!< compaction partials
     size_type capacity{0};

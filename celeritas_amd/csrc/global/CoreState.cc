@@ -94,6 +94,7 @@ CoreState::CoreState(CoreParams const& params,
     add(&s.init.track_counters, s.init.max_events);
     add(&s.init.event_live, s.init.max_events);
     add(&s.init.event_edep, s.init.max_events);
+    add(&s.init.completed, s.init.max_events);
     add(&s.init.initializers, s.init.capacity);
     add(&s.init.block_scratch, extend_block_scratch_size(num_slots) + 64);
     double const stack_factor

@@ -158,11 +158,8 @@ StepperResult Stepper::operator()()
     return this->step_impl(false);
 }
 
-StepperResult Stepper::step_impl(bool new_primaries)
+void Stepper::launch_step(bool new_primaries)
 {
-    // Generated count is per step (Stepper.cc:125-139)
-    CELER_HIP_CALL(hipMemsetAsync(&state_->ref().counters->num_generated, 0,
-                                  sizeof(std::uint32_t), state_->stream()));
     if (use_graph_ && !new_primaries && !action_times_ && !state_->warming_up())
     {
         if (!graph_exec_)
@@ -173,6 +170,32 @@ StepperResult Stepper::step_impl(bool new_primaries)
     {
         this->enqueue_step();
     }
+}
+
+StepperResult Stepper::step_impl(bool new_primaries)
+{
+    // Generated count is per step (Stepper.cc:125-139)
+    CELER_HIP_CALL(hipMemsetAsync(&state_->ref().counters->num_generated, 0,
+                                  sizeof(std::uint32_t), state_->stream()));
+    this->launch_step(new_primaries);
+    return this->make_result();
+}
+
+StepperResult Stepper::run_steps(size_type count)
+{
+    CELER_VALIDATE(count > 0, << "no steps requested");
+    if (action_times_)
+    {
+        // Per-action timing reads its events back after every step
+        StepperResult r;
+        for (size_type i = 0; i < count; ++i)
+            r = (*this)();
+        return r;
+    }
+    CELER_HIP_CALL(hipMemsetAsync(&state_->ref().counters->num_generated, 0,
+                                  sizeof(std::uint32_t), state_->stream()));
+    for (size_type i = 0; i < count; ++i)
+        this->launch_step(false);
     return this->make_result();
 }
 

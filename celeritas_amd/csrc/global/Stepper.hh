@@ -61,6 +61,10 @@ class Stepper
     void warm_up();
     //! Transport existing tracks one step
     StepperResult operator()();
+    //! Transport existing tracks several steps with one counter readback
+    //! at the end (the steps need no host input: launch sizes and ranges
+    //! are device-resident)
+    StepperResult run_steps(size_type count);
     //! Insert primaries then take a step
     StepperResult operator()(Primary const* primaries, size_type count);
     //! Kill all active tracks (next step applies the tracking cut)
@@ -105,6 +109,7 @@ class Stepper
 
     StepperResult make_result();
     StepperResult step_impl(bool new_primaries);
+    void launch_step(bool new_primaries);
 };
 
 //---------------------------------------------------------------------------//

@@ -119,6 +119,8 @@ __global__ void init_prepare_kernel(CoreCounters* c, size_type num_slots)
     c->num_initializers -= n;
     c->num_vacancies -= n;
     c->num_active = num_slots - c->num_vacancies;
+    c->steps_taken += 1;
+    c->track_steps += c->num_active;
 }
 
 /*!
@@ -443,7 +445,19 @@ __global__ void __launch_bounds__(block_size)
     if (state.init.event_tally && event < state.init.max_events)
     {
         if (live_delta != 0)
-            atomicAdd(&state.init.event_live[event], live_delta);
+        {
+            // A track only changes its own event's count by -1 (death
+            // without secondaries) or more, and every other live track or
+            // initializer of the event is still counted, so the update that
+            // takes the count to zero is the event's completion
+            int const prev = atomicAdd(&state.init.event_live[event], live_delta);
+            if (prev + live_delta == 0)
+            {
+                std::uint32_t const idx = atomicAdd(&state.counters->num_completed, 1u);
+                if (idx < state.init.max_events)
+                    state.init.completed[idx] = event;
+            }
+        }
         double const edep = state.phys.energy_deposition[slot];
         if (edep != 0)
             atomicAdd(&state.init.event_edep[event], edep);

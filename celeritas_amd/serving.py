@@ -3,7 +3,8 @@
 Wraps the native serving loop (celer_server_*): events are admitted in
 arrival order while the initializer queue has room, share the device with
 those in flight, and complete when their device-resident live-track count
-reaches zero.
+reaches zero; the device appends each completed event to a list that the
+loop reads after each batch of steps.
 """
 
 from __future__ import annotations
@@ -20,7 +21,8 @@ from . import PRIMARY_DTYPE, CeleritasError, CoreParams, CoreState, _check, _loa
 class Server:
     """One track-slot state serving a stream of events."""
 
-    def __init__(self, params: CoreParams, num_track_slots: int, stream_id: int = 0):
+    def __init__(self, params: CoreParams, num_track_slots: int, stream_id: int = 0,
+                 steps_per_sync: int = 4):
         lib = _load()
         lib.celer_server_create.restype = ctypes.c_void_p
         lib.celer_server_create.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]
@@ -35,6 +37,9 @@ class Server:
         if not self._h:
             raise CeleritasError(lib.celer_amd_last_error().decode())
         self.state = CoreState(params, _handle=lib.celer_server_state(self._h), _owner=self)
+        lib.celer_server_set_steps_per_sync.restype = ctypes.c_int
+        lib.celer_server_set_steps_per_sync.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+        _check(lib.celer_server_set_steps_per_sync(self._h, steps_per_sync))
 
     def __del__(self):
         if getattr(self, "_h", None):

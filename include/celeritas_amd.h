@@ -65,9 +65,15 @@ typedef struct CelerCounters
     uint32_t prev_secondaries;
     uint32_t secondary_stack_size;
     uint32_t error_flags;
-    uint32_t steps_taken;
+    uint32_t steps_taken; /* step iterations run on the device */
     uint32_t num_new_neutral; /* neutral tracks in the last charge-partitioned init */
-    uint32_t pad_[2];
+    /* tracks failed because a sampling loop ran out of its iteration budget
+     * (error_flags bit 3 is set with it; not fatal to the run) */
+    uint32_t num_sampling_failures;
+    /* events completed on the device since the serving loop's start */
+    uint32_t num_completed;
+    /* sum over step iterations of the active tracks */
+    unsigned long long track_steps;
 } CelerCounters;
 
 /* Track initializer: celeritas/track/TrackInitData.hh:66-80 */
@@ -363,6 +369,11 @@ CelerServer* celer_server_create(CelerParams* params,
 void celer_server_destroy(CelerServer* server);
 /* The server's state (for field access); owned by the server */
 CelerState* celer_server_state(CelerServer* server);
+/* Step iterations the server launches back to back per counter readback
+ * while no arrived event waits for admission (default 4). No counterpart in
+ * the reference, whose loop reads its counters after every step
+ * (global/Stepper.cc:125-139). */
+int celer_server_set_steps_per_sync(CelerServer* server, uint32_t steps);
 /* Serve num_events events: event e is primaries[event_offsets[e] ..
  * event_offsets[e+1]) (their event_id is replaced by e) arriving
  * arrivals[e] seconds after the call starts (non-decreasing). Writes per

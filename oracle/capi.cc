@@ -267,6 +267,11 @@ std::uint32_t orc_state_get_f64(void* s, const char* name, double* out,
             put(v);
         return n;
     }
+    if (f == "total_edep")
+    {
+        put(st.total_edep);
+        return n;
+    }
     for (auto const& t : st.tracks)
     {
         if (f == "energy") put(t.energy);

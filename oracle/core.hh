@@ -120,6 +120,7 @@ struct State
     Counters counters;
     bool in_place = true;  // TrackOrder != init_charge
     std::vector<double> calo_edep;
+    double total_edep = 0;  //!< deposited in all volumes [MeV]
 };
 
 //! Build an empty state (all slots vacant, RNG seeded from the MT stream)

@@ -796,11 +796,13 @@ void along_step_charged(State& s)
 void calo_tally(State& s)
 {
     Problem const& prob = *s.problem;
-    if (prob.num_detectors == 0)
-        return;
     for (auto const& t : s.tracks)
     {
         if (t.status == inactive || t.edep == 0)
+            continue;
+        // Every volume (the per-event tally of the serving loop)
+        s.total_edep += t.edep;
+        if (prob.num_detectors == 0)
             continue;
         std::uint32_t det = prob.volume_to_detector[t.pre_volume];
         if (det != none)
