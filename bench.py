@@ -7,15 +7,19 @@ cone about +x, started at (-22, 0, 0) cm in front of 50 layers of
 2.3 mm Pb + 5.7 mm liquid argon (40 x 40 cm), 2^20 track slots, per-layer
 energy deposition tallied. One bench step transports the whole set of
 primaries to completion through the HIP stepping loop (one primary per
-event). Inputs are synthetic; physics tables come from the built-in
+event), with the step sequence replayed from a captured hipGraph, several
+steps per counter readback and the workgroup size chosen by the autotuner
+on a sample. Inputs are synthetic; physics tables come from the built-in
 generator.
 
 Prints one JSON line with the metric (track-steps per second over the
-timed steps; events/s beside it), a roofline entry for the dominant kernel
-(algorithmic bytes per launch over its average device time, from events on
-the state stream) and the CPU baseline (the oracle, the C++ restatement of
-the same loop, timed single-threaded on a bounded sample of the same
-primaries).
+timed steps; events/s beside it) and, beside it: the same workload in the
+mixed-precision mode with its per-detector deposition against the FP64
+loop (chi^2), the barrel detector in a 3.8 T field, the served stream
+(mixed precision, Poisson arrivals at swept rates, p50/p99 latency,
+against the plain FP64 loop over the same stream), a roofline entry for
+the dominant kernel and the whole step, and the CPU baseline (the oracle,
+the C++ restatement of the same loop, on OpenMP threads over events).
 """
 
 from __future__ import annotations
@@ -70,17 +74,24 @@ def parse_args():
                     help="track slot ordering (none, init_charge, reindex_status, "
                          "reindex_particle_type, reindex_along_step_action, "
                          "reindex_step_limit_action, reindex_both_action)")
-    ap.add_argument("--cpu-primaries", type=int, default=48,
-                    help="primaries in the CPU baseline sample (0: skip)")
+    ap.add_argument("--steps-per-sync", type=int, default=4,
+                    help="steps launched (graph replays) per counter readback")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="enqueue every step's kernels instead of replaying a graph")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default workgroup size (256)")
+    ap.add_argument("--cpu-primaries", type=int, default=160,
+                    help="primaries of the same workload in the CPU baseline (0: skip)")
+    ap.add_argument("--cpu-config1", type=int, default=1000,
+                    help="10 GeV e- primaries of the CPU-runnable configuration "
+                         "(4,096 slots per thread; 0: skip)")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed-precision measurement")
+    ap.add_argument("--cylinder-primaries", type=int, default=10000,
+                    help="primaries of the barrel-detector line (0: skip)")
     ap.add_argument("--serve-events", type=int, default=64,
                     help="events of the served TestEm3 workload (0: skip serving)")
-    ap.add_argument("--serve-baseline-events", type=int, default=6,
-                    help="events of the same stream run one at a time through the plain "
-                         "FP64 loop, for the serving speedup")
-    ap.add_argument("--mixed", action="store_true",
-                    help="also measure the same workload in the mixed-precision mode "
-                         "(off by default: validated at small scale only)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel on a buffer of this size "
                          "(reference for memory-counter profiles) and exit")
@@ -112,6 +123,64 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def host_threads():
+    """CPU threads for the host baseline: the job's share (OMP_NUM_THREADS
+    where set), else the machine's cores."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return os.cpu_count() or 1
+
+
+def autotune_block(amd, problem, primaries, slots, args):
+    """Workgroup size of the step kernels from short graph-replayed
+    transports of a sample of the workload (celeritas_amd.autotune)."""
+    from celeritas_amd import autotune
+
+    if args.no_autotune:
+        return 256, []
+    sample = primaries[: min(len(primaries), 500)]
+    trials = []
+    for bs in autotune.BLOCK_SIZES:
+        rate = autotune.measure(amd, problem, sample, args.track_order, bs, slots,
+                                use_graph=not args.no_graph)
+        trials.append({"block_size": bs, "track_steps_per_s": rate})
+        log(f"autotune block {bs}: {rate:.4g} track-steps/s")
+    best = max(trials, key=lambda t: t["track_steps_per_s"])["block_size"]
+    return best, trials
+
+
+def make_stepper(amd, problem, slots, block_size, args):
+    stepper = amd.Stepper(amd.CoreParams(problem), slots)
+    stepper.state.set_block_size(block_size)
+    if not args.no_graph:
+        stepper.capture_step()
+    return stepper
+
+
+def timed_transports(stepper, primaries, args, ndet, label):
+    """Warm-up and timed full transports; per-transport detector tallies."""
+    spk = max(args.steps_per_sync, 1)
+    for w in range(args.warmup):
+        r = stepper.run(primaries, max_steps=20_000, steps_per_sync=spk)
+        log(f"{label} warmup {w}: {r}")
+    stepper.state.synchronize()
+    tallies = []
+    totals = []
+    prev = stepper.state.get("calo_edep", ndet).copy()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        totals.append(stepper.run(primaries, max_steps=20_000, steps_per_sync=spk))
+        # Per-transport detector tallies (one 800-byte readback per transport)
+        cur = stepper.state.get("calo_edep", ndet).copy()
+        tallies.append(cur - prev)
+        prev = cur
+        log(f"{label} step {k}: {totals[-1]} at {time.perf_counter() - t0:.2f}s")
+    stepper.state.synchronize()
+    elapsed = time.perf_counter() - t0
+    return totals, elapsed, np.array(tallies)
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
@@ -133,55 +202,39 @@ def main():
     if args.workload == "cylinder":
         physics = amd.generate_physics(wl.cylinder_physics_spec())
         make_problem = wl.cylinder_problem
-        problem = wl.cylinder_problem(physics=physics, init_capacity=1 << 26,
-                                      max_events=max(args.primaries, 1),
-                                      track_order=args.track_order)
         primaries = wl.cylinder_primaries(args.primaries, seed=args.seed)
         ndet = max(wl.cylinder_geometry()[2]) + 1
     else:
         physics = amd.generate_physics(wl.testem3_physics_spec())
         make_problem = wl.testem3_problem
-        problem = wl.testem3_problem(physics=physics, init_capacity=1 << 26,
-                                     max_events=max(args.primaries, 1),
-                                     track_order=args.track_order)
         primaries = wl.testem3_primaries(args.primaries, seed=args.seed)
-        ndet = max(problem["volume_to_detector"]) + 1
-    params = amd.CoreParams(problem)
+        ndet = 100
 
-    stepper = amd.Stepper(params, args.slots)
-    log(f"state: {args.slots} slots")
-    for w in range(args.warmup):
-        r = stepper.run(primaries, max_steps=20_000)
-        log(f"warmup {w}: {r}")
+    def problem_of(precision="fp64"):
+        return make_problem(physics=physics, init_capacity=1 << 26,
+                            max_events=max(args.primaries, 1),
+                            track_order=args.track_order, precision=precision)
 
-    stepper.state.synchronize()
-    t0 = time.perf_counter()
-    totals = []
-    for k in range(args.steps):
-        totals.append(stepper.run(primaries, max_steps=20_000))
-        log(f"step {k}: {totals[-1]} at {time.perf_counter() - t0:.2f}s")
-    stepper.state.synchronize()
-    elapsed = time.perf_counter() - t0
+    problem = problem_of()
+    block_size, tune_trials = autotune_block(amd, problem, primaries, args.slots, args)
+    stepper = make_stepper(amd, problem, args.slots, block_size, args)
+    log(f"state: {args.slots} slots, workgroup {block_size}, "
+        f"{'graph replay' if not args.no_graph else 'eager'}, "
+        f"{args.steps_per_sync} steps per readback")
+    totals, elapsed, tallies = timed_transports(stepper, primaries, args, ndet, "fp64")
 
     track_steps = sum(t["track_steps"] for t in totals)
     value = track_steps / elapsed
     events_per_s = args.steps * args.primaries / elapsed
-    edep = stepper.state.get("calo_edep", ndet)
     if args.workload == "cylinder":
         workload = ("Cylindrical detector (Si tracker 10 x 0.3 mm, PbWO4 ECAL 23 cm, "
                     "HCAL 20 x (5 cm brass + 4 mm scintillator)) in a uniform 3.8 T "
                     f"field, {args.primaries} primaries e-/e+/gamma 40/20/40, "
                     "log-uniform 0.5-50 GeV, isotropic, 2^20 slots, FP64, "
                     "Dormand-Prince propagation")
-        physics_label = ("Compton, Livermore PE, Bethe-Heitler, e+ annihilation, "
-                         "Moller-Bhabha ionisation, Seltzer-Berger and relativistic "
-                         "brems, Urban MSC, Urban energy-loss fluctuations")
     else:
         workload = ("TestEm3 50 x (2.3 mm Pb + 5.7 mm lAr), 10000 primaries e-/gamma "
                     "50/50, log-uniform 1-100 GeV, 5 deg cone, 2^20 slots, FP64")
-        physics_label = ("Compton, Livermore PE, Bethe-Heitler, e+ annihilation, "
-                         "Moller-Bhabha ionisation, SB/relativistic brems, Urban MSC, "
-                         "Urban energy-loss fluctuations")
 
     line = {
         "metric": "steps/s and events/s, FP32-mixed vs FP64 on 1 GPU; p50/p99 event latency",
@@ -204,48 +257,97 @@ def main():
             "primaries": args.primaries,
             "slots": args.slots,
             "track_order": args.track_order,
-            "physics": physics_label,
-            "edep_total_mev": float(edep.sum()),
+            "graph_replay": not args.no_graph,
+            "steps_per_readback": args.steps_per_sync,
+            "workgroup_size": block_size,
+            "autotune_trials": tune_trials,
+            "physics": "Compton, Livermore PE, Bethe-Heitler, e+ annihilation, "
+                       "Moller-Bhabha ionisation, SB/relativistic brems, Urban MSC, "
+                       "Urban energy-loss fluctuations",
+            "edep_mev_per_transport": float(tallies.sum()) / args.steps,
         },
     }
 
-    if args.mixed:
-        line["mixed"] = mixed_run(amd, make_problem, physics, primaries, args, value,
-                                  events_per_s)
+    if not args.no_mixed:
+        line["mixed"] = mixed_run(amd, problem_of("mixed"), primaries, args, block_size,
+                                  ndet, value, events_per_s, tallies)
+    if args.cylinder_primaries > 0 and args.workload == "testem3":
+        line["cylinder"] = cylinder_run(amd, wl, args)
     if args.serve_events > 0 and args.workload == "testem3":
         line["serve"] = serve_run(amd, wl, physics, args)
     if not args.no_roofline:
-        line["roofline"] = roofline(amd, params, primaries, args.slots, args.track_order)
-    if args.cpu_primaries > 0:
-        line["cpu_baseline"] = cpu_baseline(amd, wl, problem, args)
+        line["roofline"] = roofline(amd, problem, primaries, args.slots, args.track_order,
+                                    block_size, value)
+    if args.cpu_primaries > 0 or args.cpu_config1 > 0:
+        line["cpu_baseline"] = cpu_baseline(amd, wl, problem, physics, args)
     print(json.dumps(line), flush=True)
 
 
-def mixed_run(amd, make_problem, physics, primaries, args, fp64_value, fp64_events):
-    """Same workload with the physics kernels in the mixed-precision mode."""
-    problem = make_problem(physics=physics, init_capacity=1 << 26,
-                           max_events=max(args.primaries, 1), track_order=args.track_order,
-                           precision="mixed")
-    params = amd.CoreParams(problem)
-    stepper = amd.Stepper(params, args.slots)
-    for w in range(args.warmup):
-        log(f"mixed warmup {w}: {stepper.run(primaries, max_steps=20_000)}")
-    stepper.state.synchronize()
-    t0 = time.perf_counter()
-    steps = 0
-    for k in range(args.steps):
-        r = stepper.run(primaries, max_steps=20_000)
-        steps += r["track_steps"]
-        log(f"mixed step {k}: {r} at {time.perf_counter() - t0:.2f}s")
-    stepper.state.synchronize()
-    dt = time.perf_counter() - t0
+def mixed_run(amd, problem, primaries, args, block_size, ndet, fp64_value, fp64_events,
+              fp64_tally):
+    """Same workload with the physics kernels in the mixed-precision mode,
+    and its per-detector deposition against the FP64 loop's."""
+    from scipy import stats
+
+    stepper = make_stepper(amd, problem, args.slots, block_size, args)
+    totals, dt, tallies = timed_transports(stepper, primaries, args, ndet, "mixed")
+    steps = sum(t["track_steps"] for t in totals)
     value = steps / dt
     events = args.steps * args.primaries / dt
     log(f"mixed precision: {value:.4g} track-steps/s, {events:.4g} events/s")
-    return {"value": value, "unit": "track-steps/s", "events_per_s": events,
-            "ms_per_step": 1e3 * dt / args.steps, "speedup_events_vs_fp64": events / fp64_events,
-            "dtype": "f32 transcendentals and draws in sampling, interpolation and MSC; "
-                     "f64 positions, safety, field, energies and tallies"}
+    fails = stepper.state.counters()["num_sampling_failures"]
+    out = {"value": value, "unit": "track-steps/s", "events_per_s": events,
+           "ms_per_step": 1e3 * dt / args.steps,
+           "speedup_events_vs_fp64": events / fp64_events,
+           "sampling_failures": int(fails),
+           "dtype": "f32 transcendentals and draws in sampling, interpolation and MSC; "
+                    "f64 positions, safety, field, energies and tallies"}
+    # Per-detector deposition against the FP64 loop (itself checked against
+    # the FP64 oracle by the parity tests): each timed transport of the
+    # same primaries is an independent sample, so the spread over
+    # transports gives each detector's standard error
+    if len(tallies) >= 2 and len(fp64_tally) >= 2:
+        mm, ms = tallies.mean(0), tallies.std(0, ddof=1) / np.sqrt(len(tallies))
+        fm, fs = fp64_tally.mean(0), fp64_tally.std(0, ddof=1) / np.sqrt(len(fp64_tally))
+        sel = (ms > 0) & (fs > 0) & (mm + fm > 1e-5 * (mm + fm).max())
+        chi2 = float((((mm[sel] - fm[sel]) ** 2) / (ms[sel] ** 2 + fs[sel] ** 2)).sum())
+        ndf = int(sel.sum())
+        tot_err = float(np.sqrt((ms ** 2).sum() + (fs ** 2).sum()) / fm.sum())
+        rel = (mm[sel] - fm[sel]) / fm[sel]
+        out["edep_vs_fp64"] = {
+            "total_rel": float(mm.sum() / fm.sum() - 1), "total_rel_err": tot_err,
+            "chi2": chi2, "ndf": ndf, "p_value": float(stats.chi2.sf(chi2, ndf)),
+            "max_detector_rel": float(np.max(np.abs(rel))),
+            "samples": f"{len(tallies)} mixed and {len(fp64_tally)} FP64 transports of "
+                       "the same primaries (independent RNG streams)"}
+        log(f"mixed vs FP64 edep: total {out['edep_vs_fp64']['total_rel']:+.2e} +- "
+            f"{tot_err:.1e}, chi2 {chi2:.1f}/{ndf}")
+    return out
+
+
+def cylinder_run(amd, wl, args):
+    """The barrel detector in a 3.8 T field (FP64, Dormand-Prince, graph
+    replay, autotuned workgroup size)."""
+    physics = amd.generate_physics(wl.cylinder_physics_spec())
+    n = args.cylinder_primaries
+    primaries = wl.cylinder_primaries(n, seed=args.seed)
+    problem = wl.cylinder_problem(physics=physics, init_capacity=1 << 26,
+                                  max_events=max(n, 1), track_order=args.track_order)
+    sub = argparse.Namespace(**vars(args))
+    sub.steps, sub.warmup = max(1, min(args.steps, 2)), 1
+    block, trials = autotune_block(amd, problem, primaries, args.slots, sub)
+    stepper = make_stepper(amd, problem, args.slots, block, sub)
+    ndet = max(wl.cylinder_geometry()[2]) + 1
+    totals, dt, _ = timed_transports(stepper, primaries, sub, ndet, "cylinder")
+    steps = sum(t["track_steps"] for t in totals)
+    log(f"cylinder: {steps / dt:.4g} track-steps/s, {sub.steps * n / dt:.4g} events/s")
+    return {"workload": f"Cylindrical detector (Si tracker, PbWO4 ECAL, brass/scintillator "
+                        f"HCAL) in a uniform 3.8 T field, {n} primaries e-/e+/gamma "
+                        "40/20/40, log-uniform 0.5-50 GeV, isotropic, 2^20 slots, FP64, "
+                        "Dormand-Prince, graph replay, autotuned workgroup",
+            "value": steps / dt, "unit": "track-steps/s",
+            "events_per_s": sub.steps * n / dt, "timed_transports": sub.steps,
+            "workgroup_size": block, "autotune_trials": trials}
 
 
 def serve_run(amd, wl, physics, args):
@@ -343,13 +445,47 @@ def sorted_launch(label, track_order):
     return False
 
 
-def roofline(amd, params, primaries, slots, track_order):
-    """Dominant kernel's algorithmic bandwidth from an instrumented run."""
-    st = amd.Stepper(params, slots, action_times=True)
+#: Per-slot fields of the SoA state carried from one step to the next, in
+#: bytes, and whether a step writes them back (global/CoreData.hh,
+#: geo/GeoData.hh, phys/PhysicsData.hh, random/XorwowEngine.hh). Fields a
+#: step computes and consumes itself (cross sections, step length and
+#: actions, deposition, secondary span, element) are not persistent: in a
+#: step fused into one kernel they would stay in registers.
+PERSISTENT_FIELDS = [
+    # (field, bytes, written back by a step)
+    ("sim.track_id", 4, False), ("sim.parent_id", 4, False), ("sim.event_id", 4, False),
+    ("sim.num_steps", 4, True), ("sim.time", 8, True), ("sim.status", 1, True),
+    ("particle.particle_id", 1, False), ("particle.energy", 8, True),
+    ("geo.pos", 24, True), ("geo.dir", 24, True), ("geo.volume", 4, True),
+    ("geo.surface", 4, True), ("geo.sense", 1, True), ("geo.boundary", 1, True),
+    ("geo.failed", 1, False), ("geo.next_step", 8, True), ("geo.next_surface", 4, True),
+    ("geo.next_sense", 1, True), ("phys.interaction_mfp", 8, True),
+    ("phys.msc_range", 24, True), ("rng.xorwow", 24, True),
+]
+#: Per new track: the secondary written to the stack and read back (40 B
+#: each way) and the initializer written and read (80 B each way)
+BYTES_PER_NEW_TRACK = 2 * 40 + 2 * 80
+
+
+def step_bytes(new_tracks_per_step):
+    """Algorithmic HBM bytes of one track-step: every persistent field read
+    once and, if the step changes it, written once, plus the bytes of the
+    tracks it creates (new tracks per track-step)."""
+    persistent = sum(b * (2 if w else 1) for _, b, w in PERSISTENT_FIELDS)
+    return persistent + BYTES_PER_NEW_TRACK * new_tracks_per_step, persistent
+
+
+def roofline(amd, problem, primaries, slots, track_order, block_size, headline):
+    """Dominant kernel's algorithmic bandwidth from an instrumented run
+    (HIP events around each action's launches on the state's stream), and
+    the whole step's algorithmic bandwidth at the headline rate."""
+    st = amd.Stepper(amd.CoreParams(problem), slots, action_times=True)
+    st.state.set_block_size(block_size)
     st.diagnostics(enable=True)
-    st.run(primaries)
+    r = st.run(primaries)
     times = st.action_times()
     diag = st.diagnostics()
+    new_tracks = int(st.state.get("track_counters", len(primaries)).sum())
     ranked = sorted(times.items(), key=lambda kv: -kv[1][0])
     log("action device times [s]: " + ", ".join(f"{k}={v[0]:.4f}" for k, v in ranked))
     log(f"work counters: {diag}")
@@ -371,6 +507,10 @@ def roofline(amd, params, primaries, slots, track_order):
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get(label)
+    # Whole step: tracks created beyond the primaries per track-step
+    per_step_new = (new_tracks - len(primaries)) / max(r["track_steps"], 1)
+    b_step, persistent = step_bytes(per_step_new)
+    whole = headline * b_step / 1e9
     return {
         "bound": "hbm",
         "kernel": label,
@@ -379,38 +519,74 @@ def roofline(amd, params, primaries, slots, track_order):
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
         "traffic": traffic,
+        "traffic_source": "profiles/pmc_traffic.json: rocprofv3 FETCH_SIZE and WRITE_SIZE "
+                          "per dispatch of this kernel, scaled by the plain-copy "
+                          "calibration (not measured in this run)",
         "algorithmic_bytes_per_launch": per_launch,
         "avg_launch_us": 1e6 * avg_launch,
         "launches": launches,
         "share_of_step_time": secs / sum(v[0] for v in times.values()),
+        "work_counters": diag,
+        "step": {
+            "b_step": b_step,
+            "persistent_state_bytes": persistent,
+            "new_tracks_per_track_step": per_step_new,
+            "achieved": whole,
+            "frac": whole / HBM_PEAK_GBS,
+            "model": "persistent SoA fields read once and written once if changed, "
+                     "plus 240 B per created track (secondary and initializer written "
+                     "and read), at the headline track-steps/s",
+        },
     }
 
 
-def cpu_baseline(amd, wl, problem, args):
-    """The oracle (C++ restatement of the loop) on a sample of the primaries."""
+def cpu_baseline(amd, wl, problem, physics, args):
+    """The oracle (the C++ restatement of the loop, FP64) on the host's
+    cores: events dealt round-robin to OpenMP threads, each with its own
+    state (celer-sim's event-parallel mode, app/celer-sim/celer-sim.cc:
+    115-137). A sample of the bench workload gives the value; the
+    CPU-runnable configuration (1,000 x 10 GeV e-, 4,096 slots per thread)
+    runs in full beside it."""
     from tests import oracle_bind as ob
 
-    if args.workload == "cylinder":
-        sample = wl.cylinder_primaries(args.cpu_primaries, seed=args.seed + 1)
-    else:
-        sample = wl.testem3_primaries(args.cpu_primaries, seed=args.seed + 1)
-    for i in range(len(sample)):
-        sample["event_id"][i] = i
+    threads = host_threads()
     oprob = ob.Problem(problem)
-    st = ob.State(oprob, 4096)
-    t0 = time.perf_counter()
-    active, _ = ob.run(st, sample, max_iters=10_000_000)
-    dt = time.perf_counter() - t0
-    steps = int(active.sum())
-    return {
-        "value": steps / dt,
-        "unit": "track-steps/s",
-        "cores": 1,
-        "kind": "port",
-        "events_per_s": len(sample) / dt,
-        "sample": f"{len(sample)} primaries of the same distribution (seed {args.seed + 1}), "
-                  f"4096 slots, single thread; {steps} track-steps in {dt:.1f} s",
-    }
+    out = {"unit": "track-steps/s", "cores": threads, "kind": "port",
+           "threads_note": "OpenMP threads over events; the job's host CPU share "
+                           "(OMP_NUM_THREADS), os.cpu_count() = "
+                           f"{os.cpu_count()}"}
+    if args.cpu_primaries > 0:
+        if args.workload == "cylinder":
+            sample = wl.cylinder_primaries(args.cpu_primaries, seed=args.seed + 1)
+        else:
+            sample = wl.testem3_primaries(args.cpu_primaries, seed=args.seed + 1)
+        sample["event_id"] = np.arange(len(sample))
+        r = ob.run_events_parallel(oprob, sample, 4096, threads)
+        out["value"] = r["track_steps"] / r["seconds"]
+        out["events_per_s"] = len(sample) / r["seconds"]
+        out["sample"] = (f"{len(sample)} primaries of the bench workload's distribution "
+                         f"(seed {args.seed + 1}), one event each, {threads} threads x "
+                         f"4096 slots; {r['track_steps']} track-steps in "
+                         f"{r['seconds']:.1f} s")
+        log(f"CPU baseline: {out['value']:.4g} track-steps/s on {threads} threads")
+    if args.cpu_config1 > 0 and args.workload == "testem3":
+        n = args.cpu_config1
+        prim = wl.testem3_primaries(n, seed=args.seed + 2, emin=1e4, emax=1e4,
+                                    cone_deg=0.0, gamma_fraction=0.0)
+        prim["event_id"] = np.arange(n)
+        cprob = ob.Problem(wl.testem3_problem(physics=physics, init_capacity=1 << 20,
+                                              max_events=n))
+        r = ob.run_events_parallel(cprob, prim, 4096, threads)
+        out["config1"] = {
+            "workload": f"TestEm3, {n} x 10 GeV e- along +x from (-22, 0, 0) cm, one "
+                        f"event each, FP64, {threads} threads x 4096 slots",
+            "track_steps_per_s": r["track_steps"] / r["seconds"],
+            "events_per_s": n / r["seconds"], "seconds": r["seconds"],
+            "track_steps": r["track_steps"]}
+        log(f"CPU config 1: {n / r['seconds']:.4g} events/s")
+        if "value" not in out:
+            out["value"] = out["config1"]["track_steps_per_s"]
+    return out
 
 
 if __name__ == "__main__":

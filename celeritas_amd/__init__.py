@@ -468,10 +468,36 @@ class Stepper:
         _check(f(self._h, secs.ctypes.data, counts.ctypes.data, n, int(reset)))
         return {lab: (float(s), int(c)) for lab, s, c in zip(labels, secs, counts)}
 
-    def run(self, primaries: np.ndarray, max_steps: int = 100_000) -> dict:
+    def run_steps(self, count: int) -> StepperResult:
+        """Take ``count`` steps without primaries with one counter readback."""
+        f = _load().celer_stepper_run_steps
+        f.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]
+        f.restype = ctypes.c_int
+        r = CStepperResult()
+        _check(f(self._h, count, ctypes.byref(r)))
+        return StepperResult(generated=r.generated, queued=r.queued, active=r.active,
+                             alive=r.alive)
+
+    def run(self, primaries: np.ndarray, max_steps: int = 100_000,
+            steps_per_sync: int = 1) -> dict:
         """Transport primaries to completion (celer-sim's Transporter loop:
         Runner.cc/Transporter.cc); returns step iterations and the sum of
-        active tracks over iterations (track-steps)."""
+        active tracks over iterations (track-steps). With steps_per_sync > 1
+        the steps after the first are launched in batches with one counter
+        readback each (the counts then come from the device's step and
+        track-step counters; a batch may end with a few empty steps)."""
+        if steps_per_sync > 1:
+            c0 = self.state.counters()
+            result = self(primaries)
+            launched = 1
+            while result:
+                if launched >= max_steps:
+                    raise CeleritasError(f"transport did not finish in {max_steps} steps")
+                result = self.run_steps(steps_per_sync)
+                launched += steps_per_sync
+            c = self.state.counters()
+            return {"iterations": c["steps_taken"] - c0["steps_taken"],
+                    "track_steps": int(c["track_steps"] - c0["track_steps"])}
         result = self(primaries)
         iters, track_steps = 1, result["active"]
         while result:

@@ -431,6 +431,20 @@ int celer_stepper_step(CelerStepper* stepper,
     });
 }
 
+int celer_stepper_run_steps(CelerStepper* stepper, uint32_t count, CelerStepperResult* result)
+{
+    return guarded([&] {
+        StepperResult r = stepper->stepper->run_steps(count);
+        if (result)
+        {
+            result->generated = r.generated;
+            result->queued = r.queued;
+            result->active = r.active;
+            result->alive = r.alive;
+        }
+    });
+}
+
 int celer_stepper_warm_up(CelerStepper* stepper)
 {
     return guarded([&] { stepper->stepper->warm_up(); });

@@ -243,6 +243,12 @@ int celer_stepper_step(CelerStepper* stepper,
                        const CelerPrimary* primaries,
                        uint32_t count,
                        CelerStepperResult* result);
+/* Take count steps without new primaries, reading the counters back once
+ * at the end (kernel launch sizes and ranges are device-resident, so the
+ * steps need no host input); result is the last step's. (The reference's
+ * Stepper reads its counters after every step, global/Stepper.cc:125-139.) */
+int celer_stepper_run_steps(CelerStepper* stepper, uint32_t count,
+                            CelerStepperResult* result);
 int celer_stepper_warm_up(CelerStepper* stepper);
 int celer_stepper_kill_active(CelerStepper* stepper);
 int celer_stepper_reseed(CelerStepper* stepper, uint64_t event_id);

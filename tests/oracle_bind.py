@@ -450,3 +450,21 @@ def msc_tail_parts(zeff: float, tau: float, theta0: float, true_path: float,
     f(zeff, inp.ctypes.data, out.ctypes.data)
     keys = ("xsi", "x", "c", "d", "prob", "qprob", "xmean_1", "xmean_2", "ea")
     return dict(zip(keys, out.tolist()))
+
+
+def run_events_parallel(problem: "Problem", primaries: np.ndarray, num_slots: int,
+                        num_threads: int, num_calo: int = 0) -> dict:
+    """Event-parallel transport on OpenMP threads (one state per thread)."""
+    from celeritas_amd import PRIMARY_DTYPE
+
+    f = load().orc_run_events_parallel
+    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
+                  ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]
+    f.restype = ctypes.c_int
+    p = np.ascontiguousarray(primaries, dtype=PRIMARY_DTYPE)
+    out = np.zeros(4)
+    calo = np.zeros(max(num_calo, 1))
+    _check(f(problem.handle, p.ctypes.data, len(p), num_slots, num_threads, out.ctypes.data,
+             calo.ctypes.data if num_calo else None, num_calo))
+    return {"track_steps": int(out[0]), "iterations": int(out[1]), "seconds": out[2],
+            "threads": int(out[3]), "calo_edep": calo[:num_calo]}
