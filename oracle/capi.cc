@@ -984,6 +984,12 @@ void orc_msc_tail_parts(double zeff, double const* in, double* out)
 }
 
 extern "C" {
+//! Plant a deliberate mistake in the grid calculators (0: none)
+void orc_grid_set_mutation(int m)
+{
+    orc::g_grid_mutation = m;
+}
+
 //! Evaluate a log-uniform grid built from (emin, emax, size, prime index,
 //! values): kind 0 = cross section (XsCalculator), 1 = range
 //! (RangeCalculator), 2 = inverse range (InverseRangeCalculator, x = range)

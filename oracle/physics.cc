@@ -157,6 +157,13 @@ double range_to_step(PhysicsData const& pd, double range)
 //---------------------------------------------------------------------------//
 // XS GRID
 //---------------------------------------------------------------------------//
+//! Planted mistakes for the grid-calculator pins (0: none). 1: the upper
+//! point of the bracket that is the prime index is not unscaled; 2: the
+//! bracket is interpolated in log(E) instead of E; 3: the range below the
+//! grid scales linearly instead of with sqrt(E); 4: the inverse range below
+//! the grid scales linearly instead of quadratically.
+int g_grid_mutation = 0;
+
 double XsGridO::energy(std::uint32_t i) const
 {
     return std::exp(front + delta * i);
@@ -188,13 +195,18 @@ double XsGridO::calc(double e) const
     lower = std::min(lower, size - 2);
     double upper_e = std::exp(front + delta * (lower + 1));
     double upper_xs = values[lower + 1];
-    if (lower + 1 == prime)
+    if (lower + 1 == prime && g_grid_mutation != 1)
         upper_xs /= upper_e;
     double lower_e = std::exp(front + delta * lower);
     double lower_xs = values[lower];
     // Interpolator<linear, linear>: fma(slope, x - x_l, y_l)
     double slope = ((-lower_xs) + upper_xs) / ((-lower_e) + upper_e);
     double result = std::fma(slope, (-lower_e) + e, lower_xs);
+    if (g_grid_mutation == 2)
+    {
+        double const t = (loge - (front + delta * lower)) / delta;
+        result = lower_xs + t * (upper_xs - lower_xs);
+    }
     if (lower >= prime)
         result /= e;
     return result;
@@ -756,7 +768,7 @@ double calc_range(XsGridO const& g, double energy)
     // RangeCalculator::operator() (grid/RangeCalculator.hh:80-110)
     double const loge = std::log(energy);
     if (loge <= g.front)
-        return g.values[0] * std::exp(0.5 * (loge - g.front));
+        return g.values[0] * std::exp((g_grid_mutation == 3 ? 1.0 : 0.5) * (loge - g.front));
     if (loge >= g.back)
         return g.values[g.size - 1];
     auto const i = std::min(static_cast<std::uint32_t>((loge - g.front) / g.delta), g.size - 2);
@@ -773,7 +785,7 @@ double calc_inverse_range(XsGridO const& g, double range)
     if (range < r.front())
     {
         double const f = range / r.front();
-        return std::exp(g.front) * (f * f);
+        return std::exp(g.front) * (g_grid_mutation == 4 ? f : f * f);
     }
     if (range >= r.back())
         return std::exp(g.back);

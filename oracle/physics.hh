@@ -198,6 +198,8 @@ void build_physics(Json const& inp, PhysicsData& out);
 //! Range and inverse range on a range grid (grid/RangeCalculator.hh,
 //! grid/InverseRangeCalculator.hh)
 double calc_range(XsGridO const& g, double energy);
+//! Planted mistake of the grid calculators for the pin tests (0: none)
+extern int g_grid_mutation;
 double calc_inverse_range(XsGridO const& g, double range);
 
 // Step actions (each loops over all slots, applying to matching tracks)
