@@ -140,6 +140,10 @@ CELER_FORCEINLINE_FUNCTION double tan(double x)
 {
     return static_cast<double>(::tanf(static_cast<float>(x)));
 }
+CELER_FORCEINLINE_FUNCTION double tanh(double x)
+{
+    return static_cast<double>(::tanhf(static_cast<float>(x)));
+}
 #else
 using std::atan;
 using std::cbrt;
@@ -152,6 +156,7 @@ using std::pow;
 using std::sin;
 using std::sqrt;
 using std::tan;
+using std::tanh;
 #endif
 }  // namespace mth
 

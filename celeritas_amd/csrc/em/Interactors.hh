@@ -81,7 +81,7 @@ CELER_FUNCTION Interaction sample_interaction(ModelClass model,
             MaterialRecord const& m = p.materials[ctx.material];
             ElementRecord const& el
                 = p.elements[p.components[m.elem_begin + ctx.element].element];
-            return sample_bethe_heitler(p, ctx.state, el, energy, dir, rng);
+            return sample_bethe_heitler(p, ctx.state, m, el, energy, dir, rng);
         }
         case ModelClass::moller_bhabha:
             return sample_moller_bhabha(p, ctx.state, ctx.particle, ctx.material,

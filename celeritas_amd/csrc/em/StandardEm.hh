@@ -7,8 +7,8 @@
 //! Sampling methods follow the reference interactors, which implement the
 //! Geant4 Physics Reference Manual algorithms:
 //! - Bethe-Heitler pair production with the Tsai screening functions and
-//!   Coulomb correction (em/interactor/BetheHeitlerInteractor.hh:165-321);
-//!   LPM suppression applies only above 100 GeV and is not modeled here.
+//!   Coulomb correction (em/interactor/BetheHeitlerInteractor.hh:165-321),
+//!   with LPM suppression above 100 GeV when the physics enables it.
 //! - Moller (e-e-) and Bhabha (e+e-) delta-ray production above the cut
 //!   (em/interactor/MollerBhabhaInteractor.hh:119-157,
 //!   em/distribution/{Moller,Bhabha}EnergyDistribution.hh,
@@ -25,6 +25,7 @@
 #include "phys/PhysicsImpl.hh"
 #include "random/XorwowEngine.hh"
 #include "InteractorCommon.hh"
+#include "LPM.hh"
 
 namespace celeritas_amd
 {
@@ -87,9 +88,28 @@ CELER_FORCEINLINE_FUNCTION double bh_screening_f2(double d)
                    : 41.326 - d * (5.848 - 0.902 * d);
 }
 
+//! Screening functions Phi_1 and Phi_2 (BetheHeitlerInteractor.hh:338-356)
+CELER_FORCEINLINE_FUNCTION void bh_screening_phi(double d, double* phi1, double* phi2)
+{
+    if (d > 1.4)
+    {
+        *phi1 = 21.0190 - 4.145 * mth::log(d + 0.958);
+        *phi2 = *phi1;
+    }
+    else
+    {
+        *phi1 = 20.806 - d * (3.190 - 0.5710 * d);
+        *phi2 = 20.234 - d * (2.126 - 0.0903 * d);
+    }
+}
+
+//! Photon energy above which Bethe-Heitler applies LPM suppression [MeV]
+inline constexpr double bh_lpm_threshold = 1e5;
+
 template<class Engine>
 CELER_FUNCTION Interaction sample_bethe_heitler(PhysicsParamsRef const& p,
                                                 CoreStateRef const& s,
+                                                MaterialRecord const& mat,
                                                 ElementRecord const& el,
                                                 double energy,
                                                 double const* dir,
@@ -124,6 +144,9 @@ CELER_FUNCTION Interaction sample_bethe_heitler(PhysicsParamsRef const& p,
         double const st = (half - eps_min) * (half - eps_min) * f10;
         double const sf = 1.5 * f20;
         double const p_f1 = st / (st + sf);
+        // LPM without dielectric suppression (BetheHeitlerData.hh:51-54)
+        bool const use_lpm = p.scalars.enable_lpm && energy > bh_lpm_threshold;
+        double const lpm_energy = mat.rad_length * lpm::lpm_constant();
         double g;
         unsigned int iter = 0;
         do
@@ -132,13 +155,38 @@ CELER_FUNCTION Interaction sample_bethe_heitler(PhysicsParamsRef const& p,
             {
                 eps = half - (half - eps_min) * mth::cbrt(draw_canonical(rng));
                 double const delta = 136 / el.cbrt_z * eps0 / (eps * (1 - eps));
-                g = (bh_screening_f1(delta) - f_z) / f10;
+                if (use_lpm)
+                {
+                    double phi1, phi2;
+                    bh_screening_phi(delta, &phi1, &phi2);
+                    lpm::Functions const f = lpm::calc_functions(
+                        el.cbrt_z, mat.electron_density, lpm_energy, false, energy, eps);
+                    g = f.xi * ((2 * f.phi + f.g) * phi1 - f.g * phi2 - f.phi * f_z) / f10;
+                }
+                else
+                {
+                    g = (bh_screening_f1(delta) - f_z) / f10;
+                }
             }
             else
             {
                 eps = eps_min + (half - eps_min) * draw_canonical(rng);
                 double const delta = 136 / el.cbrt_z * eps0 / (eps * (1 - eps));
-                g = (bh_screening_f2(delta) - f_z) / f20;
+                if (use_lpm)
+                {
+                    double phi1, phi2;
+                    bh_screening_phi(delta, &phi1, &phi2);
+                    lpm::Functions const f = lpm::calc_functions(
+                        el.cbrt_z, mat.electron_density, lpm_energy, false, energy, eps);
+                    g = f.xi
+                        * ((f.phi + half * f.g) * phi1 + half * f.g * phi2
+                           - half * (f.g + f.phi) * f_z)
+                        / f20;
+                }
+                else
+                {
+                    g = (bh_screening_f2(delta) - f_z) / f20;
+                }
             }
         } while (g < draw_canonical(rng) && within_budget(iter, rng));
     }

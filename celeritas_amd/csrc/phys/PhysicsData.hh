@@ -232,7 +232,8 @@ struct PhysicsScalars
     std::uint8_t failure_action;
     std::uint8_t msc_action;
     std::uint8_t fixed_step_action;
-    std::uint8_t pad_;
+    //! LPM suppression in the relativistic brems and Bethe-Heitler models
+    std::uint8_t enable_lpm;
 };
 
 //! Per-model constants (filled for the models in the problem)

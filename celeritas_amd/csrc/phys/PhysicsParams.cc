@@ -149,6 +149,7 @@ std::vector<GridHost> grids_of(JsonValue const& proc,
 constexpr double alpha_fine = 7.2973525693e-3;  // CODATA 2018
 constexpr double r_electron_cm = 2.8179403262e-13;
 constexpr double avogadro = 6.02214076e23;
+constexpr double amu_gram = 1.66053906660e-24;  // CODATA 2018 [g]
 
 /*!
  * Riemann zeta(n) - 1 for integer n >= 3.
@@ -207,7 +208,8 @@ double calc_mass_radiation_coeff(int z, double atomic_mass)
             lrad = std::log(184.15 * std::pow(z, -1.0 / 3));
             lrad_prime = std::log(1194.0 * std::pow(z, -2.0 / 3));
     }
-    return 4 * alpha_fine * r_electron_cm * r_electron_cm * avogadro / atomic_mass
+    // Per gram: the atom's mass is A amu (mat/detail/Utils.cc:102-107)
+    return 4 * alpha_fine * r_electron_cm * r_electron_cm / (atomic_mass * amu_gram)
            * (z * z * (lrad - calc_coulomb_correction(z)) + z * lrad_prime);
 }
 
@@ -473,6 +475,23 @@ PhysicsParams::PhysicsParams(JsonValue const& inp, ActionRegistry* reg)
                 }
             }
             r.num_elements = static_cast<std::uint32_t>(h.components.size()) - r.elem_begin;
+            if (r.rad_length == 0 && r.number_density > 0 && r.num_elements > 0)
+            {
+                // Derived as MaterialParams does (MaterialParams.cc:496-519):
+                // 1 / (sum_i w_i (1/X0)_i rho), rho = n sum_i w_i A_i amu
+                double avg_amu = 0, rad_coeff = 0;
+                for (std::uint32_t i = r.elem_begin; i < h.components.size(); ++i)
+                {
+                    ElementComponent const& ec = h.components[i];
+                    CELER_VALIDATE(ec.element < h.elements.size(),
+                                   << "material refers to an unknown element");
+                    ElementRecord const& el = h.elements[ec.element];
+                    avg_amu += ec.fraction * el.atomic_mass;
+                    rad_coeff += ec.fraction * el.mass_radiation_coeff;
+                }
+                double const density = r.number_density * avg_amu * amu_gram;
+                r.rad_length = 1 / (rad_coeff * density);
+            }
             CELER_VALIDATE(r.num_elements <= static_cast<std::uint32_t>(max_elements),
                            << "too many elements in a material");
             h.materials.push_back(r);
@@ -493,6 +512,7 @@ PhysicsParams::PhysicsParams(JsonValue const& inp, ActionRegistry* reg)
             = o.get("lowest_electron_energy", sc.lowest_electron_energy);
         sc.fixed_step_limiter = o.get("fixed_step_limiter", 0.0);
         sc.apply_post_interaction_cut = o.get_bool("apply_cuts", false);
+        sc.enable_lpm = o.get_bool("lpm", false) ? 1 : 0;
     }
 
     // Cutoffs

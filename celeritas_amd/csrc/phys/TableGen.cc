@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "corecel/DeviceRuntime.hh"
+#include "em/LPM.hh"
 
 namespace celeritas_amd
 {
@@ -179,7 +180,8 @@ double mass_radiation_coeff(Element const& el)
             lrad = std::log(184.15 * std::pow(z, -1.0 / 3));
             lrad_p = std::log(1194.0 * std::pow(z, -2.0 / 3));
     }
-    return 4 * alpha * r_e * r_e * avogadro / el.atomic_mass
+    // Per gram: the atom's mass is A amu (mat/detail/Utils.cc:102-107)
+    return 4 * alpha * r_e * r_e / (el.atomic_mass * amu_gram)
            * (z * z * (lrad - el.fc) + z * lrad_p);
 }
 
@@ -236,9 +238,30 @@ double sb_chi(Element const& el, double energy, double kappa)
            + fx * ((1 - fy) * v(ix + 1, iy) + fy * v(ix + 1, iy + 1));
 }
 
+//! Relativistic DCS with LPM suppression (RBDiffXsCalculator.hh:186-205)
+//! when enabled and the total energy exceeds the material's threshold
+double rb_dxsec_lpm(Element const& el, Material const& mat, double total, double k,
+                    bool lpm)
+{
+    double const density_factor = mat.electron_density * lpm::migdal_constant();
+    double const lpm_energy = mat.rad_length * lpm::lpm_constant();
+    if (!lpm || !(total > lpm_energy * std::sqrt(density_factor)))
+        return rb_dxsec(el, total, k);
+    double const epsilon = total / k;
+    lpm::Functions const f = lpm::calc_functions(el.cbrt_z, mat.electron_density,
+                                                 lpm_energy, true, k, epsilon);
+    double const y = k / total;
+    double const onemy = 1 - y;
+    double const y2 = 0.25 * (y * y);
+    double const term = f.xi * (y2 * f.g + (onemy + 2 * y2) * f.phi);
+    return std::max(term * el.rb_factor1 + onemy * el.rb_factor2, 0.0);
+}
+
 //! k dsigma/dk per atom [cm^2] with dielectric suppression, as sampled by
-//! the SB (E < 1 GeV) and relativistic samplers
-double brems_k_dsigma(Element const& el, Material const& mat, double energy, double k)
+//! the SB (E < 1 GeV) and relativistic samplers (the latter with LPM
+//! suppression when enabled, as Geant4's relativistic model tabulates it)
+double brems_k_dsigma(Element const& el, Material const& mat, double energy, double k,
+                      bool lpm)
 {
     double const total = energy + electron_mass;
     double const dens = mat.electron_density * 4 * pi * r_e * lambdabar_e * lambdabar_e
@@ -249,7 +272,8 @@ double brems_k_dsigma(Element const& el, Material const& mat, double energy, dou
         double const beta2 = energy * (energy + 2 * electron_mass) / (total * total);
         return el.z * el.z / beta2 * sb_chi(el, energy, k / energy) * millibarn * supp;
     }
-    return 16.0 / 3 * alpha * r_e * r_e * el.z * el.z * rb_dxsec(el, total, k) * supp;
+    return 16.0 / 3 * alpha * r_e * r_e * el.z * el.z * rb_dxsec_lpm(el, mat, total, k, lpm)
+           * supp;
 }
 
 //! Bethe-Heitler screening functions (as sampled by the interactor)
@@ -715,6 +739,9 @@ JsonValue generate_physics(JsonValue const& spec)
 
     JsonValue const opts = spec.has("options") ? spec.at("options") : JsonValue::object();
     int const bpd = static_cast<int>(opts.get_int("bins_per_decade", 7));
+    // LPM suppression of the relativistic brems and Bethe-Heitler models:
+    // on unless the spec turns it off (GeantPhysicsOptions::lpm)
+    bool const enable_lpm = opts.get_bool("lpm", true);
     LogGrid const grid{opts.get("emin", 1e-4), opts.get("emax", 1e8), 0};
     int const npts = static_cast<int>(std::lround(bpd * std::log10(grid.emax / grid.emin))) + 1;
     LogGrid const g{grid.emin, grid.emax, npts};
@@ -862,7 +889,7 @@ JsonValue generate_physics(JsonValue const& spec)
                 double const kc = std::min(cut_gamma[mi], e);
                 double brem = atom_sum(m, [&](Element const& el) {
                     return integrate_log(
-                        [&](double k) { return brems_k_dsigma(el, m, e, k); },
+                        [&](double k) { return brems_k_dsigma(el, m, e, k, enable_lpm); },
                         kc * 1e-6, kc, 6);
                 });
                 dedx[mi][i] = ioni_dedx(m, e, ecut[mi], electron) + brem;
@@ -905,7 +932,7 @@ JsonValue generate_physics(JsonValue const& spec)
             double const kc = cut_gamma[mi];
             if (e <= kc)
                 return 0.0;
-            return integrate_log([&](double k) { return brems_k_dsigma(el, m, e, k) / k; },
+            return integrate_log([&](double k) { return brems_k_dsigma(el, m, e, k, enable_lpm) / k; },
                                  kc, e, 8);
         };
         JsonValue brem_models;
@@ -939,7 +966,7 @@ JsonValue generate_physics(JsonValue const& spec)
                     return 0.0;
                 return atom_sum(m, [&](Element const& el) {
                     return integrate_log(
-                        [&](double k) { return brems_k_dsigma(el, m, e, k) / k; }, kc, e, 8);
+                        [&](double k) { return brems_k_dsigma(el, m, e, k, enable_lpm) / k; }, kc, e, 8);
                 });
             });
     }
@@ -1002,6 +1029,7 @@ JsonValue generate_physics(JsonValue const& spec)
     out.obj["cutoffs"] = cuts;
     JsonValue options = JsonValue::object();
     options.obj["eloss_fluctuation"] = JsonValue::boolean(true);
+    options.obj["lpm"] = JsonValue::boolean(enable_lpm);
     for (auto const& kv : opts.obj)
         if (kv.first != "bins_per_decade" && kv.first != "emin" && kv.first != "emax"
             && kv.first != "msc")

@@ -689,6 +689,43 @@ double orc_rb_dxsec(void* problem, std::uint32_t element, double total_energy,
     auto const& pd = *static_cast<OrcProblem*>(problem)->p->physics;
     return rb_dxsec(pd.elements.at(element), total_energy, gamma_energy);
 }
+
+//! Relativistic DCS with the LPM switch of RBDiffXsCalculator (the element
+//! is a component of the material)
+double orc_rb_dxsec_lpm(void* problem, std::uint32_t material, std::uint32_t component,
+                        int enable_lpm, double total_energy, double gamma_energy)
+{
+    auto const& pd = *static_cast<OrcProblem*>(problem)->p->physics;
+    MaterialDef const& mat = pd.materials.at(material);
+    ElementDef const& el = pd.elements.at(mat.elements.at(component).first);
+    return RbDxsec(el, mat, enable_lpm != 0, total_energy)(gamma_energy);
+}
+
+//! Plant a deliberate mistake in the LPM functions (0: none)
+void orc_lpm_set_mutation(int m)
+{
+    orc::g_lpm_mutation = m;
+}
+
+//! Material radiation length [cm] as the physics data holds it
+double orc_material_rad_length(void* problem, std::uint32_t material)
+{
+    auto const& pd = *static_cast<OrcProblem*>(problem)->p->physics;
+    return pd.materials.at(material).rad_length;
+}
+
+//! LPM functions (xi, G, phi) for a material component
+void orc_lpm_functions(void* problem, std::uint32_t material, std::uint32_t component,
+                       int dielectric, double gamma_energy, double epsilon, double* out)
+{
+    auto const& pd = *static_cast<OrcProblem*>(problem)->p->physics;
+    MaterialDef const& mat = pd.materials.at(material);
+    ElementDef const& el = pd.elements.at(mat.elements.at(component).first);
+    LpmFunctions const f = lpm_functions(el, mat, dielectric != 0, gamma_energy, epsilon);
+    out[0] = f.xi;
+    out[1] = f.g;
+    out[2] = f.phi;
+}
 }
 
 extern "C" {

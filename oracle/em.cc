@@ -136,6 +136,27 @@ double bh_f2(double d)
                    : 41.326 - d * (5.848 - 0.902 * d);
 }
 
+//! Screening functions Phi_1, Phi_2 (BetheHeitlerInteractor::screening_phi1_phi2)
+struct BhPhi
+{
+    double phi1, phi2;
+};
+BhPhi bh_phi1_phi2(double d)
+{
+    BhPhi r;
+    if (d > 1.4)
+    {
+        r.phi1 = 21.0190 - 4.145 * std::log(d + 0.958);
+        r.phi2 = r.phi1;
+    }
+    else
+    {
+        r.phi1 = 20.806 - d * (3.190 - 0.5710 * d);
+        r.phi2 = 20.234 - d * (2.126 - 0.0903 * d);
+    }
+    return r;
+}
+
 template<class Engine>
 Interaction bethe_heitler(InteractorInput const& in, State& s, Engine& rng)
 {
@@ -173,6 +194,9 @@ Interaction bethe_heitler(InteractorInput const& in, State& s, Engine& rng)
         double const st = (half - eps_min) * (half - eps_min) * f10;
         double const sf = 1.5 * f20;
         double const p_f1 = st / (st + sf);
+        // LPM above 100 GeV when enabled; no dielectric suppression
+        // (BetheHeitlerInteractor.hh:105-109,147-153, BetheHeitlerData.hh:51)
+        bool const lpm = pd.enable_lpm && E > 1e5;
         double g;
         do
         {
@@ -180,13 +204,35 @@ Interaction bethe_heitler(InteractorInput const& in, State& s, Engine& rng)
             {
                 eps = half - (half - eps_min) * std::cbrt(rng());
                 double delta = 136 / el.cbrt_z * eps0 / (eps * (1 - eps));
-                g = (bh_f1(delta) - f_z) / f10;
+                if (lpm)
+                {
+                    BhPhi const sc = bh_phi1_phi2(delta);
+                    LpmFunctions const f = lpm_functions(el, mat, false, E, eps);
+                    g = f.xi * ((2 * f.phi + f.g) * sc.phi1 - f.g * sc.phi2 - f.phi * f_z)
+                        / f10;
+                }
+                else
+                {
+                    g = (bh_f1(delta) - f_z) / f10;
+                }
             }
             else
             {
                 eps = eps_min + (half - eps_min) * rng();
                 double delta = 136 / el.cbrt_z * eps0 / (eps * (1 - eps));
-                g = (bh_f2(delta) - f_z) / f20;
+                if (lpm)
+                {
+                    BhPhi const sc = bh_phi1_phi2(delta);
+                    LpmFunctions const f = lpm_functions(el, mat, false, E, eps);
+                    g = f.xi
+                        * ((f.phi + half * f.g) * sc.phi1 + half * f.g * sc.phi2
+                           - half * (f.g + f.phi) * f_z)
+                        / f20;
+                }
+                else
+                {
+                    g = (bh_f2(delta) - f_z) / f20;
+                }
             }
         } while (g < rng());
     }
@@ -403,13 +449,16 @@ Interaction brem_final_state(InteractorInput const& in, double mass,
     return r;
 }
 
-//! RBEnergySampler (em/interactor/detail/RBEnergySampler.hh), LPM off
+//! RBEnergySampler (em/interactor/detail/RBEnergySampler.hh) over
+//! RBDiffXsCalculator (LPM when the model enables it and the total energy
+//! is above the material's threshold)
 template<class Engine>
-double sample_rb_energy(ElementDef const& el, double electron_density,
+double sample_rb_energy(ElementDef const& el, MaterialDef const& mat, bool enable_lpm,
                         double inc_energy, double mass, double kc, Engine& rng)
 {
     double const total = inc_energy + mass;
-    double const dens = electron_density * migdal_constant() * (total * total);
+    RbDxsec const dxsec(el, mat, enable_lpm, total);
+    double const dens = dxsec.density_corr;
     double const tmin = std::min(kc, inc_energy);
     double const tmax = std::min(high_energy_limit, inc_energy);
     double const a = tmin * tmin + dens;
@@ -419,7 +468,7 @@ double sample_rb_energy(ElementDef const& el, double electron_density,
     do
     {
         k = std::sqrt(a * std::exp(logratio * rng()) - dens);
-        dsigma = rb_dxsec(el, total, k);
+        dsigma = dxsec(k);
     } while (dsigma < max_xs * rng());
     return k;
 }
@@ -446,7 +495,7 @@ Interaction brems(InteractorInput const& in, State& s, Engine& rng, int which)
     bool const use_rb = which == 1 || (which == 2 && in.energy >= sb_upper_limit);
     double k;
     if (use_rb)
-        k = sample_rb_energy(el, mat.electron_density, in.energy, mass, kc, rng);
+        k = sample_rb_energy(el, mat, pd.enable_lpm, in.energy, mass, kc, rng);
     else
         k = sample_sb_energy(el, mat.electron_density, in.energy, mass, kc,
                              in.particle == pd.electron ? 0 : 1, rng);
@@ -715,6 +764,145 @@ double rb_dxsec(ElementDef const& el, double total_energy, double gamma_energy)
     return std::max(dxsec, 0.0);
 }
 
+//---------------------------------------------------------------------------//
+// LPM SUPPRESSION (em/xs/LPMCalculator.hh)
+//---------------------------------------------------------------------------//
+double lpm_constant()
+{
+    // alpha (m_e c^2)^2 / (2 h c) (PhysicsConstants.hh:46-55), evaluated in
+    // CGS from the electron mass in grams (whose rest energy differs from
+    // 0.51099895 MeV by 7.5e-12 relative), then converted to MeV/cm
+    constexpr double c_light = 2.99792458e10;  // [cm/s]
+    constexpr double h_planck = 6.62607015e-27;  // [erg s]
+    constexpr double electron_mass = 9.1093837015e-28;  // [g]
+    constexpr double mev = 1.602176634e-6;  // [erg]
+    double const emc2 = electron_mass * (c_light * c_light);
+    return alpha_fine * (emc2 * emc2) / (2 * h_planck * c_light) / mev;
+}
+
+namespace
+{
+//! Horner evaluation with fma, highest coefficient last (PolyEvaluator)
+template<std::size_t N>
+double poly(double const (&c)[N], double x)
+{
+    double r = c[N - 1];
+    for (std::size_t i = N - 1; i-- > 0;)
+        r = std::fma(x, r, c[i]);
+    return r;
+}
+
+double lpm_phi(double s)
+{
+    if (s < 0.01)
+        return s * poly({6.0, -6 * pi}, s);
+    if (s < 1.55)
+    {
+        double const a = poly({0.623, 0.796, 0.658}, s);
+        double const b = poly({-6.0, -6 * (3 - pi), 1 / a}, s);
+        return 1 - std::exp(s * b);
+    }
+    double const s2 = s * s;
+    return 1 - 0.01190476 / (s2 * s2);
+}
+
+double lpm_g(double s, double phi)
+{
+    if (s < 0.01)
+        return poly({-2 * phi, 12.0}, s);
+    if (s < 0.415827)
+    {
+        double const a = poly({1.0, 3.936, 4.97, -0.05, 7.5}, s);
+        double const b = poly({-4.0, -8 / a}, s);
+        double const psi = 1 - std::exp(s * b);
+        return 3 * psi - 2 * phi;
+    }
+    if (s < 1.9156)
+        return std::tanh(poly({-0.160723, 3.755030, -1.798138, 0.672827, -0.120772}, s));
+    double const s2 = s * s;
+    return 1 - 0.0230655 / (s2 * s2);
+}
+}  // namespace
+
+//! Planted mistakes for the LPM pin tests (0: none). 1: no dielectric
+//! suppression; 2: xi without Stanev's cubic correction; 3: G and phi
+//! exchanged in the brems DCS
+int g_lpm_mutation = 0;
+
+LpmFunctions lpm_functions(ElementDef const& el, MaterialDef const& mat,
+                           bool dielectric_suppression, double gamma_energy,
+                           double epsilon)
+{
+    if (g_lpm_mutation == 1)
+        dielectric_suppression = false;
+    double const lpm_energy = mat.rad_length * lpm_constant();
+    // s' (Stanev Eq. 21 input)
+    double const s_prime = std::sqrt(
+        lpm_energy / (8 * epsilon * gamma_energy * std::fabs(epsilon - 1)));
+    double const r = el.cbrt_z / 184.15;
+    double const s1 = r * r;
+    double const sqrt_two = 1.41421356237309504880;
+    double xi = 2;
+    if (s_prime > 1)
+    {
+        xi = 1;
+    }
+    else if (s_prime > sqrt_two * s1)
+    {
+        double const log_s1 = std::log(sqrt_two * s1);
+        double const h = std::log(s_prime) / log_s1;
+        xi = 1 + h - (g_lpm_mutation == 2 ? 0.0 : 0.08) * (1 - h) * h * (2 - h) / log_s1;
+    }
+    double s = s_prime / std::sqrt(xi);
+    if (dielectric_suppression)
+    {
+        double const ek = epsilon * gamma_energy;
+        double const k_p_sq = mat.electron_density * migdal_constant() * (ek * ek);
+        s *= (1 + k_p_sq / (gamma_energy * gamma_energy));
+        xi = 2;
+        if (s > 1)
+            xi = 1;
+        else if (s > s1)
+            xi = 1 + std::log(s) / std::log(s1);
+    }
+    double const phi = lpm_phi(s);
+    if (xi * phi > 1 || s > 0.57)
+        xi = 1 / phi;
+    LpmFunctions out;
+    out.phi = phi;
+    out.xi = xi;
+    out.g = lpm_g(s, phi);
+    return out;
+}
+
+RbDxsec::RbDxsec(ElementDef const& el_, MaterialDef const& mat_, bool lpm, double total)
+    : el(el_), mat(mat_), total_energy(total)
+{
+    double const density_factor = mat.electron_density * migdal_constant();
+    density_corr = density_factor * (total * total);
+    double const lpm_energy = mat.rad_length * lpm_constant();
+    double const lpm_threshold = lpm_energy * std::sqrt(density_factor);
+    enable_lpm = lpm && total > lpm_threshold;
+}
+
+double RbDxsec::operator()(double gamma_energy) const
+{
+    if (!enable_lpm)
+        return rb_dxsec(el, total_energy, gamma_energy);
+    // RBDiffXsCalculator::dxsec_per_atom_lpm; the relativistic model
+    // includes the dielectric suppression
+    double const epsilon = total_energy / gamma_energy;
+    LpmFunctions lpm = lpm_functions(el, mat, true, gamma_energy, epsilon);
+    if (g_lpm_mutation == 3)
+        std::swap(lpm.g, lpm.phi);
+    double const y = gamma_energy / total_energy;
+    double const onemy = 1 - y;
+    double const y2 = 0.25 * (y * y);
+    double const term = lpm.xi * (y2 * lpm.g + (onemy + 2 * y2) * lpm.phi);
+    double const dxsec = term * el.rb_factor1 + onemy * el.rb_factor2;
+    return std::max(dxsec, 0.0);
+}
+
 double sb_positron_scale(int z, double mass, double kc, double inc_energy,
                          double exit_energy)
 {
@@ -969,9 +1157,10 @@ double mass_radiation_coeff(int z, double atomic_mass)
             lrad = std::log(184.15 * std::pow(z, -1.0 / 3));
             lrad_prime = std::log(1194.0 * std::pow(z, -2.0 / 3));
     }
-    constexpr double avogadro = 6.02214076e23;
+    // Per gram: the atom's mass is A amu (mat/detail/Utils.cc:102-107)
+    constexpr double amu_gram = 1.66053906660e-24;
     double const factor = 4 * alpha_fine * r_electron * r_electron;
-    return factor * avogadro / atomic_mass
+    return factor / (atomic_mass * amu_gram)
            * (z * z * (lrad - coulomb_correction(z)) + z * lrad_prime);
 }
 

@@ -131,6 +131,34 @@ SBTableO make_sb_table(Json const& j);
 void set_rb_element_data(double electron_mass, ElementDef& el);
 //! Relativistic DCS without LPM (RBDiffXsCalculator::dxsec_per_atom)
 double rb_dxsec(ElementDef const& el, double total_energy, double gamma_energy);
+
+//! LPM suppression functions (em/xs/LPMCalculator.hh)
+struct LpmFunctions
+{
+    double xi = 0, g = 0, phi = 0;
+};
+//! Planted mistake of the LPM functions for the pin tests (0: none)
+extern int g_lpm_mutation;
+//! Characteristic LPM energy per unit radiation length [MeV/cm]
+double lpm_constant();
+//! xi, G and phi at epsilon = (lepton energy) / (photon energy)
+LpmFunctions lpm_functions(ElementDef const& el, MaterialDef const& mat,
+                           bool dielectric_suppression, double gamma_energy,
+                           double epsilon);
+
+//! Relativistic DCS of an incident lepton (RBDiffXsCalculator): with LPM
+//! when enabled and the total energy exceeds the material's threshold
+struct RbDxsec
+{
+    ElementDef const& el;
+    MaterialDef const& mat;
+    double total_energy;
+    double density_corr = 0;
+    bool enable_lpm = false;
+
+    RbDxsec(ElementDef const& el, MaterialDef const& mat, bool lpm, double total);
+    double operator()(double gamma_energy) const;
+};
 //! Positron DCS scaling (SBPositronXsCorrector)
 double sb_positron_scale(int z, double mass, double kc, double inc_energy,
                          double exit_energy);

@@ -280,6 +280,21 @@ void build_physics(Json const& inp, PhysicsData& out)
                     d.elements.emplace_back(
                         static_cast<std::uint32_t>(c.num("element", 0)),
                         c.num("fraction", 1));
+            if (d.rad_length == 0 && d.number_density > 0 && !d.elements.empty())
+            {
+                // Derived as MaterialParams does (MaterialParams.cc:496-519):
+                // 1 / (sum_i w_i (1/X0)_i rho), rho = n sum_i w_i A_i amu
+                constexpr double amu_gram = 1.66053906660e-24;
+                double avg_amu = 0, rad_coeff = 0;
+                for (auto const& c : d.elements)
+                {
+                    ElementDef const& el = out.elements.at(c.first);
+                    avg_amu += c.second * el.atomic_mass;
+                    rad_coeff += c.second * el.mass_radiation_coeff;
+                }
+                double const density = d.number_density * avg_amu * amu_gram;
+                d.rad_length = 1 / (rad_coeff * density);
+            }
             out.materials.push_back(d);
         }
     }
@@ -296,6 +311,7 @@ void build_physics(Json const& inp, PhysicsData& out)
         out.fixed_step_limiter = opts->num("fixed_step_limiter", 0);
         out.apply_post_interaction_cut = opts->flag("apply_cuts", false);
         out.fluct = opts->flag("eloss_fluctuation", false);
+        out.enable_lpm = opts->flag("lpm", false);
     }
     if (out.fluct)
     {

@@ -178,6 +178,9 @@ struct PhysicsData
     double lowest_electron_energy = 0.001;
     double fixed_step_limiter = 0;
     bool apply_post_interaction_cut = false;
+    //! LPM suppression in the relativistic brems and Bethe-Heitler models
+    //! (GeantPhysicsOptions::lpm)
+    bool enable_lpm = false;
     // Along-step models
     MscDataO msc;
     bool fluct = false;

@@ -1,7 +1,8 @@
 """Device bremsstrahlung against the oracle on identical uniform streams.
 
 Seltzer-Berger with the br29 table (electrons and positrons), the
-relativistic model in Pb and the combined model across its 1 GeV switch;
+relativistic model in Pb (with and without LPM suppression) and the
+combined model across its 1 GeV switch;
 FP64 results must match to 1e-12 relative.
 """
 
@@ -25,13 +26,22 @@ CASES = [
     ("combined_brem", "cu", pb.POSITRON, (0.025, 1e5)),
     ("relativistic_brem", "pb", pb.ELECTRON, (1e3, 1e6)),
     ("relativistic_brem", "pb", pb.POSITRON, (1e3, 1e6)),
+    # LPM suppression (above ~0.5 GeV total energy in Pb)
+    ("relativistic_brem", "pb_lpm", pb.ELECTRON, (1e2, 1e6)),
+    ("relativistic_brem", "pb_lpm", pb.POSITRON, (1e2, 1e6)),
 ]
+
+
+def _lpm(prob):
+    prob["physics"]["options"] = {"lpm": True}
+    return prob
 
 
 @pytest.fixture(scope="module")
 def setups(amd):
     out = {}
-    for key, prob in (("cu", pem.sb_copper()), ("pb", pem.rb_lead())):
+    for key, prob in (("cu", pem.sb_copper()), ("pb", pem.rb_lead()),
+                      ("pb_lpm", _lpm(pem.rb_lead()))):
         params = amd.CoreParams(prob)
         out[key] = (params, amd.CoreState(params, N), ob.Problem(prob))
     return out

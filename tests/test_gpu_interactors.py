@@ -5,6 +5,8 @@ Each sample gets its own stretch of uniforms from a seeded std::mt19937
 the device result must match the oracle to 1e-12 relative (FP64).
 """
 
+import zlib
+
 import numpy as np
 import pytest
 
@@ -44,7 +46,7 @@ def setup(amd):
 @pytest.mark.parametrize("model,particle,erange,material", CASES)
 def test_interactor_parity(amd, setup, model, particle, erange, material):
     params, state, oprob = setup
-    rng = np.random.default_rng(hash(model) % 2**32 + particle)
+    rng = np.random.default_rng(zlib.crc32(model.encode()) + particle)
     energies = np.exp(rng.uniform(np.log(erange[0]), np.log(erange[1]), N))
     dirs = random_dirs(rng, N)
     uniforms = ob.mt_uniforms(1234 + particle, N * STRIDE)
@@ -57,4 +59,29 @@ def test_interactor_parity(amd, setup, model, particle, erange, material):
     np.testing.assert_array_equal(gpu[:, 6], cpu[:, 6])  # secondary count
     np.testing.assert_array_equal(gpu[:, 7], cpu[:, 7])  # secondary particle
     np.testing.assert_array_equal(gpu[:, 12], cpu[:, 12])
+    gpu_compare.assert_records_close(gpu, cpu, rtol=1e-12, atol=1e-12)
+
+
+@pytest.fixture(scope="module")
+def setup_lpm(amd):
+    prob = pb.interactor_host()
+    prob["physics"]["options"] = {"lpm": True}
+    params = amd.CoreParams(prob)
+    state = amd.CoreState(params, N)
+    return params, state, ob.Problem(prob)
+
+
+def test_bethe_heitler_lpm_parity(amd, setup_lpm):
+    """Bethe-Heitler with LPM suppression (active above 100 GeV) in Cu."""
+    params, state, oprob = setup_lpm
+    rng = np.random.default_rng(31)
+    energies = np.exp(rng.uniform(np.log(1e4), np.log(1e7), N))
+    dirs = random_dirs(rng, N)
+    uniforms = ob.mt_uniforms(4321, N * STRIDE)
+    cpu = ob.interactor_replay_batch(oprob, "bethe_heitler", pb.GAMMA_ID, energies, dirs,
+                                     uniforms, STRIDE)
+    gpu = amd.verify_interactor(params, state, "bethe_heitler", pb.GAMMA_ID, energies, dirs,
+                                uniforms, STRIDE)
+    np.testing.assert_array_equal(gpu[:, 0], cpu[:, 0])
+    np.testing.assert_array_equal(gpu[:, 6], cpu[:, 6])
     gpu_compare.assert_records_close(gpu, cpu, rtol=1e-12, atol=1e-12)
