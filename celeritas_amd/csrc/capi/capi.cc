@@ -17,6 +17,7 @@
 #include "global/Stepper.hh"
 #include "track/TrackActions.hh"
 #include "track/TrackKernels.hh"
+#include "user/UserActions.hh"
 
 using namespace celeritas_amd;
 
@@ -25,6 +26,7 @@ static_assert(sizeof(CelerCounters) == sizeof(CoreCounters), "Counters layout");
 static_assert(sizeof(CelerTrackInitializer) == sizeof(TrackInitializer),
               "TrackInitializer layout");
 static_assert(sizeof(CelerSecondary) == sizeof(Secondary), "Secondary layout");
+static_assert(sizeof(CelerStepSelection) == sizeof(StepSelection), "StepSelection layout");
 
 struct CelerParams
 {
@@ -725,3 +727,245 @@ int celer_params_export_json(const CelerParams*, char* buf, size_t len, size_t* 
 }
 
 }  // extern "C"
+
+//---------------------------------------------------------------------------//
+// USER ACTIONS
+//---------------------------------------------------------------------------//
+struct CelerActionDiagnostic
+{
+    std::shared_ptr<ActionDiagnostic> diag;
+};
+
+struct CelerStepDiagnostic
+{
+    std::shared_ptr<StepDiagnostic> diag;
+};
+
+struct CelerStepCollector
+{
+    std::shared_ptr<StepCollector> coll;
+};
+
+struct CelerDetectorSteps
+{
+    DetectorStepOutput out;
+};
+
+namespace
+{
+template<class V>
+int copy_column(V const& v, void* dst, size_t bytes, char const* name)
+{
+    return guarded([&] {
+        CELER_VALIDATE(!v.empty() || bytes == 0,
+                       << "detector step column '" << name << "' is empty (not selected or no hits)");
+        size_t const need = v.size() * sizeof(v[0]);
+        CELER_VALIDATE(bytes >= need, << "buffer of " << bytes << " bytes for column '"
+                                      << name << "' needs " << need);
+        if (need)
+            std::memcpy(dst, v.data(), need);
+    });
+}
+}  // namespace
+
+extern "C" {
+int celer_action_diagnostic_create(CelerParams* params, CelerActionDiagnostic** out)
+{
+    return guarded([&] {
+        CELER_VALIDATE(params && out, << "null argument");
+        auto h = std::make_unique<CelerActionDiagnostic>();
+        h->diag = ActionDiagnostic::make_and_insert(*params->params);
+        *out = h.release();
+    });
+}
+
+void celer_action_diagnostic_destroy(CelerActionDiagnostic* diag)
+{
+    delete diag;
+}
+
+int celer_action_diagnostic_shape(const CelerActionDiagnostic* diag,
+                                  uint32_t* num_particles,
+                                  uint32_t* num_actions)
+{
+    return guarded([&] {
+        CELER_VALIDATE(diag && num_particles && num_actions, << "null argument");
+        *num_particles = diag->diag->num_particles();
+        *num_actions = diag->diag->num_actions();
+    });
+}
+
+int celer_action_diagnostic_counts(const CelerActionDiagnostic* diag, uint64_t* out, size_t size)
+{
+    return guarded([&] {
+        CELER_VALIDATE(diag && (out || size == 0), << "null argument");
+        auto counts = diag->diag->calc_actions();
+        CELER_VALIDATE(size >= counts.size(), << "output needs " << counts.size()
+                                              << " entries, got " << size);
+        std::copy(counts.begin(), counts.end(), out);
+    });
+}
+
+int celer_action_diagnostic_clear(CelerActionDiagnostic* diag)
+{
+    return guarded([&] {
+        CELER_VALIDATE(diag, << "null argument");
+        diag->diag->clear();
+    });
+}
+
+int celer_step_diagnostic_create(CelerParams* params,
+                                 uint32_t max_step_bin,
+                                 CelerStepDiagnostic** out)
+{
+    return guarded([&] {
+        CELER_VALIDATE(params && out, << "null argument");
+        auto h = std::make_unique<CelerStepDiagnostic>();
+        h->diag = StepDiagnostic::make_and_insert(*params->params, max_step_bin);
+        *out = h.release();
+    });
+}
+
+void celer_step_diagnostic_destroy(CelerStepDiagnostic* diag)
+{
+    delete diag;
+}
+
+int celer_step_diagnostic_shape(const CelerStepDiagnostic* diag,
+                                uint32_t* num_particles,
+                                uint32_t* num_bins)
+{
+    return guarded([&] {
+        CELER_VALIDATE(diag && num_particles && num_bins, << "null argument");
+        *num_particles = diag->diag->num_particles();
+        *num_bins = diag->diag->num_bins();
+    });
+}
+
+int celer_step_diagnostic_counts(const CelerStepDiagnostic* diag, uint64_t* out, size_t size)
+{
+    return guarded([&] {
+        CELER_VALIDATE(diag && (out || size == 0), << "null argument");
+        auto counts = diag->diag->calc_steps();
+        CELER_VALIDATE(size >= counts.size(), << "output needs " << counts.size()
+                                              << " entries, got " << size);
+        std::copy(counts.begin(), counts.end(), out);
+    });
+}
+
+int celer_step_diagnostic_clear(CelerStepDiagnostic* diag)
+{
+    return guarded([&] {
+        CELER_VALIDATE(diag, << "null argument");
+        diag->diag->clear();
+    });
+}
+
+int celer_step_collector_create(CelerParams* params,
+                                const CelerStepSelection* selection,
+                                const uint32_t* volume_to_detector,
+                                uint32_t num_volumes,
+                                int nonzero_energy_deposition,
+                                uint32_t hit_capacity,
+                                CelerStepCollector** out)
+{
+    return guarded([&] {
+        CELER_VALIDATE(params && selection && out, << "null argument");
+        StepCollector::Input inp;
+        std::memcpy(static_cast<void*>(&inp.selection), selection, sizeof(StepSelection));
+        if (volume_to_detector)
+            inp.volume_to_detector.assign(volume_to_detector, volume_to_detector + num_volumes);
+        inp.nonzero_energy_deposition = nonzero_energy_deposition != 0;
+        inp.hit_capacity = hit_capacity;
+        auto h = std::make_unique<CelerStepCollector>();
+        h->coll = StepCollector::make_and_insert(*params->params, std::move(inp));
+        *out = h.release();
+    });
+}
+
+void celer_step_collector_destroy(CelerStepCollector* coll)
+{
+    delete coll;
+}
+
+int celer_step_collector_read(CelerStepCollector* coll,
+                              uint32_t stream_id,
+                              CelerDetectorSteps** out)
+{
+    return guarded([&] {
+        CELER_VALIDATE(coll && out, << "null argument");
+        auto h = std::make_unique<CelerDetectorSteps>();
+        h->out = coll->coll->read(stream_id);
+        *out = h.release();
+    });
+}
+
+void celer_detector_steps_destroy(CelerDetectorSteps* steps)
+{
+    delete steps;
+}
+
+uint32_t celer_detector_steps_size(const CelerDetectorSteps* steps)
+{
+    return steps ? steps->out.size() : 0;
+}
+
+int celer_detector_steps_overflow(const CelerDetectorSteps* steps,
+                                  uint32_t* overflow_steps,
+                                  uint32_t* dropped_hits)
+{
+    return guarded([&] {
+        CELER_VALIDATE(steps && overflow_steps && dropped_hits, << "null argument");
+        *overflow_steps = steps->out.overflow_steps;
+        *dropped_hits = steps->out.dropped_hits;
+    });
+}
+
+int celer_detector_steps_column(const CelerDetectorSteps* steps,
+                                const char* name,
+                                void* dst,
+                                size_t bytes)
+{
+    if (!steps || !name || (!dst && bytes))
+        return guarded([] { CELER_VALIDATE(false, << "null argument"); });
+    DetectorStepOutput const& o = steps->out;
+    std::string n(name);
+    if (n == "detector")
+        return copy_column(o.detector, dst, bytes, name);
+    if (n == "track_id")
+        return copy_column(o.track_id, dst, bytes, name);
+    if (n == "event_id")
+        return copy_column(o.event_id, dst, bytes, name);
+    if (n == "parent_id")
+        return copy_column(o.parent_id, dst, bytes, name);
+    if (n == "track_step_count")
+        return copy_column(o.track_step_count, dst, bytes, name);
+    if (n == "action_id")
+        return copy_column(o.action_id, dst, bytes, name);
+    if (n == "particle")
+        return copy_column(o.particle, dst, bytes, name);
+    if (n == "step_length")
+        return copy_column(o.step_length, dst, bytes, name);
+    if (n == "energy_deposition")
+        return copy_column(o.energy_deposition, dst, bytes, name);
+    for (int p = 0; p < 2; ++p)
+    {
+        std::string const prefix = p == 0 ? "pre." : "post.";
+        if (n.compare(0, prefix.size(), prefix) != 0)
+            continue;
+        std::string const f = n.substr(prefix.size());
+        DetectorStepOutput::Point const& pt = o.points[p];
+        if (f == "time")
+            return copy_column(pt.time, dst, bytes, name);
+        if (f == "pos")
+            return copy_column(pt.pos, dst, bytes, name);
+        if (f == "dir")
+            return copy_column(pt.dir, dst, bytes, name);
+        if (f == "volume_id")
+            return copy_column(pt.volume_id, dst, bytes, name);
+        if (f == "energy")
+            return copy_column(pt.energy, dst, bytes, name);
+    }
+    return guarded([&] { CELER_VALIDATE(false, << "unknown detector step column '" << n << "'"); });
+}
+}

@@ -451,6 +451,112 @@ int celer_verify_geo_locate(const void* params_ref,
                             uint32_t* out,
                             void* stream);
 
+/*---------------------------------------------------------------------------*
+ * USER ACTIONS: diagnostics and detector step points. Each is created on
+ * the params before the stepper (or state) whose steps it observes; the
+ * handle shares ownership of its actions with the params' registry.
+ *---------------------------------------------------------------------------*/
+typedef struct CelerActionDiagnostic CelerActionDiagnostic;
+typedef struct CelerStepDiagnostic CelerStepDiagnostic;
+typedef struct CelerStepCollector CelerStepCollector;
+typedef struct CelerDetectorSteps CelerDetectorSteps;
+
+/* Replaces ActionDiagnostic::make_and_insert(core)
+ * (src/celeritas/user/ActionDiagnostic.cc:40-52): counts of post-step
+ * actions by particle over valid tracks, at the post-step order. */
+int celer_action_diagnostic_create(CelerParams* params, CelerActionDiagnostic** out);
+void celer_action_diagnostic_destroy(CelerActionDiagnostic* diag);
+/* Shape of the counts: [num_particles][num_actions] (0 before a stepper or
+ * state has been set up with it) */
+int celer_action_diagnostic_shape(const CelerActionDiagnostic* diag,
+                                  uint32_t* num_particles,
+                                  uint32_t* num_actions);
+/* Replaces ActionDiagnostic::calc_actions (ActionDiagnostic.cc:171-195):
+ * counts summed over streams, written row-major into out[size] */
+int celer_action_diagnostic_counts(const CelerActionDiagnostic* diag,
+                                   uint64_t* out,
+                                   size_t size);
+int celer_action_diagnostic_clear(CelerActionDiagnostic* diag);
+
+/* Replaces StepDiagnostic::make_and_insert(core, max_step_bin)
+ * (src/celeritas/user/StepDiagnostic.cc:30-42): steps per killed track by
+ * particle, bins 0..max_step_bin plus overflow. */
+int celer_step_diagnostic_create(CelerParams* params,
+                                 uint32_t max_step_bin,
+                                 CelerStepDiagnostic** out);
+void celer_step_diagnostic_destroy(CelerStepDiagnostic* diag);
+int celer_step_diagnostic_shape(const CelerStepDiagnostic* diag,
+                                uint32_t* num_particles,
+                                uint32_t* num_bins);
+/* Replaces StepDiagnostic::calc_steps (StepDiagnostic.cc:115-135) */
+int celer_step_diagnostic_counts(const CelerStepDiagnostic* diag, uint64_t* out, size_t size);
+int celer_step_diagnostic_clear(CelerStepDiagnostic* diag);
+
+/* Track properties gathered at a step point (StepPointSelection,
+ * src/celeritas/user/StepData.hh:26-60); nonzero selects */
+typedef struct CelerStepPointSelection
+{
+    uint8_t time;
+    uint8_t pos;
+    uint8_t dir;
+    uint8_t volume_id;
+    uint8_t energy;
+} CelerStepPointSelection;
+
+/* Track properties gathered every step (StepSelection, StepData.hh:64-120);
+ * points[0] is the pre-step point, points[1] the post-step point */
+typedef struct CelerStepSelection
+{
+    CelerStepPointSelection points[2];
+    uint8_t event_id;
+    uint8_t parent_id;
+    uint8_t track_step_count;
+    uint8_t action_id;
+    uint8_t step_length;
+    uint8_t particle;
+    uint8_t energy_deposition;
+} CelerStepSelection;
+
+/* Replaces StepCollector::make_and_insert(core, interfaces)
+ * (src/celeritas/user/StepCollector.cc:33-100) for detector hits: inserts
+ * the "step-gather-pre" and "step-gather-post" actions. volume_to_detector
+ * (num_volumes entries, 0xffffffff: not a detector) selects the slots whose
+ * steps are kept; null keeps every active track with detector 0. With
+ * nonzero_energy_deposition, post-step points without deposition are
+ * skipped (StepGatherExecutor.hh:96-105). Hits accumulate on the device,
+ * hit_capacity per stream, until read. */
+int celer_step_collector_create(CelerParams* params,
+                                const CelerStepSelection* selection,
+                                const uint32_t* volume_to_detector,
+                                uint32_t num_volumes,
+                                int nonzero_energy_deposition,
+                                uint32_t hit_capacity,
+                                CelerStepCollector** out);
+void celer_step_collector_destroy(CelerStepCollector* coll);
+/* Replaces copy_steps<device>(DetectorStepOutput*, state)
+ * (src/celeritas/user/DetectorSteps.cu:140-200): copies every hit gathered
+ * on the stream since the last read, in step order and within a step in
+ * slot order, then empties the device buffer. */
+int celer_step_collector_read(CelerStepCollector* coll,
+                              uint32_t stream_id,
+                              CelerDetectorSteps** out);
+void celer_detector_steps_destroy(CelerDetectorSteps* steps);
+/* Number of hits; steps dropped because the buffer was full, and their hits */
+uint32_t celer_detector_steps_size(const CelerDetectorSteps* steps);
+int celer_detector_steps_overflow(const CelerDetectorSteps* steps,
+                                  uint32_t* overflow_steps,
+                                  uint32_t* dropped_hits);
+/* Copy a column (DetectorStepOutput member): "detector", "track_id",
+ * "event_id", "parent_id", "track_step_count", "action_id", "particle"
+ * (uint32), "step_length", "energy_deposition" (double), and per point
+ * "pre." or "post." + "time", "energy" (double), "pos", "dir" (3 doubles
+ * per hit), "volume_id" (uint32; 0xffffffff outside the world). Returns an
+ * error for an unselected column or a too-small buffer. */
+int celer_detector_steps_column(const CelerDetectorSteps* steps,
+                                const char* name,
+                                void* dst,
+                                size_t bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,6 +11,7 @@
 
 #include "core.hh"
 #include "physics.hh"
+#include "user.hh"
 #include "problem.hh"
 #include "rng.hh"
 #include "loop.hh"
@@ -1159,6 +1160,113 @@ int orc_run_events_parallel(void* problem, void const* prim, std::uint32_t nprim
                     calo[d] += d < tallies[t].size() ? tallies[t][d] : 0.0;
             }
         }
+    });
+}
+}
+
+//---------------------------------------------------------------------------//
+// USER OBSERVERS
+//---------------------------------------------------------------------------//
+extern "C" {
+//! Attach observers to a state: action diagnostic over num_actions columns
+//! (0: off), step diagnostic with step_bins bins (0: off), step collector
+//! (collect != 0) with an optional volume -> detector map
+int orc_state_enable_user(void* s, std::uint32_t num_actions, std::uint32_t step_bins,
+                          int collect, std::uint32_t const* detector_map,
+                          std::uint32_t num_volumes, int nonzero_edep)
+{
+    return guard([&] {
+        State& st = *static_cast<OrcState*>(s)->s;
+        auto u = std::make_shared<UserObservers>();
+        std::size_t const np = st.problem->physics->particles.size();
+        if (num_actions)
+        {
+            u->action_diag = true;
+            u->num_actions = num_actions;
+            u->action_counts.assign(np * num_actions, 0);
+        }
+        if (step_bins)
+        {
+            u->step_diag = true;
+            u->step_bins = step_bins;
+            u->step_counts.assign(np * step_bins, 0);
+        }
+        u->collect = collect != 0;
+        if (detector_map)
+            u->detector_map.assign(detector_map, detector_map + num_volumes);
+        u->nonzero_edep = nonzero_edep != 0;
+        st.user = std::move(u);
+    });
+}
+
+//! Copy the action (which 0) or step (which 1) diagnostic counts
+int orc_state_user_counts(void* s, int which, std::uint64_t* out, std::uint32_t size)
+{
+    return guard([&] {
+        State& st = *static_cast<OrcState*>(s)->s;
+        if (!st.user)
+            throw std::runtime_error("oracle: no user observers");
+        auto const& v = which == 0 ? st.user->action_counts : st.user->step_counts;
+        if (size < v.size())
+            throw std::runtime_error("oracle: counts buffer too small");
+        std::copy(v.begin(), v.end(), out);
+    });
+}
+
+//! Plant a deliberate mistake in the user observers (0: none)
+void orc_user_set_mutation(int m)
+{
+    orc::g_user_mutation = m;
+}
+
+//! Number of collected hits
+std::uint32_t orc_state_num_hits(void* s)
+{
+    State& st = *static_cast<OrcState*>(s)->s;
+    return st.user ? static_cast<std::uint32_t>(st.user->hits.size()) : 0;
+}
+
+//! Copy and clear the hits: 9 u32 per hit (detector, track, event, parent,
+//! step count, action, particle, pre volume, post volume) and 18 doubles
+//! (step length, edep, then per point time, pos[3], dir[3], energy)
+int orc_state_take_hits(void* s, std::uint32_t* u32, double* f64, std::uint32_t size)
+{
+    return guard([&] {
+        State& st = *static_cast<OrcState*>(s)->s;
+        if (!st.user)
+            throw std::runtime_error("oracle: no user observers");
+        auto& hits = st.user->hits;
+        if (size < hits.size())
+            throw std::runtime_error("oracle: hit buffer too small");
+        for (std::size_t i = 0; i < hits.size(); ++i)
+        {
+            StepHit const& h = hits[i];
+            std::uint32_t* u = u32 + 9 * i;
+            u[0] = h.detector;
+            u[1] = h.track_id;
+            u[2] = h.event_id;
+            u[3] = h.parent_id;
+            u[4] = h.track_step_count;
+            u[5] = h.action_id;
+            u[6] = h.particle;
+            u[7] = h.volume[0];
+            u[8] = h.volume[1];
+            double* f = f64 + 18 * i;
+            f[0] = h.step_length;
+            f[1] = h.edep;
+            for (int p = 0; p < 2; ++p)
+            {
+                double* q = f + 2 + 8 * p;
+                q[0] = h.time[p];
+                for (int j = 0; j < 3; ++j)
+                {
+                    q[1 + j] = h.pos[p][j];
+                    q[4 + j] = h.dir[p][j];
+                }
+                q[7] = h.energy[p];
+            }
+        }
+        hits.clear();
     });
 }
 }

@@ -105,6 +105,7 @@ struct Counters
 };
 
 struct Problem;  // geometry + physics (defined by the stepping-loop oracle)
+struct UserObservers;  // diagnostics and step gather (user.hh)
 
 struct State
 {
@@ -121,6 +122,8 @@ struct State
     bool in_place = true;  // TrackOrder != init_charge
     std::vector<double> calo_edep;
     double total_edep = 0;  //!< deposited in all volumes [MeV]
+    //! Optional user observers (null: none)
+    std::shared_ptr<UserObservers> user;
 };
 
 //! Build an empty state (all slots vacant, RNG seeded from the MT stream)

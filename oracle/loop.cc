@@ -8,6 +8,7 @@
 
 #include "physics.hh"
 #include "problem.hh"
+#include "user.hh"
 
 namespace orc
 {
@@ -48,6 +49,7 @@ StepResult step_once(State& s)
     extend_from_primaries(s);
     initialize_tracks(s);
     pre_step(s);
+    user_gather_pre(s);
     along_step_neutral(s);
     if (ids.along_user != ids.along_neutral)
         along_step_charged(s);
@@ -58,7 +60,9 @@ StepResult step_once(State& s)
         interact(s, m);
     boundary(s);
     tracking_cut(s);
+    user_action_diag(s);
     calo_tally(s);
+    user_post(s);
     extend_from_secondaries(s);
     return {s.counters.num_generated, s.counters.num_initializers,
             s.counters.num_active, s.counters.num_alive};

@@ -197,6 +197,67 @@ class State:
         w = np.ascontiguousarray(words, dtype=np.uint32)
         load().orc_state_set_rng(self._h, w.ctypes.data)
 
+    # User observers (oracle/user.hh)
+    def enable_user(self, num_actions: int = 0, step_bins: int = 0, collect: bool = False,
+                    volume_to_detector=None, nonzero_edep: bool = False):
+        f = load().orc_state_enable_user
+        f.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
+                      ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int]
+        f.restype = ctypes.c_int
+        vmap = None
+        if volume_to_detector is not None:
+            vmap = np.ascontiguousarray(
+                [0xFFFFFFFF if d is None or d < 0 else d for d in volume_to_detector],
+                dtype=np.uint32)
+        _check(f(self._h, num_actions, step_bins, int(collect),
+                 None if vmap is None else vmap.ctypes.data,
+                 0 if vmap is None else len(vmap), int(nonzero_edep)))
+        self._num_actions = num_actions
+        self._step_bins = step_bins
+        self._vmap = vmap
+
+    def _user_counts(self, which: int, width: int) -> np.ndarray:
+        f = load().orc_state_user_counts
+        f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint32]
+        f.restype = ctypes.c_int
+        out = np.zeros(64 * width, dtype=np.uint64)
+        _check(f(self._h, which, out.ctypes.data, len(out)))
+        return out
+
+    def action_counts(self, num_particles: int) -> np.ndarray:
+        out = self._user_counts(0, self._num_actions)
+        return out[: num_particles * self._num_actions].reshape(num_particles, -1)
+
+    def step_counts(self, num_particles: int) -> np.ndarray:
+        out = self._user_counts(1, self._step_bins)
+        return out[: num_particles * self._step_bins].reshape(num_particles, -1)
+
+    def take_hits(self) -> dict:
+        """Copy and clear the collected hits as DetectorStepOutput-like
+        columns (every field; the caller picks the selected ones)."""
+        n = load().orc_state_num_hits
+        n.argtypes = [ctypes.c_void_p]
+        n.restype = ctypes.c_uint32
+        count = n(self._h)
+        u = np.zeros((count, 9), dtype=np.uint32)
+        d = np.zeros((count, 18), dtype=np.float64)
+        f = load().orc_state_take_hits
+        f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]
+        f.restype = ctypes.c_int
+        _check(f(self._h, u.ctypes.data if count else None, d.ctypes.data if count else None,
+                 count))
+        out = {"detector": u[:, 0], "track_id": u[:, 1], "event_id": u[:, 2],
+               "parent_id": u[:, 3], "track_step_count": u[:, 4], "action_id": u[:, 5],
+               "particle": u[:, 6], "pre.volume_id": u[:, 7], "post.volume_id": u[:, 8],
+               "step_length": d[:, 0], "energy_deposition": d[:, 1]}
+        for i, p in enumerate(("pre", "post")):
+            q = d[:, 2 + 8 * i: 10 + 8 * i]
+            out[f"{p}.time"] = q[:, 0]
+            out[f"{p}.pos"] = q[:, 1:4]
+            out[f"{p}.dir"] = q[:, 4:7]
+            out[f"{p}.energy"] = q[:, 7]
+        return out
+
 
 def rng_draw_batch(states: np.ndarray, draws: int, kind: str = "u32"):
     k = {"u32": 0, "f64": 1, "f32": 2}[kind]

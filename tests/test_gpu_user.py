@@ -1,0 +1,161 @@
+"""Device user actions (diagnostics, step collector) against the reference's
+golden values and the oracle.
+
+- SimpleCompton diagnostics and the KN MC-truth points reproduce the
+  reference's golden values (Diagnostic.test.cc:110-131,
+  StepCollector.test.cc:179-226), as the oracle does.
+- TestEm3 with full physics: over the first steps (where device and oracle
+  states agree) the action and step tallies are identical and the detector
+  hits (lAr gaps, steps with deposition only) match the oracle in number,
+  order and integer fields, reals to 1e-9.
+- A hit buffer too small for a step drops that step's hits and reports it;
+  graph-replayed steps gather the same hits as eager ones.
+"""
+
+import numpy as np
+import pytest
+
+from celeritas_amd import user
+from celeritas_amd import workloads as wl
+from tests import oracle_bind as ob
+from tests import problems as pb
+from tests import test_oracle_user as tou
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(stepper, primaries, num_steps):
+    r = stepper(primaries)
+    while r and num_steps > 1:
+        num_steps -= 1
+        r = stepper()
+
+
+def test_simple_compton_diagnostics_device(amd):
+    params = amd.CoreParams(pb.simple_compton())
+    adiag = user.ActionDiagnostic(params)
+    sdiag = user.StepDiagnostic(params, 20)
+    stepper = amd.Stepper(params, 256)
+    _run(stepper, tou._gammas(256, [-22.0, 0, 0], False), 32)
+    assert adiag.calc_actions_map() == tou.EXPECTED_KEYS
+    assert sdiag.calc_steps().ravel().tolist() == tou.EXPECTED_STEPS
+
+
+def _mctruth_device(amd, num_tracks, num_steps):
+    params = amd.CoreParams(pb.simple_compton())
+    sel = user.StepSelection(pre=user.StepPointSelection(pos=True, dir=True, volume_id=True),
+                             event_id=True, track_step_count=True)
+    coll = user.StepCollector(params, sel, hit_capacity=1024)
+    stepper = amd.Stepper(params, num_tracks)
+    _run(stepper, tou._gammas(num_tracks, [0.0, 0, 0], True), num_steps)
+    h = coll.read()
+    assert h["overflow_steps"] == 0
+    order = np.lexsort((h["track_step_count"], h["track_id"], h["event_id"]))
+    return {k: (v[order] if isinstance(v, np.ndarray) else v) for k, v in h.items()}
+
+
+def test_kn_mctruth_device(amd):
+    h = _mctruth_device(amd, 8, 1)
+    np.testing.assert_array_equal(h["event_id"], range(8))
+    np.testing.assert_array_equal(h["track_step_count"], [1] * 8)
+    np.testing.assert_array_equal(h["pre.volume_id"], [1] * 8)
+    np.testing.assert_array_equal(h["pre.dir"].ravel(), [1, 0, 0] * 8)
+    h = _mctruth_device(amd, 4, 2)
+    np.testing.assert_array_equal(h["event_id"], [0, 0, 1, 1, 2, 2, 3, 3])
+    np.testing.assert_array_equal(h["track_id"], [0] * 8)
+    np.testing.assert_array_equal(h["track_step_count"], [1, 2] * 4)
+    np.testing.assert_array_equal(h["pre.volume_id"], [1, 1, 1, 1, 1, 2, 1, 2])
+    np.testing.assert_allclose(h["pre.pos"].ravel(), tou.TWO_STEP_POS, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(h["pre.dir"].ravel(), tou.TWO_STEP_DIR, rtol=1e-12, atol=1e-14)
+
+
+@pytest.fixture(scope="module")
+def testem3():
+    import celeritas_amd as amd_pkg
+
+    physics = amd_pkg.generate_physics(wl.testem3_physics_spec())
+    return wl.testem3_problem(physics=physics, seed=20220511, init_capacity=1 << 18,
+                              max_events=1 << 12)
+
+
+def _gap_detectors(prob):
+    names = prob["geometry"]["universes"][0]["cell_names"]
+    return [int(n[4:]) if n.startswith("gap_") else -1 for n in names]
+
+
+INT_COLUMNS = ("detector", "track_id", "event_id", "parent_id", "track_step_count",
+               "action_id", "particle", "pre.volume_id", "post.volume_id")
+REAL_COLUMNS = ("step_length", "energy_deposition", "pre.time", "pre.pos", "pre.dir",
+                "pre.energy", "post.time", "post.pos", "post.dir", "post.energy")
+
+
+def test_testem3_user_actions_match_oracle(amd, testem3):
+    params = amd.CoreParams(testem3)
+    labels = params.action_labels()
+    adiag = user.ActionDiagnostic(params)
+    sdiag = user.StepDiagnostic(params, 20)
+    dets = _gap_detectors(testem3)
+    coll = user.StepCollector(params, user.StepSelection.all(), volume_to_detector=dets,
+                              nonzero_energy_deposition=True, hit_capacity=1 << 16)
+    stepper = amd.Stepper(params, 4096)
+    oprob = ob.Problem(testem3)
+    ost = ob.State(oprob, 4096)
+    olabels = oprob.action_labels()
+    ost.enable_user(num_actions=len(olabels), step_bins=22, collect=True,
+                    volume_to_detector=dets, nonzero_edep=True)
+    prim = wl.testem3_primaries(8, emin=1e3, emax=1e4, seed=3)
+    total = 0
+    for step in range(16):
+        r = stepper(prim) if step == 0 else stepper()
+        if step == 0:
+            ost.insert_primaries(prim)
+        o = ob.step_once(ost)
+        assert dict(r) == o
+        h = coll.read()
+        oh = ost.take_hits()
+        print(f"step {step}: {len(h['detector'])} device hits, {len(oh['detector'])} oracle",
+              flush=True)
+        assert h["overflow_steps"] == 0
+        for c in INT_COLUMNS:
+            np.testing.assert_array_equal(h[c], oh[c], err_msg=f"step {step} column {c}")
+        for c in REAL_COLUMNS:
+            np.testing.assert_allclose(h[c], oh[c], rtol=1e-9, atol=1e-9,
+                                       err_msg=f"step {step} column {c}")
+        total += len(h["detector"])
+    assert total > 0
+    # Tallies: the device registry has the oracle's actions first, plus the
+    # user actions (never a post-step action)
+    npart = len(user.particle_labels(params))
+    dev = adiag.calc_actions()
+    assert labels[: len(olabels)] == olabels
+    np.testing.assert_array_equal(dev[:, : len(olabels)], ost.action_counts(npart))
+    assert not dev[:, len(olabels):].any()
+    np.testing.assert_array_equal(sdiag.calc_steps(), ost.step_counts(npart))
+
+
+def test_hit_buffer_overflow_and_graph_replay(amd, testem3):
+    dets = _gap_detectors(testem3)
+    sel = user.StepSelection(post=user.StepPointSelection(energy=True), energy_deposition=True)
+    prim = wl.testem3_primaries(8, emin=1e3, emax=1e4, seed=5)
+
+    def run(capacity, graph):
+        params = amd.CoreParams(testem3)
+        coll = user.StepCollector(params, sel, volume_to_detector=dets,
+                                  hit_capacity=capacity)
+        stepper = amd.Stepper(params, 4096)
+        if graph:
+            stepper.capture_step()
+        _run(stepper, prim, 12)
+        return coll.read()
+
+    eager = run(1 << 16, False)
+    replay = run(1 << 16, True)
+    assert eager["overflow_steps"] == 0 and len(eager["detector"]) > 0
+    for c in ("detector", "track_id", "energy_deposition", "post.energy"):
+        np.testing.assert_array_equal(eager[c], replay[c])
+    # A buffer smaller than the hits drops whole steps and counts them
+    small = run(64, False)
+    n = len(small["detector"])
+    assert small["overflow_steps"] > 0
+    assert n + small["dropped_hits"] == len(eager["detector"])
+    assert n <= 64
