@@ -592,6 +592,32 @@ def verify_geo_locate(params: CoreParams, points) -> np.ndarray:
     return out
 
 
+def verify_geo_trace(params: CoreParams, rays, max_segments: int = 64):
+    """Trace rays (rows of pos[3], dir[3]) through the geometry on the
+    device until each leaves the world: lists of (volumes, distances,
+    midpoint safeties) per ray."""
+    lib_ = _load()
+    f = lib_.celer_verify_geo_trace
+    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
+                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                  ctypes.c_void_p]
+    f.restype = ctypes.c_int
+    lib_.celer_params_device_ref.restype = ctypes.c_void_p
+    lib_.celer_params_device_ref.argtypes = [ctypes.c_void_p]
+    r = np.ascontiguousarray(rays, np.float64).reshape(-1, 6)
+    n, m = len(r), int(max_segments)
+    counts = np.zeros(n, np.uint32)
+    vols = np.zeros(n * m, np.uint32)
+    dists = np.zeros(n * m, np.float64)
+    safe = np.zeros(n * m, np.float64)
+    rc = f(lib_.celer_params_device_ref(params.handle), r.ctypes.data, n, m,
+           counts.ctypes.data, vols.ctypes.data, dists.ctypes.data, safe.ctypes.data, None)
+    if rc != 0:
+        raise CeleritasError(f"celer_verify_geo_trace failed with HIP error {rc}")
+    return [(vols[i * m:i * m + c], dists[i * m:i * m + c], safe[i * m:i * m + c])
+            for i, c in enumerate(counts)]
+
+
 def generate_physics(spec: dict | str) -> dict:
     """Expand a compact physics description into full tables (host only).
 
@@ -609,3 +635,41 @@ def generate_physics(spec: dict | str) -> dict:
     buf = ctypes.create_string_buffer(needed.value)
     _check(f(text, buf, needed.value, ctypes.byref(needed)))
     return json.loads(buf.value.decode())
+
+
+def _read_json_text(fn, *args) -> dict:
+    """Call a C entry point that fills a caller buffer with JSON text."""
+    fn.restype = ctypes.c_int
+    needed = ctypes.c_size_t(0)
+    _check(fn(*args, None, 0, ctypes.byref(needed)))
+    buf = ctypes.create_string_buffer(needed.value)
+    _check(fn(*args, buf, needed.value, ctypes.byref(needed)))
+    return json.loads(buf.value.decode())
+
+
+def read_seltzer_berger(z: int, directory: str | None = None) -> dict:
+    """Seltzer-Berger table of element ``z`` from a G4EMLOW ``brem_SB``
+    folder (``$G4LEDATA/brem_SB`` when ``directory`` is None), as the
+    element's ``sb_table`` ({"x", "y", "values"}).
+
+    Mirrors ``SeltzerBergerReader::operator()(AtomicNumber)``
+    (io/SeltzerBergerReader.cc:54-102).
+    """
+    f = _load().celer_read_seltzer_berger
+    f.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t,
+                  ctypes.POINTER(ctypes.c_size_t)]
+    return _read_json_text(f, directory.encode() if directory else None, int(z))
+
+
+def read_livermore_pe(z: int, directory: str | None = None) -> dict:
+    """Livermore photoelectric data of element ``z`` from a G4EMLOW
+    ``livermore/phot_epics2014`` folder (from ``$G4LEDATA`` when
+    ``directory`` is None), as the element's ``livermore_pe`` object.
+
+    Mirrors ``LivermorePEReader::operator()(AtomicNumber)``
+    (io/LivermorePEReader.cc:50-209).
+    """
+    f = _load().celer_read_livermore_pe
+    f.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t,
+                  ctypes.POINTER(ctypes.c_size_t)]
+    return _read_json_text(f, directory.encode() if directory else None, int(z))

@@ -15,6 +15,7 @@
 #include "global/CoreState.hh"
 #include "phys/TableGen.hh"
 #include "global/Stepper.hh"
+#include "io/G4EmlowReaders.hh"
 #include "track/TrackActions.hh"
 #include "track/TrackKernels.hh"
 #include "user/UserActions.hh"
@@ -83,6 +84,17 @@ int guarded(F&& f)
         g_last_error = "unknown C++ exception";
     }
     return 1;
+}
+
+//! Copy text into a caller buffer, reporting the size it needs (with NUL)
+void copy_text(std::string const& s, char* buf, std::size_t len, std::size_t* needed)
+{
+    if (needed)
+        *needed = s.size() + 1;
+    CELER_VALIDATE(buf == nullptr || len > s.size(),
+                   << "output buffer too small (" << s.size() + 1 << " bytes needed)");
+    if (buf)
+        std::memcpy(buf, s.c_str(), s.size() + 1);
 }
 
 template<class T, class F>
@@ -705,13 +717,23 @@ int celer_hip_reseed(const void* params_ref,
 int celer_generate_physics(const char* spec_json, char* buf, size_t len, size_t* needed)
 {
     return guarded([&] {
-        std::string s = dump_json(generate_physics(parse_json(spec_json)));
-        if (needed)
-            *needed = s.size() + 1;
-        CELER_VALIDATE(buf == nullptr || len > s.size(),
-                       << "output buffer too small (" << s.size() + 1 << " bytes needed)");
-        if (buf)
-            std::memcpy(buf, s.c_str(), s.size() + 1);
+        copy_text(dump_json(generate_physics(parse_json(spec_json))), buf, len, needed);
+    });
+}
+
+int celer_read_seltzer_berger(const char* dir, int z, char* buf, size_t len, size_t* needed)
+{
+    return guarded([&] {
+        SBTable const t = dir ? SeltzerBergerReader(dir)(z) : SeltzerBergerReader()(z);
+        copy_text(dump_json(to_json(t)), buf, len, needed);
+    });
+}
+
+int celer_read_livermore_pe(const char* dir, int z, char* buf, size_t len, size_t* needed)
+{
+    return guarded([&] {
+        JsonValue const r = dir ? LivermorePEReader(dir)(z) : LivermorePEReader()(z);
+        copy_text(dump_json(r), buf, len, needed);
     });
 }
 

@@ -1,11 +1,12 @@
 """Problem descriptions of the calorimeter workloads.
 
-TestEm3 sampling calorimeter (geometry: data/testem3-flat.org.json, the
-reference's test/geocel/data/testem3-flat.org.json): 50 layers along x of
-2.3 mm Pb followed by 5.7 mm liquid argon, 40 x 40 cm transverse, from
-x = -20 cm, inside a 48 cm vacuum cube. The reference's ORANGE file assigns
-gap = lAr and absorber = Pb, while its GDML (testem3-flat.gdml:77-84) has
-gap = Pb (2.3 mm) and absorber = lAr (5.7 mm); the GDML assignment is used.
+TestEm3 sampling calorimeter (built here in ORANGE form, as the reference's
+test/geocel/data/testem3-flat.org.py lays it out): 50 layers along x of
+2.3 mm Pb ("gap_i") followed by 5.7 mm liquid argon ("absorber_i"),
+40 x 40 cm transverse, from x = -20 cm, inside a 48 cm vacuum cube. The
+reference's ORANGE file assigns gap = lAr and absorber = Pb, while its GDML
+(testem3-flat.gdml:77-84) has gap = Pb (2.3 mm) and absorber = lAr
+(5.7 mm); the GDML assignment is used.
 
 Physics: standard EM (Compton, Livermore photoelectric, Bethe-Heitler
 conversion, e+ annihilation, Moller-Bhabha ionisation, combined
@@ -17,20 +18,113 @@ published binding energies with a synthetic photoelectric shell model.
 
 from __future__ import annotations
 
-import json
 import math
-import os
 
 import numpy as np
 
-_DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
-
 ELECTRON, POSITRON, GAMMA = 0, 1, 2
 
+#: TestEm3 layout [cm]: world half-width, calorimeter half-width, layers,
+#: Pb and lAr thicknesses
+TESTEM3_WORLD_HALF = 24.0
+TESTEM3_CALO_HALF = 20.0
+TESTEM3_LAYERS = 50
+TESTEM3_PB = 0.23
+TESTEM3_LAR = 0.57
 
-def load_orange(name: str) -> dict:
-    with open(os.path.join(_DATA, name)) as f:
-        return json.load(f)
+
+def _cell(faces_senses, flags=None):
+    """A simple-unit cell from (surface, inside) pairs: the intersection of
+    the half-spaces, with faces in surface order (ORANGE's cell layout)."""
+    faces = sorted(f for f, _ in faces_senses)
+    idx = {f: k for k, f in enumerate(faces)}
+    terms = []
+    for f, inside in sorted(faces_senses):
+        terms.append(f"{idx[f]}" + (" ~" if inside else ""))
+    logic = terms[0] + "".join(f" {t} &" for t in terms[1:])
+    cell = {"faces": faces, "logic": logic, "num_intersections": len(faces),
+            "zorder": 2}
+    if flags is not None:
+        cell["flags"] = flags
+    return cell
+
+
+def testem3_geometry() -> dict:
+    """ORANGE-format TestEm3 geometry (one simple unit): the exterior, the
+    world (inside the world cube, outside the calorimeter), then gap_i and
+    absorber_i for each layer. Each layer's planes are placed from its own
+    start, front + i * pitch, Pb then lAr, and the last layer ends on the
+    calorimeter's back face."""
+    types, data, names = [], [], []
+
+    def add(kind, value, name):
+        # Rounding residue at the origin is written as 0, as ORANGE's
+        # surface simplification does for near-zero plane positions
+        types.append(kind)
+        data.append(0.0 if abs(value) < 1e-12 else float(value))
+        names.append(name)
+        return len(types) - 1
+
+    w, c = TESTEM3_WORLD_HALF, TESTEM3_CALO_HALF
+    world = [add(k, s * w, f"world.{m}{k[1]}") for k in ("px", "py", "pz")
+             for s, m in ((-1, "m"), (1, "p"))]
+    calo_front = add("px", -c, "calorimeter.mx")
+    calo_back = add("px", c, "calorimeter.px")
+    calo_y = [add("py", -c, "calorimeter.my"), add("py", c, "calorimeter.py")]
+    calo_z = [add("pz", -c, "calorimeter.mz"), add("pz", c, "calorimeter.pz")]
+    # Planes between layers: after the Pb of layer i, then after its lAr
+    # (each layer starts at its own offset from the front face)
+    pitch = TESTEM3_PB + TESTEM3_LAR
+    front_x = -TESTEM3_LAYERS * pitch / 2
+    planes = []
+    for i in range(TESTEM3_LAYERS):
+        x = front_x + pitch * i + TESTEM3_PB
+        mid = add("px", x, f"layer_{i}.pb_lar")
+        x += TESTEM3_LAR
+        end = calo_back if i + 1 == TESTEM3_LAYERS else add("px", x, f"layer_{i}.back")
+        planes.append((mid, end))
+
+    def box(xlo, xhi):
+        return [(xlo, False), (xhi, True), (calo_y[0], False), (calo_y[1], True),
+                (calo_z[0], False), (calo_z[1], True)]
+
+    world_box = [(world[0], False), (world[1], True), (world[2], False),
+                 (world[3], True), (world[4], False), (world[5], True)]
+    exterior = _cell(world_box, flags=1)
+    # ORANGE writes the exterior as the complement of the world box
+    exterior["logic"] += " ~"
+    calo = box(calo_front, calo_back)
+    wfaces = sorted(f for f, _ in world_box + calo)
+    idx = {f: k for k, f in enumerate(wfaces)}
+
+    def conj(pairs):
+        terms = [f"{idx[f]}" + (" ~" if inside else "") for f, inside in sorted(pairs)]
+        return terms[0] + "".join(f" {t} &" for t in terms[1:])
+
+    world_cell = {"faces": wfaces, "flags": 1,
+                  "logic": conj(world_box) + " " + conj(calo) + " ~ &",
+                  "num_intersections": len(wfaces), "zorder": 2}
+    cells, cell_names = [exterior, world_cell], ["[EXTERIOR]", "world"]
+    front = calo_front
+    for i, (mid, end) in enumerate(planes):
+        cells.append(_cell(box(front, mid)))
+        cell_names.append(f"gap_{i}")
+        cells.append(_cell(box(mid, end)))
+        cell_names.append(f"absorber_{i}")
+        front = end
+    return {
+        "_format": "SCALE ORANGE",
+        "_version": 0,
+        "universes": [{
+            "_type": "simple unit",
+            "bbox": [[-w, -w, -w], [w, w, w]],
+            "cell_names": cell_names,
+            "cells": cells,
+            "md": {"name": "world"},
+            "surface_names": names,
+            "surfaces": {"data": data, "sizes": [1] * len(types), "types": types},
+        }],
+    }
 
 
 def testem3_physics_spec() -> dict:
@@ -68,7 +162,7 @@ def testem3_problem(physics: dict | None = None, seed: int = 20220511,
                     max_events: int = 1 << 16, track_order: str = "none",
                     precision: str = "fp64") -> dict:
     """TestEm3 problem; pass expanded ``physics`` to share tables."""
-    geo = load_orange("testem3-flat.org.json")
+    geo = testem3_geometry()
     names = geo["universes"][0]["cell_names"]
     vol_to_mat, vol_to_det = [], []
     for name in names:

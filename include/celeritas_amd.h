@@ -188,6 +188,21 @@ uint32_t celer_params_num_volumes(const CelerParams* params);
  */
 int celer_generate_physics(const char* spec_json, char* buf, size_t len,
                            size_t* needed);
+/*
+ * Read one element's G4EMLOW data as JSON (host only). dir is the folder
+ * holding the files, or NULL to use $G4LEDATA/brem_SB (Seltzer-Berger) or
+ * $G4LEDATA/livermore/phot_epics2014 (Livermore photoelectric). The result
+ * is the element's "sb_table" ({x, y, values}) or "livermore_pe" object
+ * ({xs_hi, xs_lo, thresh_lo, thresh_hi, shells}); physics specs name the
+ * same folders under "g4emlow" to build tables from them.
+ * Replaces io/SeltzerBergerReader.hh:28-52 (SeltzerBergerReader::operator())
+ * and io/LivermorePEReader.hh:27-52 (LivermorePEReader::operator()).
+ * With buf == NULL only *needed (bytes incl. NUL) is set.
+ */
+int celer_read_seltzer_berger(const char* dir, int z, char* buf, size_t len,
+                              size_t* needed);
+int celer_read_livermore_pe(const char* dir, int z, char* buf, size_t len,
+                            size_t* needed);
 /* Serialize host-side tables (physics, geometry) as JSON for checkers */
 int celer_params_export_json(const CelerParams* params, char* buf, size_t len,
                              size_t* needed);
@@ -450,6 +465,24 @@ int celer_verify_geo_locate(const void* params_ref,
                             uint32_t n,
                             uint32_t* out,
                             void* stream);
+/*
+ * VERIFICATION: ray tracing as the linear propagator drives the geometry
+ * (OrangeTrackView::find_next_step / move_to_boundary / cross_boundary /
+ * find_safety, orange/OrangeTrackView.hh:555-775). Each of n rays is 6
+ * doubles (pos [cm], dir). Per ray, up to max_segments segments are
+ * written at [ray * max_segments + k]: the volume, the distance to its
+ * boundary and the safety distance at the segment's midpoint; counts[ray]
+ * is the number of segments (tracing stops on leaving the world).
+ */
+int celer_verify_geo_trace(const void* params_ref,
+                           const double* rays,
+                           uint32_t n,
+                           uint32_t max_segments,
+                           uint32_t* counts,
+                           uint32_t* volumes,
+                           double* distances,
+                           double* safeties,
+                           void* stream);
 
 /*---------------------------------------------------------------------------*
  * USER ACTIONS: diagnostics and detector step points. Each is created on

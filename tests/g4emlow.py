@@ -1,5 +1,7 @@
-"""Readers for the G4EMLOW ASCII data files the reference's tests hold
-(copied as fixtures under tests/golden/g4emlow/).
+"""Independent numpy parsers of the G4EMLOW ASCII data files the
+reference's tests hold (copied as fixtures under tests/golden/g4emlow/),
+used to cross-check the library's own readers (celeritas_amd
+read_seltzer_berger / read_livermore_pe).
 
 The Seltzer-Berger format follows the reference reader
 (src/celeritas/io/SeltzerBergerReader.cc): a header "dummy ny nx", the ny

@@ -109,3 +109,17 @@ def interactor_host():
 
 
 ELECTRON, POSITRON, GAMMA_ID = 0, 1, 2
+
+
+def geometry_only(name):
+    """A geometry with particle definitions and no physics, for tracking
+    checks: every volume but the exterior maps to material 0."""
+    geom = load_geometry(name)
+    nvol = len(geom["universes"][0]["cell_names"])
+    return {
+        "geometry": geom,
+        "volume_to_material": [-1] + [0] * (nvol - 1),
+        "physics": {"particles": [GAMMA, ELECTRON_HALF]},
+        "track_init": {"capacity": 1024, "max_events": 16, "track_order": "none"},
+        "seed": 1,
+    }

@@ -89,3 +89,74 @@ def test_cylinder_locate_matches_oracle(amd):
 
 def test_two_boxes_locate_matches_oracle(amd):
     _compare(amd, problems.two_boxes_track_init(), 20_000, seed=3)
+
+
+def _trace_compare(amd, prob, rays, min_segments):
+    """Device ray traces against the oracle's: volumes exact, distances and
+    midpoint safeties to 1e-12 relative."""
+    params = amd.CoreParams(prob)
+    dev = amd_pkg.verify_geo_trace(params, rays, max_segments=256)
+    oprob = ob.Problem(prob)
+    nseg = 0
+    for i, r in enumerate(rays):
+        vols, dists, safe = oprob.trace(r[:3], r[3:], max_segments=256, safety=True)
+        dv, dd, ds = dev[i]
+        assert np.array_equal(dv, vols), f"ray {i} {r}: volumes {dv} != {vols}"
+        np.testing.assert_allclose(dd, dists, rtol=1e-12, atol=1e-13, err_msg=f"ray {i}")
+        np.testing.assert_allclose(ds, safe, rtol=1e-12, atol=1e-13, err_msg=f"ray {i}")
+        nseg += len(vols)
+    assert nseg >= min_segments
+    return dev
+
+
+def _random_rays(geometry, n, seed, shrink=0.9):
+    rng = np.random.default_rng(seed)
+    lo, hi = (np.asarray(b, dtype=np.float64) for b in geometry["universes"][0]["bbox"])
+    c, h = (lo + hi) / 2, shrink * (hi - lo) / 2
+    pos = c + h * (2 * rng.random((n, 3)) - 1)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    return np.hstack([pos, d])
+
+
+def test_simple_cms_trace_golden(amd):
+    """SimpleCmsGeantTest.trace (orange/OrangeGeant.test.cc:170-200) on the
+    device; the last safety of the second ray is the .org.json world
+    cell's (650), as the oracle test explains."""
+    prob = problems.geometry_only("simple-cms.org.json")
+    names = prob["geometry"]["universes"][0]["cell_names"]
+    params = amd.CoreParams(prob)
+    (v1, d1, s1), (v2, d2, s2) = amd_pkg.verify_geo_trace(
+        params, [[-75, 0, 0, 1, 0, 0], [25, 0, 701, 0, 0, -1]])
+    assert [names[v] for v in v1] == [
+        "si_tracker", "vacuum_tube", "si_tracker", "em_calorimeter", "had_calorimeter",
+        "sc_solenoid", "fe_muon_chambers", "world"]
+    np.testing.assert_allclose(d1, [45, 60, 95, 50, 100, 100, 325, 300], rtol=1e-12)
+    np.testing.assert_allclose(s1, [22.5, 700, 47.5, 25, 50, 50, 162.5, 150], rtol=1e-12)
+    assert [names[v] for v in v2] == ["world", "vacuum_tube", "world"]
+    np.testing.assert_allclose(d2, [1, 1400, 1300], rtol=1e-12)
+    np.testing.assert_allclose(s2, [0.5, 5, 650], rtol=1e-12)
+
+
+def test_simple_cms_trace_matches_oracle(amd):
+    prob = problems.geometry_only("simple-cms.org.json")
+    rays = _random_rays(prob["geometry"], 2000, seed=11)
+    _trace_compare(amd, prob, rays, min_segments=4000)
+
+
+def test_testem3_trace_matches_oracle(amd):
+    physics = amd_pkg.generate_physics(wl.testem3_physics_spec())
+    prob = wl.testem3_problem(physics=physics)
+    rays = _random_rays(prob["geometry"], 1000, seed=12)
+    # Rays along +x through every layer as well
+    y = np.linspace(-19, 19, 50)
+    along = np.stack([np.full(50, -23.0), y, -y / 2, np.ones(50), np.zeros(50),
+                      np.zeros(50)], axis=1)
+    _trace_compare(amd, prob, np.vstack([rays, along]), min_segments=50 * 100)
+
+
+def test_cylinder_trace_matches_oracle(amd):
+    physics = amd_pkg.generate_physics(wl.cylinder_physics_spec())
+    prob = wl.cylinder_problem(physics=physics)
+    rays = _random_rays(prob["geometry"], 1000, seed=13)
+    _trace_compare(amd, prob, rays, min_segments=2000)
