@@ -390,11 +390,17 @@ GeoParams::GeoParams(GeoInput const& inp)
             }
         }
         CELER_VALIDATE(stack.size() == 1, << "unbalanced logic in volume " << v);
+        // The boxes only prune candidates, so they are bumped outward (as
+        // ORANGE bumps its BIH boxes): a position moved onto a face can
+        // land a few ulps outside the exact box of the volume it enters
         Box const& bb = stack.back().first;
+        auto bump = [](double x) {
+            return std::isfinite(x) ? 1e-8 * std::max(1.0, std::fabs(x)) : 0.0;
+        };
         for (int k = 0; k < 3; ++k)
-            h_bbox.push_back(bb.lo[k]);
+            h_bbox.push_back(bb.lo[k] - bump(bb.lo[k]));
         for (int k = 0; k < 3; ++k)
-            h_bbox.push_back(bb.hi[k]);
+            h_bbox.push_back(bb.hi[k] + bump(bb.hi[k]));
     }
     // Point-location tree over the non-implicit volumes with a finite,
     // non-empty box; unbounded ones are tested after it, empty ones never

@@ -56,7 +56,8 @@ def tmp(tmp_path_factory):
 def host_loop(tmp):
     exe = tmp / "host_loop"
     cmd = ["g++", *FLAGS, os.path.join(NATIVE, "host_loop.cc"),
-           os.path.join(NATIVE, "host_loop_mixed.cc"), *product_sources(),
+           os.path.join(NATIVE, "host_loop_mixed.cc"),
+           os.path.join(NATIVE, "hip_host_stubs.cc"), *product_sources(),
            "-Wl,--unresolved-symbols=ignore-all", "-o", str(exe)]
     subprocess.run(cmd, check=True, capture_output=True, timeout=900)
     return exe
