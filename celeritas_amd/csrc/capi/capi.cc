@@ -11,12 +11,14 @@
 
 #include "accel/LocalTransporter.hh"
 #include "accel/ServingLoop.hh"
+#include "global/CoreActions.hh"
 #include "global/CoreParams.hh"
 #include "global/CoreState.hh"
 #include "phys/TableGen.hh"
 #include "global/Stepper.hh"
 #include "io/G4EmlowReaders.hh"
 #include "track/TrackActions.hh"
+#include "track/SortKernels.hh"
 #include "track/TrackKernels.hh"
 #include "user/UserActions.hh"
 
@@ -95,6 +97,27 @@ void copy_text(std::string const& s, char* buf, std::size_t len, std::size_t* ne
                    << "output buffer too small (" << s.size() + 1 << " bytes needed)");
     if (buf)
         std::memcpy(buf, s.c_str(), s.size() + 1);
+}
+
+CoreParamsRef const& params_of(void const* ref)
+{
+    return *static_cast<CoreParamsRef const*>(ref);
+}
+
+CoreStateRef const& state_of(void const* ref)
+{
+    return *static_cast<CoreStateRef const*>(ref);
+}
+
+ThreadSlots slots_of(CelerThreadSlots const* ts)
+{
+    ThreadSlots r;
+    if (ts)
+    {
+        r.range = ts->range;
+        r.slots = ts->slots;
+    }
+    return r;
 }
 
 template<class T, class F>
@@ -719,6 +742,113 @@ int celer_generate_physics(const char* spec_json, char* buf, size_t len, size_t*
     return guarded([&] {
         copy_text(dump_json(generate_physics(parse_json(spec_json))), buf, len, needed);
     });
+}
+
+const uint32_t* celer_state_track_slots(const CelerState* state)
+{
+    return state->state->ref().track_slots;
+}
+
+const uint32_t* celer_state_action_offsets(const CelerState* state)
+{
+    return state->state->ref().action_offsets;
+}
+
+int celer_hip_pre_step_launch(const void* params_ref,
+                              const void* state_ref,
+                              const CelerThreadSlots* ts,
+                              void* stream)
+{
+    auto const& p = params_of(params_ref);
+    auto launch = p.scalars.precision == PrecisionMode::mixed ? launch_pre_step_mixed
+                                                             : launch_pre_step;
+    return launch(p, state_of(state_ref), slots_of(ts), static_cast<hipStream_t>(stream));
+}
+
+int celer_hip_along_step_launch(const void* params_ref,
+                                const void* state_ref,
+                                uint32_t action_id,
+                                int charged,
+                                const CelerThreadSlots* ts,
+                                void* stream)
+{
+    auto const& p = params_of(params_ref);
+    if (action_id >= p.scalars.num_actions)
+        return hipErrorInvalidValue;
+    auto const action = static_cast<std::uint8_t>(action_id);
+    auto const st = static_cast<hipStream_t>(stream);
+    if (!charged)
+        return launch_along_step_neutral(p, state_of(state_ref), action, slots_of(ts), st);
+    auto launch = p.scalars.precision == PrecisionMode::mixed
+                      ? launch_along_step_charged_mixed
+                      : launch_along_step_charged;
+    return launch(p, state_of(state_ref), action, slots_of(ts), st);
+}
+
+int celer_hip_discrete_select_launch(const void* params_ref,
+                                     const void* state_ref,
+                                     const CelerThreadSlots* ts,
+                                     void* stream)
+{
+    auto const& p = params_of(params_ref);
+    auto launch = p.scalars.precision == PrecisionMode::mixed ? launch_discrete_select_mixed
+                                                             : launch_discrete_select;
+    return launch(p, state_of(state_ref), slots_of(ts), static_cast<hipStream_t>(stream));
+}
+
+int celer_hip_interaction_launch(const void* params_ref,
+                                 const void* state_ref,
+                                 uint32_t action_id,
+                                 uint32_t model_class,
+                                 const CelerThreadSlots* ts,
+                                 void* stream)
+{
+    auto const& p = params_of(params_ref);
+    if (action_id >= p.scalars.num_actions)
+        return hipErrorInvalidValue;
+    auto launch = p.scalars.precision == PrecisionMode::mixed ? launch_interaction_mixed
+                                                             : launch_interaction;
+    return launch(p, state_of(state_ref), static_cast<std::uint8_t>(action_id), model_class,
+                  slots_of(ts), static_cast<hipStream_t>(stream));
+}
+
+int celer_hip_boundary_launch(const void* params_ref,
+                              const void* state_ref,
+                              const CelerThreadSlots* ts,
+                              void* stream)
+{
+    return launch_boundary(params_of(params_ref), state_of(state_ref), slots_of(ts),
+                           static_cast<hipStream_t>(stream));
+}
+
+int celer_hip_tracking_cut_launch(const void* params_ref,
+                                  const void* state_ref,
+                                  const CelerThreadSlots* ts,
+                                  void* stream)
+{
+    return launch_tracking_cut(params_of(params_ref), state_of(state_ref), slots_of(ts),
+                               static_cast<hipStream_t>(stream));
+}
+
+int celer_hip_calo_tally_launch(const void* params_ref,
+                                const void* state_ref,
+                                uint32_t num_detectors,
+                                const CelerThreadSlots* ts,
+                                void* stream)
+{
+    return launch_calo_tally(params_of(params_ref), state_of(state_ref), num_detectors,
+                             slots_of(ts), static_cast<hipStream_t>(stream));
+}
+
+int celer_hip_sort_tracks(const void* state_ref,
+                          uint32_t key_kind,
+                          uint32_t num_keys,
+                          void* stream)
+{
+    if (key_kind > static_cast<uint32_t>(SortKey::post_action))
+        return hipErrorInvalidValue;
+    return launch_sort_tracks(state_of(state_ref), static_cast<SortKey>(key_kind), num_keys,
+                              static_cast<hipStream_t>(stream));
 }
 
 int celer_read_seltzer_berger(const char* dir, int z, char* buf, size_t len, size_t* needed)

@@ -365,6 +365,85 @@ int celer_hip_reseed(const void* params_ref,
                      uint64_t event_id,
                      void* stream);
 
+/*
+ * Per-action kernels (the executors each step action's step(params,
+ * CoreState<device>&) launches: ActionLauncher.device.hh:102-135 with
+ * ConditionalTrackExecutor). A launch visits the threads of a thread map:
+ * with range == NULL every slot (through slots, or in place when slots is
+ * NULL); with range pointing at &action_offsets[key] (device memory, as
+ * returned by celer_state_action_offsets + key) only that sorted key's
+ * threads, a count the host never reads. Kernels skip slots whose status
+ * or action does not match, so any map is correct; a sorted one packs the
+ * work. The mixed-precision build of a kernel is used when the params ask
+ * for it. Returns a hipError_t value.
+ */
+typedef struct CelerThreadSlots
+{
+    const uint32_t* range; /* {begin, end} of a sorted key, or NULL */
+    const uint32_t* slots; /* thread -> slot (track_slots), or NULL */
+} CelerThreadSlots;
+
+/* Device pointers of a state's thread -> slot map and per-key offsets
+ * (max 64 keys + 1), written by celer_hip_sort_tracks */
+const uint32_t* celer_state_track_slots(const CelerState* state);
+const uint32_t* celer_state_action_offsets(const CelerState* state);
+
+/* phys/detail/PreStepExecutor.hh:46-115 (PreStepAction) */
+int celer_hip_pre_step_launch(const void* params_ref,
+                              const void* state_ref,
+                              const CelerThreadSlots* ts,
+                              void* stream);
+/* global/alongstep/AlongStepNeutralAction.cc / AlongStepGeneralLinearAction.cc
+ * / AlongStepUniformMscAction.cc: tracks whose along-step action is
+ * action_id; charged != 0 runs the charged (MSC, energy loss, field) body */
+int celer_hip_along_step_launch(const void* params_ref,
+                                const void* state_ref,
+                                uint32_t action_id,
+                                int charged,
+                                const CelerThreadSlots* ts,
+                                void* stream);
+/* phys/detail/DiscreteSelectExecutor.hh:38-63 (DiscreteSelectAction) */
+int celer_hip_discrete_select_launch(const void* params_ref,
+                                     const void* state_ref,
+                                     const CelerThreadSlots* ts,
+                                     void* stream);
+/* em/executor/<Model>Executor.hh through phys/InteractionApplier.hh:106-171:
+ * tracks whose post-step action is action_id, sampled by model class
+ * 0 klein_nishina, 1 livermore_pe, 2 bethe_heitler, 3 moller_bhabha,
+ * 4 seltzer_berger, 5 relativistic_brem, 6 eplusgg, 7 combined_brem */
+int celer_hip_interaction_launch(const void* params_ref,
+                                 const void* state_ref,
+                                 uint32_t action_id,
+                                 uint32_t model_class,
+                                 const CelerThreadSlots* ts,
+                                 void* stream);
+/* geo/detail/BoundaryExecutor.hh:42-84 (BoundaryAction) */
+int celer_hip_boundary_launch(const void* params_ref,
+                              const void* state_ref,
+                              const CelerThreadSlots* ts,
+                              void* stream);
+/* phys/detail/TrackingCutExecutor.hh:49-84 (TrackingCutAction) */
+int celer_hip_tracking_cut_launch(const void* params_ref,
+                                  const void* state_ref,
+                                  const CelerThreadSlots* ts,
+                                  void* stream);
+/* user/detail/SimpleCaloExecutor.hh:48-66: per-detector deposition of
+ * num_detectors bins (volume -> detector map of the params) */
+int celer_hip_calo_tally_launch(const void* params_ref,
+                                const void* state_ref,
+                                uint32_t num_detectors,
+                                const CelerThreadSlots* ts,
+                                void* stream);
+/* track/SortTracksAction.cc:102-131 with detail/TrackSortUtils.cu:145-206:
+ * reindex track_slots stably by a per-slot key (0 status, 1 particle type,
+ * 2 along-step action, 3 post-step action; inactive slots take key
+ * num_keys - 1) and write the per-key offsets to action_offsets[0..num_keys]
+ * on the device (1 <= num_keys <= 64) */
+int celer_hip_sort_tracks(const void* state_ref,
+                          uint32_t key_kind,
+                          uint32_t num_keys,
+                          void* stream);
+
 /*---------------------------------------------------------------------------*
  * SERVING: continuous batching of events that arrive over time. No
  * counterpart in the reference, whose offload front end transports one

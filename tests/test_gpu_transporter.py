@@ -55,3 +55,69 @@ def test_max_steps(params):
     t2.Flush()  # kills the active tracks and finishes
     assert t2.GetBufferSize() == 0
     np.testing.assert_array_equal(t2.state.get("status") == 2, False)
+
+
+def test_single_flush_equals_stepper(amd, params):
+    """One flush of an event's tracks is the stepper's transport of the same
+    primaries after the event's reseed (LocalTransporter.cc:199-300): the
+    same kernels on the same RNG streams give the same deposition."""
+    prim = wl.testem3_primaries(6, seed=21, emin=500.0, emax=2000.0)
+    t = accel.LocalTransporter(params, 4096)
+    t.InitializeEvent(7)
+    t.Push(prim)
+    t.Flush()
+    t.Finalize()
+    via_transporter = t.state.get("calo_edep", 100).copy()
+
+    stepper = amd.Stepper(params, 4096)
+    stepper.reseed(7)
+    ev0 = prim.copy()
+    ev0["event_id"] = 0  # the transporter offloads every track as event 0
+    stepper.run(ev0, max_steps=1_000_000)
+    direct = stepper.state.get("calo_edep", 100)
+    assert via_transporter.sum() > 0
+    np.testing.assert_allclose(via_transporter, direct, rtol=1e-12, atol=0)
+
+
+def test_flushed_edep_vs_oracle(amd, params):
+    """Per-detector deposition of events offloaded with auto-flush (several
+    flushes per event) against independent oracle runs of the same
+    primary distribution: chi^2 of batch means over detectors with signal,
+    p > 1e-3."""
+    from scipy import stats
+
+    from tests import oracle_bind as ob
+
+    phys = amd_pkg.generate_physics(wl.testem3_physics_spec())
+    oprob = ob.Problem(wl.testem3_problem(physics=phys, seed=4242, init_capacity=1 << 18,
+                                          max_events=16))
+    nbatch, nprim, energy = 12, 6, 2e3
+    t = accel.LocalTransporter(params, 1 << 14, auto_flush=2)
+
+    def transported(b):
+        prim = wl.testem3_primaries(nprim, emin=energy, emax=energy * 1.0001, seed=700 + b)
+        before = t.state.get("calo_edep", 100).copy()
+        t.InitializeEvent(b)
+        t.Push(prim)  # three auto-flushes of two tracks each
+        assert t.GetBufferSize() == 0
+        t.Flush()
+        return t.state.get("calo_edep", 100) - before
+
+    def oracle(b):
+        prim = wl.testem3_primaries(nprim, emin=energy, emax=energy * 1.0001, seed=900 + b)
+        st = ob.State(oprob, 1 << 12)
+        ob.run(st, prim)
+        return st.get_f64("calo_edep")
+
+    g = np.array([transported(b) for b in range(nbatch)])
+    o = np.array([oracle(b) for b in range(nbatch)])
+    gm, gs = g.mean(0), g.std(0, ddof=1) / np.sqrt(nbatch)
+    om, os_ = o.mean(0), o.std(0, ddof=1) / np.sqrt(nbatch)
+    sel = (gs > 0) & (os_ > 0) & (gm + om > 1e-3 * (gm + om).max())
+    chi2 = float((((gm - om) ** 2) / (gs**2 + os_**2))[sel].sum())
+    ndf = int(sel.sum())
+    p = stats.chi2.sf(chi2, ndf)
+    print(f"transporter vs oracle: {gm.sum():.1f} / {om.sum():.1f} MeV per event, "
+          f"chi2 {chi2:.1f}/{ndf}, p = {p:.3g}")
+    assert ndf > 20
+    assert p > 1e-3
