@@ -58,7 +58,7 @@ CELER_FUNCTION std::uint32_t sample_element(PhysicsParamsRef const& p,
     std::uint32_t i = 0;
     for (; i + 1 < m.num_elements; ++i)
     {
-        if (phys::calc_grid(p.grids[first + i], p.reals, energy) > u)
+        if (phys::calc_grid(p.grids[first + i], p.grid_reals, energy) > u)
             break;
     }
     return i;

@@ -114,7 +114,7 @@ CELER_FUNCTION void pre_step_track(CoreParamsRef const& params,
         if (part.eloss_process >= 0)
         {
             double const range
-                = phys::calc_range(pp.grids[part.range_grid + mat], pp.reals, energy);
+                = phys::calc_range(pp.grids[part.range_grid + mat], pp.grid_reals, energy);
             ps.dedx_range[slot] = range;
             double const eloss_step = phys::range_to_step(pp.scalars, range);
             if (eloss_step <= step)

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <chrono>
 
+#include "corecel/ScopedProfiling.hh"
 #include "track/TrackActions.hh"
 #include "track/TrackKernels.hh"
 #include "CoreParams.hh"
@@ -38,6 +39,8 @@ Stepper::Stepper(StepperInput input)
     });
     times_.assign(actions_.size(), 0.0);
     counts_.assign(actions_.size(), 0);
+    for (auto const& a : actions_)
+        labels_.emplace_back(a->label());
 
     ActionId pid = reg.find_action(ExtendFromPrimariesAction::label_str());
     CELER_VALIDATE(pid != invalid_id,
@@ -109,6 +112,7 @@ void Stepper::enqueue_step()
         }
         for (std::size_t i = 0; i < actions_.size(); ++i)
         {
+            ScopedProfiling prof(labels_[i].c_str());
             CELER_HIP_CALL(hipEventRecord(events_[2 * i], state_->stream()));
             actions_[i]->step(*params_, *state_);
             CELER_HIP_CALL(hipEventRecord(events_[2 * i + 1], state_->stream()));
@@ -117,8 +121,11 @@ void Stepper::enqueue_step()
     }
     else
     {
-        for (auto const& a : actions_)
-            a->step(*params_, *state_);
+        for (std::size_t i = 0; i < actions_.size(); ++i)
+        {
+            ScopedProfiling prof(labels_[i].c_str());
+            actions_[i]->step(*params_, *state_);
+        }
     }
 }
 
@@ -164,6 +171,7 @@ void Stepper::launch_step(bool new_primaries)
     {
         if (!graph_exec_)
             this->capture_step();
+        ScopedProfiling prof("step-graph");
         CELER_HIP_CALL(hipGraphLaunch(graph_exec_, state_->stream()));
     }
     else

@@ -6,6 +6,7 @@
 #pragma once
 
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "ActionInterface.hh"
@@ -98,6 +99,7 @@ class Stepper
     std::shared_ptr<CoreParams const> params_;
     std::shared_ptr<CoreState> state_;
     std::vector<std::shared_ptr<StepActionInterface const>> actions_;
+    std::vector<std::string> labels_;  //!< profiler range names of actions_
     std::shared_ptr<ExtendFromPrimariesAction const> primaries_action_;
     bool action_times_{false};
     std::vector<double> times_;

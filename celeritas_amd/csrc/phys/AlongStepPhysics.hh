@@ -306,12 +306,12 @@ struct MscTrack
     CELER_FUNCTION double calc_msc_mfp(double e) const
     {
         XsGrid const& g = p.grids[p.msc.xs_grid + 2 * mat + par];
-        double const xsec = phys::calc_grid(g, p.reals, e) / (e * e);
+        double const xsec = phys::calc_grid(g, p.grid_reals, e) / (e * e);
         return 1 / xsec;
     }
     CELER_FUNCTION double calc_inverse_range(double r) const
     {
-        return phys::calc_inverse_range(p.grids[part.range_grid + mat], p.reals, r);
+        return phys::calc_inverse_range(p.grids[part.range_grid + mat], p.grid_reals, r);
     }
     CELER_FUNCTION UrbanMscParMat const& pmdata() const
     {
@@ -322,7 +322,7 @@ struct MscTrack
     {
         if (step <= range * msc_dtrl)
         {
-            double const dedx = phys::calc_grid(p.grids[part.dedx_grid + mat], p.reals, energy);
+            double const dedx = phys::calc_grid(p.grids[part.dedx_grid + mat], p.grid_reals, energy);
             return energy - step * dedx;
         }
         return calc_inverse_range(range - step);

@@ -254,7 +254,8 @@ struct PhysicsParamsRef
     std::uint32_t num_materials{0};
     ParticlePhysics const* particle{nullptr};
     XsGrid const* grids{nullptr};
-    double const* reals{nullptr};
+    double const* grid_reals{nullptr};  //!< XsGrid values, energy_max_xs
+    double const* reals{nullptr};  //!< SB and Livermore tables
     // Particle properties
     double const* mass{nullptr};  //!< [MeV/c^2]
     double const* charge{nullptr};  //!< [e]
@@ -280,6 +281,10 @@ struct PhysicsParamsRef
     UrbanFluctuation const* fluct{nullptr};  //!< per material
     std::uint32_t fluct_enabled{0};
     ModelData model{};
+    //! Contiguous region holding particle, grids, grid_reals, mass, charge,
+    //! materials, msc_material, msc_par_mat and fluct (16-byte aligned)
+    char const* hot_base{nullptr};
+    std::uint32_t hot_bytes{0};
 };
 
 //---------------------------------------------------------------------------//

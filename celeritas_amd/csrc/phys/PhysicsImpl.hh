@@ -126,7 +126,7 @@ CELER_FUNCTION double calc_xs(PhysicsParamsRef const& p,
 {
     double const energy = gp.energy;
     if (proc.xs_grid != invalid_id)
-        return calc_grid(p.grids[proc.xs_grid + mat], p.reals, gp);
+        return calc_grid(p.grids[proc.xs_grid + mat], p.grid_reals, gp);
     if (proc.process_class == static_cast<std::uint32_t>(ProcessClass::annihilation))
     {
         return eplusgg_macro_xs(p.model.electron_mass,
@@ -153,7 +153,7 @@ CELER_FUNCTION double calc_max_xs(PhysicsParamsRef const& p,
                                   GridPoint& gp)
 {
     double const energy = gp.energy;
-    double const emax = p.reals[proc.energy_max_xs + mat];
+    double const emax = p.grid_reals[proc.energy_max_xs + mat];
     double const exi = energy * p.scalars.min_eprime_over_e;
     if (emax >= exi && emax < energy)
         return calc_xs(p, proc, mat, emax);
@@ -235,13 +235,13 @@ CELER_FUNCTION double calc_mean_eloss(PhysicsParamsRef const& p,
                                       double range,
                                       double step)
 {
-    double eloss = step * calc_grid(p.grids[part.dedx_grid + mat], p.reals, energy);
+    double eloss = step * calc_grid(p.grids[part.dedx_grid + mat], p.grid_reals, energy);
     if (eloss >= energy * p.scalars.linear_loss_limit)
     {
         if (step == range)
             return energy;
         eloss = energy
-                - calc_inverse_range(p.grids[part.range_grid + mat], p.reals, range - step);
+                - calc_inverse_range(p.grids[part.range_grid + mat], p.grid_reals, range - step);
     }
     return eloss;
 }

@@ -6,6 +6,8 @@
 #include "PhysicsParams.hh"
 
 #include <algorithm>
+#include <cstring>
+#include <type_traits>
 #include <cmath>
 #include <map>
 
@@ -229,7 +231,8 @@ struct PhysicsParams::HostData
 {
     std::vector<ParticlePhysics> particle;
     std::vector<XsGrid> grids;
-    std::vector<double> reals;
+    std::vector<double> grid_reals;  //!< XsGrid values and energy_max_xs
+    std::vector<double> reals;  //!< SB and Livermore tables
     std::vector<double> mass;
     std::vector<double> charge;
     std::vector<double> cutoff_energy;
@@ -237,13 +240,9 @@ struct PhysicsParams::HostData
     std::vector<ElementComponent> components;
     std::vector<ElementRecord> elements;
 
-    DeviceVector<ParticlePhysics> d_particle;
-    DeviceVector<XsGrid> d_grids;
+    DeviceVector<char> d_hot;  //!< the tables the step kernels stage in LDS
     DeviceVector<double> d_reals;
-    DeviceVector<double> d_mass;
-    DeviceVector<double> d_charge;
     DeviceVector<double> d_cutoff;
-    DeviceVector<MaterialRecord> d_materials;
     DeviceVector<ElementComponent> d_components;
     DeviceVector<ElementRecord> d_elements;
     std::vector<std::uint32_t> sb_argmax;
@@ -255,9 +254,6 @@ struct PhysicsParams::HostData
     std::vector<UrbanFluctuation> fluct;
     std::vector<UrbanMscMaterial> msc_material;
     std::vector<UrbanMscParMat> msc_par_mat;
-    DeviceVector<UrbanFluctuation> d_fluct;
-    DeviceVector<UrbanMscMaterial> d_msc_material;
-    DeviceVector<UrbanMscParMat> d_msc_par_mat;
     DeviceVector<LivermoreElement> d_pe_elements;
     DeviceVector<LivermoreShell> d_pe_shells;
 
@@ -331,8 +327,8 @@ struct PhysicsParams::HostData
             x.loge_delta = g.delta;
             x.size = static_cast<std::uint32_t>(g.values.size());
             x.prime_index = g.prime;
-            x.offset = static_cast<std::uint32_t>(reals.size());
-            reals.insert(reals.end(), g.values.begin(), g.values.end());
+            x.offset = static_cast<std::uint32_t>(grid_reals.size());
+            grid_reals.insert(grid_reals.end(), g.values.begin(), g.values.end());
             grids.push_back(x);
         }
         return first;
@@ -592,7 +588,7 @@ PhysicsParams::PhysicsParams(JsonValue const& inp, ActionRegistry* reg)
         rec.energy_max_xs = invalid_id;
         if (rec.integral)
         {
-            rec.energy_max_xs = static_cast<std::uint32_t>(h.reals.size());
+            rec.energy_max_xs = static_cast<std::uint32_t>(h.grid_reals.size());
             for (std::size_t m = 0; m < num_materials_; ++m)
             {
                 double e_max = 0, xs_max = 0;
@@ -609,7 +605,7 @@ PhysicsParams::PhysicsParams(JsonValue const& inp, ActionRegistry* reg)
                         }
                     }
                 }
-                h.reals.push_back(e_max);
+                h.grid_reals.push_back(e_max);
             }
         }
         auto dedx = grids_of(*pj, "dedx_grid", "dedx", nullptr, num_materials_);
@@ -804,14 +800,50 @@ PhysicsParams::PhysicsParams(JsonValue const& inp, ActionRegistry* reg)
         }
     }
 
-    // Upload
-    h.d_particle = to_device(h.particle);
-    h.d_grids = to_device(h.grids);
+    // Upload. The small tables the step kernels read per track (particle
+    // and process records, grid headers and values, material, MSC and
+    // fluctuation records, masses and charges) share one contiguous region
+    // that a kernel can copy into LDS and rebase into.
+    std::vector<char> hot;
+    auto place = [&hot](auto const& v) {
+        std::size_t const off = (hot.size() + 15) / 16 * 16;
+        std::size_t const n = v.size() * sizeof(v[0]);
+        hot.resize(off + n);
+        if (n)
+            std::memcpy(hot.data() + off, v.data(), n);
+        return off;
+    };
+    std::size_t const o_particle = place(h.particle);
+    std::size_t const o_grids = place(h.grids);
+    std::size_t const o_grid_reals = place(h.grid_reals);
+    std::size_t const o_mass = place(h.mass);
+    std::size_t const o_charge = place(h.charge);
+    std::size_t const o_materials = place(h.materials);
+    std::size_t const o_msc_material = place(h.msc_material);
+    std::size_t const o_msc_par_mat = place(h.msc_par_mat);
+    std::size_t const o_fluct = place(h.fluct);
+    hot.resize((hot.size() + 15) / 16 * 16);
+    h.d_hot = to_device(hot);
+    char const* base = h.d_hot.data();
+    auto at = [base](std::size_t off, auto const& v) {
+        using T = std::remove_const_t<std::remove_reference_t<decltype(v[0])>>;
+        return v.empty() ? static_cast<T const*>(nullptr)
+                         : reinterpret_cast<T const*>(base + off);
+    };
+    ref_.hot_base = base;
+    ref_.hot_bytes = static_cast<std::uint32_t>(hot.size());
+    ref_.particle = at(o_particle, h.particle);
+    ref_.grids = at(o_grids, h.grids);
+    ref_.grid_reals = at(o_grid_reals, h.grid_reals);
+    ref_.mass = at(o_mass, h.mass);
+    ref_.charge = at(o_charge, h.charge);
+    ref_.materials = at(o_materials, h.materials);
+    ref_.msc_material = at(o_msc_material, h.msc_material);
+    ref_.msc_par_mat = at(o_msc_par_mat, h.msc_par_mat);
+    ref_.fluct = at(o_fluct, h.fluct);
+
     h.d_reals = to_device(h.reals);
-    h.d_mass = to_device(h.mass);
-    h.d_charge = to_device(h.charge);
     h.d_cutoff = to_device(h.cutoff_energy);
-    h.d_materials = to_device(h.materials);
     h.d_components = to_device(h.components);
     h.d_elements = to_device(h.elements);
     h.d_sb_argmax = to_device(h.sb_argmax);
@@ -820,23 +852,12 @@ PhysicsParams::PhysicsParams(JsonValue const& inp, ActionRegistry* reg)
     ref_.elem_cdf_index = h.d_elem_cdf_index.data();
     h.d_pe_elements = to_device(h.pe_elements);
     h.d_pe_shells = to_device(h.pe_shells);
-    h.d_fluct = to_device(h.fluct);
-    h.d_msc_material = to_device(h.msc_material);
-    h.d_msc_par_mat = to_device(h.msc_par_mat);
-    ref_.fluct = h.d_fluct.data();
-    ref_.msc_material = h.d_msc_material.data();
-    ref_.msc_par_mat = h.d_msc_par_mat.data();
     ref_.pe_elements = h.d_pe_elements.data();
     ref_.pe_shells = h.d_pe_shells.data();
     ref_.num_particles = this->num_particles();
     ref_.num_materials = num_materials_;
-    ref_.particle = h.d_particle.data();
-    ref_.grids = h.d_grids.data();
     ref_.reals = h.d_reals.data();
-    ref_.mass = h.d_mass.data();
-    ref_.charge = h.d_charge.data();
     ref_.cutoff_energy = h.d_cutoff.data();
-    ref_.materials = h.d_materials.data();
     ref_.components = h.d_components.data();
     ref_.elements = h.d_elements.data();
 }

@@ -118,7 +118,7 @@ int main(int argc, char** argv)
         // Along-step physics of charged particles
         if (part.eloss_process < 0) continue;
         double const energy = logu(1e-3, 1e5);
-        double const range = phys::calc_range(p.grids[part.range_grid + mat], p.reals, energy);
+        double const range = phys::calc_range(p.grids[part.range_grid + mat], p.grid_reals, energy);
         bool const lepton = pid == p.electron_id || pid == p.positron_id;
         double step = range * logu(1e-6, 1.0);
         double safety = U(gen) < 0.2 ? 0.0 : logu(1e-7, 5.0);

@@ -133,8 +133,9 @@ def _lockstep(amd, prob, prim, slots, steps, sorted_launch, num_detectors=0):
         ca, cb = a.counters(), b.counters()
         assert ca == cb, f"step {k}: {ca} != {cb}"
     if num_detectors:
-        np.testing.assert_array_equal(a.get("calo_edep", num_detectors),
-                                      b.get("calo_edep", num_detectors))
+        # Deposits are summed in LDS with atomics, whose order varies
+        np.testing.assert_allclose(a.get("calo_edep", num_detectors),
+                                   b.get("calo_edep", num_detectors), rtol=1e-13, atol=0)
     return a
 
 

@@ -117,6 +117,12 @@ def test_testem3_user_actions_match_oracle(amd, testem3):
               flush=True)
         assert h["overflow_steps"] == 0
         for c in INT_COLUMNS:
+            if c == "track_id":
+                # Secondaries take IDs from a per-event atomic counter on
+                # the device, so within a step only the set of IDs is fixed
+                np.testing.assert_array_equal(np.sort(h[c]), np.sort(oh[c]),
+                                              err_msg=f"step {step} column {c}")
+                continue
             np.testing.assert_array_equal(h[c], oh[c], err_msg=f"step {step} column {c}")
         for c in REAL_COLUMNS:
             np.testing.assert_allclose(h[c], oh[c], rtol=1e-9, atol=1e-9,
@@ -151,8 +157,10 @@ def test_hit_buffer_overflow_and_graph_replay(amd, testem3):
     eager = run(1 << 16, False)
     replay = run(1 << 16, True)
     assert eager["overflow_steps"] == 0 and len(eager["detector"]) > 0
-    for c in ("detector", "track_id", "energy_deposition", "post.energy"):
+    for c in ("detector", "energy_deposition", "post.energy"):
         np.testing.assert_array_equal(eager[c], replay[c])
+    # Secondary track IDs come from an atomic counter: the same set
+    np.testing.assert_array_equal(np.sort(eager["track_id"]), np.sort(replay["track_id"]))
     # A buffer smaller than the hits drops whole steps and counts them
     small = run(64, False)
     n = len(small["detector"])
