@@ -851,6 +851,35 @@ int celer_hip_sort_tracks(const void* state_ref,
                               static_cast<hipStream_t>(stream));
 }
 
+int celer_hip_partition_keys(const uint8_t* keys,
+                             uint32_t n,
+                             uint32_t num_keys,
+                             uint32_t* sorted,
+                             uint32_t* offsets,
+                             void* stream)
+{
+    if (num_keys < 1 || num_keys > max_sort_keys || (n > 0 && (!keys || !sorted)) || !offsets)
+        return hipErrorInvalidValue;
+    auto const st = static_cast<hipStream_t>(stream);
+    std::uint32_t* counts = nullptr;
+    std::size_t const words = sort_counts_size(n);
+    if (words > 0)
+    {
+        hipError_t err = hipMallocAsync(reinterpret_cast<void**>(&counts),
+                                        words * sizeof(std::uint32_t), st);
+        if (err != hipSuccess)
+            return err;
+    }
+    hipError_t err = launch_partition_keys(keys, n, num_keys, counts, sorted, offsets, st);
+    if (counts)
+    {
+        hipError_t const ferr = hipFreeAsync(counts, st);
+        if (err == hipSuccess)
+            err = ferr;
+    }
+    return err;
+}
+
 int celer_read_seltzer_berger(const char* dir, int z, char* buf, size_t len, size_t* needed)
 {
     return guarded([&] {

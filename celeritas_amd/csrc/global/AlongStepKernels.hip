@@ -11,6 +11,7 @@
 #include "CoreActions.hh"
 #include "corecel/WaveOps.hh"
 #include "AlongStepExecutor.hh"
+#include "StagedTables.hh"
 
 namespace celeritas_amd
 {
@@ -20,11 +21,13 @@ constexpr int block_size = 256;
 
 template<bool UseField>
 __global__ void __launch_bounds__(block_size)
-    along_step_charged_kernel(CoreParamsRef const params,
+    along_step_charged_kernel(CoreParamsRef const params_in,
                               CoreStateRef const state,
                               std::uint8_t const action,
                               ThreadSlots const ts)
 {
+    extern __shared__ uint4 staged_tables[];
+    CoreParamsRef const params = stage_tables(params_in, staged_tables);
     size_type slot;
     if (!thread_to_slot(ts, blockIdx.x * blockDim.x + threadIdx.x, state.size, &slot))
         return;
@@ -46,9 +49,11 @@ hipError_t CELER_PREC_FN(launch_along_step_charged)(CoreParamsRef const& p,
                                      hipStream_t st)
 {
     if (p.field.enabled)
-        along_step_charged_kernel<true><<<grid_of(s), s.block_size, 0, st>>>(p, s, action, ts);
+        along_step_charged_kernel<true>
+            <<<grid_of(s), s.block_size, staged_table_bytes(p), st>>>(p, s, action, ts);
     else
-        along_step_charged_kernel<false><<<grid_of(s), s.block_size, 0, st>>>(p, s, action, ts);
+        along_step_charged_kernel<false>
+            <<<grid_of(s), s.block_size, staged_table_bytes(p), st>>>(p, s, action, ts);
     return hipGetLastError();
 }
 

@@ -45,6 +45,16 @@ hipError_t launch_interaction(CoreParamsRef const&,
                               std::uint32_t model,
                               ThreadSlots const&,
                               hipStream_t);
+//! Every discrete model in one launch (tracks whose post-step action is
+//! one of the model actions)
+hipError_t launch_interactions(CoreParamsRef const&,
+                               CoreStateRef const&,
+                               ThreadSlots const&,
+                               hipStream_t);
+hipError_t launch_interactions_mixed(CoreParamsRef const&,
+                                     CoreStateRef const&,
+                                     ThreadSlots const&,
+                                     hipStream_t);
 hipError_t launch_boundary(CoreParamsRef const&,
                            CoreStateRef const&,
                            ThreadSlots const&,

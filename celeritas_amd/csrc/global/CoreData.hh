@@ -207,6 +207,9 @@ struct ThreadSlots
 {
     std::uint32_t const* range{nullptr};  //!< &action_offsets[action], or null
     std::uint32_t const* slots{nullptr};  //!< thread -> slot, or null (identity)
+    //! End of a range spanning several keys (&action_offsets[last + 1]);
+    //! null: the range is the one key's, [range[0], range[1])
+    std::uint32_t const* range_end{nullptr};
 };
 
 //! Number of workgroups covering the state's slots
@@ -221,7 +224,8 @@ thread_to_slot(ThreadSlots const& ts, size_type tid, size_type size, size_type* 
     if (ts.range)
     {
         size_type const begin = ts.range[0];
-        if (tid >= ts.range[1] - begin)
+        size_type const end = ts.range_end ? *ts.range_end : ts.range[1];
+        if (tid >= end - begin)
             return false;
         *slot = ts.slots[begin + tid];
         return true;

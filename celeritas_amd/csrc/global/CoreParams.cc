@@ -106,6 +106,7 @@ CoreParams::CoreParams(Input inp)
     sc.num_actions = reg_->num_actions();
     sc.track_order = inp.track_order;
     sc.precision = inp.precision;
+    fuse_interactions_ = inp.fuse_interactions;
 
     ref_.geo = geo_->device_ref();
     ref_.volume_to_material = d_vol_to_mat_.data();
@@ -233,6 +234,7 @@ std::shared_ptr<CoreParams> CoreParams::from_json(std::string const& text)
     CELER_VALIDATE(prec == "fp64" || prec == "mixed",
                    << "precision must be 'fp64' or 'mixed'");
     inp.precision = prec == "fp64" ? PrecisionMode::fp64 : PrecisionMode::mixed;
+    inp.fuse_interactions = doc.get_bool("fuse_interactions", true);
     return std::make_shared<CoreParams>(std::move(inp));
 }
 

@@ -43,6 +43,9 @@ class CoreParams
         size_type max_events{4096};
         TrackOrder track_order{TrackOrder::none};
         PrecisionMode precision{PrecisionMode::fp64};
+        //! Sample every discrete model in one launch (the first model
+        //! action launches it; the others launch nothing)
+        bool fuse_interactions{true};
         //! Along-step action for charged particles (label), created by caller
         std::string along_step_label;
         //! Uniform magnetic field (enabled for the uniform-field along-step)
@@ -63,6 +66,7 @@ class CoreParams
     size_type max_events() const { return max_events_; }
     TrackOrder track_order() const { return ref_.scalars.track_order; }
     PrecisionMode precision() const { return ref_.scalars.precision; }
+    bool fuse_interactions() const { return fuse_interactions_; }
     std::vector<std::uint32_t> const& volume_to_material() const
     {
         return vol_to_mat_;
@@ -77,6 +81,7 @@ class CoreParams
     size_type init_capacity_;
     size_type max_events_;
     size_type num_detectors_{0};
+    bool fuse_interactions_{true};
     std::vector<std::uint32_t> vol_to_mat_;
     DeviceVector<std::uint32_t> d_vol_to_mat_;
     DeviceVector<std::uint32_t> d_vol_to_det_;

@@ -15,6 +15,7 @@
 #include "CoreActions.hh"
 
 #include "corecel/WaveOps.hh"
+#include "StagedTables.hh"
 #include "StepExecutors.hh"
 
 namespace celeritas_amd
@@ -37,10 +38,12 @@ using exec::status_of;
 // PRE-STEP
 //---------------------------------------------------------------------------//
 __global__ void __launch_bounds__(block_size)
-    pre_step_kernel(CoreParamsRef const params,
+    pre_step_kernel(CoreParamsRef const params_in,
                     CoreStateRef const state,
                     ThreadSlots const ts)
 {
+    extern __shared__ uint4 staged_tables[];
+    CoreParamsRef const params = stage_tables(params_in, staged_tables);
     size_type const tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid == 0)
         state.counters->secondary_stack_size = 0;
@@ -82,10 +85,12 @@ __global__ void __launch_bounds__(block_size)
 // DISCRETE SELECT
 //---------------------------------------------------------------------------//
 __global__ void __launch_bounds__(block_size)
-    discrete_select_kernel(CoreParamsRef const params,
+    discrete_select_kernel(CoreParamsRef const params_in,
                            CoreStateRef const state,
                            ThreadSlots const ts)
 {
+    extern __shared__ uint4 staged_tables[];
+    CoreParamsRef const params = stage_tables(params_in, staged_tables);
     size_type slot;
     if (!thread_to_slot(ts, blockIdx.x * blockDim.x + threadIdx.x, state.size, &slot))
         return;
@@ -118,6 +123,52 @@ __global__ void __launch_bounds__(block_size)
     if (state.diag)
         wave::count_active_lanes(state.diag + diag_interaction);
     exec::interaction_track<M>(params, state, slot);
+}
+
+/*!
+ * All discrete models in one launch: each thread samples the model its
+ * track selected. Over the union of the models' sorted ranges consecutive
+ * threads share a model, so waves stay uniform, and the short latency-
+ * bound runs of the separate per-model launches overlap instead of
+ * following one another.
+ */
+__global__ void __launch_bounds__(block_size)
+    interactions_kernel(CoreParamsRef const params_in,
+                        CoreStateRef const state,
+                        ThreadSlots const ts)
+{
+    extern __shared__ uint4 staged_tables[];
+    CoreParamsRef const params = stage_tables(params_in, staged_tables);
+    size_type slot;
+    if (!thread_to_slot(ts, blockIdx.x * blockDim.x + threadIdx.x, state.size, &slot))
+        return;
+    if (status_of(state, slot) == TrackStatus::inactive)
+        return;
+    std::uint32_t const model
+        = std::uint32_t(state.sim.post_step_action[slot])
+          - params.phys.scalars.first_model_action;
+    if (model >= params.phys.scalars.num_models)
+        return;
+    if (state.diag)
+        wave::count_active_lanes(state.diag + diag_interaction);
+    switch (static_cast<ModelClass>(params.phys.model_class[model]))
+    {
+#define CELER_MODEL_CASE(M)                                  \
+    case ModelClass::M:                                      \
+        exec::interaction_track<ModelClass::M>(params, state, slot); \
+        break;
+        CELER_MODEL_CASE(klein_nishina)
+        CELER_MODEL_CASE(livermore_pe)
+        CELER_MODEL_CASE(bethe_heitler)
+        CELER_MODEL_CASE(moller_bhabha)
+        CELER_MODEL_CASE(seltzer_berger)
+        CELER_MODEL_CASE(relativistic_brem)
+        CELER_MODEL_CASE(eplusgg)
+        CELER_MODEL_CASE(combined_brem)
+#undef CELER_MODEL_CASE
+        default:
+            break;
+    }
 }
 
 //---------------------------------------------------------------------------//
@@ -269,7 +320,7 @@ hipError_t CELER_PREC_FN(launch_pre_step)(CoreParamsRef const& p,
                            ThreadSlots const& ts,
                            hipStream_t st)
 {
-    pre_step_kernel<<<grid_of(s), s.block_size, 0, st>>>(p, s, ts);
+    pre_step_kernel<<<grid_of(s), s.block_size, staged_table_bytes(p), st>>>(p, s, ts);
     return hipGetLastError();
 }
 
@@ -291,7 +342,7 @@ hipError_t CELER_PREC_FN(launch_discrete_select)(CoreParamsRef const& p,
                                   ThreadSlots const& ts,
                                   hipStream_t st)
 {
-    discrete_select_kernel<<<grid_of(s), s.block_size, 0, st>>>(p, s, ts);
+    discrete_select_kernel<<<grid_of(s), s.block_size, staged_table_bytes(p), st>>>(p, s, ts);
     return hipGetLastError();
 }
 
@@ -320,6 +371,15 @@ hipError_t CELER_PREC_FN(launch_interaction)(CoreParamsRef const& p,
         default:
             return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t CELER_PREC_FN(launch_interactions)(CoreParamsRef const& p,
+                               CoreStateRef const& s,
+                               ThreadSlots const& ts,
+                               hipStream_t st)
+{
+    interactions_kernel<<<grid_of(s), s.block_size, staged_table_bytes(p), st>>>(p, s, ts);
     return hipGetLastError();
 }
 

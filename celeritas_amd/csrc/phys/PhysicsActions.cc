@@ -34,6 +34,21 @@ void DiscreteSelectAction::step(CoreParams const& params, CoreState& state) cons
 void ModelAction::step(CoreParams const& params, CoreState& state) const
 {
     bool const mixed = params.precision() == PrecisionMode::mixed;
+    if (params.fuse_interactions())
+    {
+        // The first model action samples every model's tracks in one
+        // launch over the union of their (consecutive) sorted ranges
+        auto const& sc = params.device_ref().phys.scalars;
+        if (this->action_id() != sc.first_model_action)
+            return;
+        ThreadSlots ts = make_thread_slots(params, state, *this);
+        if (ts.range)
+            ts.range_end = state.ref().action_offsets + sc.first_model_action
+                           + sc.num_models;
+        CELER_HIP_CALL((mixed ? launch_interactions_mixed : launch_interactions)(
+            params.device_ref(), state.ref(), ts, state.stream()));
+        return;
+    }
     CELER_HIP_CALL((mixed ? launch_interaction_mixed : launch_interaction)(
         params.device_ref(), state.ref(), static_cast<std::uint8_t>(this->action_id()),
         model_class_, make_thread_slots(params, state, *this), state.stream()));

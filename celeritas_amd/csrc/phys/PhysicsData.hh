@@ -232,6 +232,7 @@ struct PhysicsScalars
     std::uint8_t failure_action;
     std::uint8_t msc_action;
     std::uint8_t fixed_step_action;
+    std::uint8_t first_model_action;  //!< model actions are consecutive
     //! LPM suppression in the relativistic brems and Bethe-Heitler models
     std::uint8_t enable_lpm;
 };
@@ -270,6 +271,8 @@ struct PhysicsParamsRef
     ElementComponent const* components{nullptr};
     //! Per model and material: first element-CDF grid (ModelRange::elem_cdf)
     std::uint32_t const* elem_cdf_index{nullptr};
+    //! Model class of each model action (action - first_model_action)
+    std::uint32_t const* model_class{nullptr};
     ElementRecord const* elements{nullptr};
     std::uint32_t const* sb_argmax{nullptr};  //!< per-x argmax of SB tables
     LivermoreElement const* pe_elements{nullptr};  //!< per element
@@ -282,7 +285,8 @@ struct PhysicsParamsRef
     std::uint32_t fluct_enabled{0};
     ModelData model{};
     //! Contiguous region holding particle, grids, grid_reals, mass, charge,
-    //! materials, msc_material, msc_par_mat and fluct (16-byte aligned)
+    //! materials, msc_material, msc_par_mat, fluct and model_class
+    //! (16-byte aligned)
     char const* hot_base{nullptr};
     std::uint32_t hot_bytes{0};
 };

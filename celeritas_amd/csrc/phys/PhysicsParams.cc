@@ -254,6 +254,7 @@ struct PhysicsParams::HostData
     std::vector<UrbanFluctuation> fluct;
     std::vector<UrbanMscMaterial> msc_material;
     std::vector<UrbanMscParMat> msc_par_mat;
+    std::vector<std::uint32_t> model_class;  //!< per model action
     DeviceVector<LivermoreElement> d_pe_elements;
     DeviceVector<LivermoreShell> d_pe_shells;
 
@@ -689,6 +690,9 @@ PhysicsParams::PhysicsParams(JsonValue const& inp, ActionRegistry* reg)
                 continue;
             }
             rec.models[k].action = reg->next_id();
+            if (num_models == 0)
+                sc.first_model_action = static_cast<std::uint8_t>(reg->next_id());
+            h.model_class.push_back(static_cast<std::uint32_t>(mc));
             model_actions[mlabel] = {reg->next_id(), mc};
             reg->insert(std::make_shared<ModelAction>(
                 reg->next_id(), mlabel, static_cast<std::uint32_t>(mc)));
@@ -822,6 +826,7 @@ PhysicsParams::PhysicsParams(JsonValue const& inp, ActionRegistry* reg)
     std::size_t const o_msc_material = place(h.msc_material);
     std::size_t const o_msc_par_mat = place(h.msc_par_mat);
     std::size_t const o_fluct = place(h.fluct);
+    std::size_t const o_model_class = place(h.model_class);
     hot.resize((hot.size() + 15) / 16 * 16);
     h.d_hot = to_device(hot);
     char const* base = h.d_hot.data();
@@ -841,6 +846,7 @@ PhysicsParams::PhysicsParams(JsonValue const& inp, ActionRegistry* reg)
     ref_.msc_material = at(o_msc_material, h.msc_material);
     ref_.msc_par_mat = at(o_msc_par_mat, h.msc_par_mat);
     ref_.fluct = at(o_fluct, h.fluct);
+    ref_.model_class = at(o_model_class, h.model_class);
 
     h.d_reals = to_device(h.reals);
     h.d_cutoff = to_device(h.cutoff_energy);

@@ -23,7 +23,21 @@ hipError_t launch_sort_tracks(CoreStateRef const& state,
                               unsigned int num_keys,
                               hipStream_t stream);
 
-//! Words of sort_counts scratch needed for a state size
+/*!
+ * Stable multi-split of the indices 0..n-1 by a byte key per index (keys
+ * at or above num_keys - 1 take the last key): sorted lists, for each key,
+ * its indices in increasing order, and offsets[0..num_keys] its thread
+ * ranges. counts is scratch of sort_counts_size(n) words.
+ */
+hipError_t launch_partition_keys(std::uint8_t const* keys,
+                                 size_type n,
+                                 unsigned int num_keys,
+                                 std::uint32_t* counts,
+                                 std::uint32_t* sorted,
+                                 std::uint32_t* offsets,
+                                 hipStream_t stream);
+
+//! Words of histogram scratch needed to split n indices
 size_type sort_counts_size(size_type num_slots);
 
 //---------------------------------------------------------------------------//

@@ -37,36 +37,65 @@ inline unsigned int num_tiles(size_type n)
     return (n + tile_size - 1) / tile_size;
 }
 
-//! Sort key of a slot; inactive slots take the last key
-__device__ __forceinline__ unsigned int
-sort_key(CoreStateRef const& s, SortKey kind, size_type slot, unsigned int last)
+//! Sort key of a slot from the track state; inactive slots take the last key
+struct StateKey
 {
-    auto const status = static_cast<TrackStatus>(s.sim.status[slot]);
-    if (status == TrackStatus::inactive)
-        return last;
-    unsigned int k;
-    switch (kind)
+    CoreStateRef state;
+    SortKey kind;
+
+    __device__ __forceinline__ unsigned int operator()(size_type slot, unsigned int last) const
     {
-        case SortKey::status:
-            return 0;
-        case SortKey::particle:
-            k = s.particle.particle_id[slot];
-            break;
-        case SortKey::along_action:
-            k = s.sim.along_step_action[slot];
-            break;
-        default:
-            k = s.sim.post_step_action[slot];
-            break;
+        auto const status = static_cast<TrackStatus>(state.sim.status[slot]);
+        if (status == TrackStatus::inactive)
+            return last;
+        unsigned int k;
+        switch (kind)
+        {
+            case SortKey::status:
+                return 0;
+            case SortKey::particle:
+                k = state.particle.particle_id[slot];
+                break;
+            case SortKey::along_action:
+                k = state.sim.along_step_action[slot];
+                break;
+            default:
+                k = state.sim.post_step_action[slot];
+                break;
+        }
+        return k < last ? k : last;
     }
-    return k < last ? k : last;
-}
+};
+
+//! Key of an index from a caller's byte array, clamped to the last key
+struct ArrayKey
+{
+    std::uint8_t const* keys;
+
+    __device__ __forceinline__ unsigned int operator()(size_type i, unsigned int last) const
+    {
+        unsigned int const k = keys[i];
+        return k < last ? k : last;
+    }
+};
+
+//! Where a multi-split reads and writes: n indices, the key-major per-tile
+//! histogram table (scratch, num_keys * ntiles words), the sorted indices
+//! and the per-key offsets (num_keys + 1 words)
+struct SplitRefs
+{
+    size_type n;
+    std::uint32_t* counts;
+    std::uint32_t* sorted;
+    std::uint32_t* offsets;
+};
 
 //---------------------------------------------------------------------------//
 //! Per-tile key histogram, stored key-major: counts[key * ntiles + tile]
+template<class KeyFn>
 __global__ void __launch_bounds__(block_size)
-    sort_count_kernel(CoreStateRef const state,
-                      SortKey const kind,
+    sort_count_kernel(KeyFn const key_of,
+                      SplitRefs const refs,
                       unsigned int const num_keys,
                       unsigned int const ntiles)
 {
@@ -77,28 +106,35 @@ __global__ void __launch_bounds__(block_size)
     size_type const base = blockIdx.x * tile_size;
     for (int i = 0; i < items_per_thread; ++i)
     {
-        size_type const slot = base + i * block_size + threadIdx.x;
-        if (slot < state.size)
-            atomicAdd(&hist[sort_key(state, kind, slot, num_keys - 1)], 1u);
+        size_type const idx = base + i * block_size + threadIdx.x;
+        if (idx < refs.n)
+            atomicAdd(&hist[key_of(idx, num_keys - 1)], 1u);
     }
     __syncthreads();
     for (unsigned int k = threadIdx.x; k < num_keys; k += block_size)
-        state.sort_counts[k * ntiles + blockIdx.x] = hist[k];
+        refs.counts[k * ntiles + blockIdx.x] = hist[k];
 }
 
 //! Exclusive scan of the histogram table (one workgroup); the first tile
-//! of each key gives the key's thread offset, and offsets[num_keys] = size.
+//! of each key gives the key's thread offset, and offsets[num_keys] = n.
 //! Each thread owns a contiguous chunk: it sums the chunk, one workgroup
 //! scan orders the chunk sums, and the thread rewrites its chunk from its
 //! base (one workgroup-wide scan instead of one per 1024 entries).
 __global__ void __launch_bounds__(1024)
-    sort_scan_kernel(CoreStateRef const state,
+    sort_scan_kernel(SplitRefs const refs,
                      unsigned int const num_keys,
                      unsigned int const ntiles)
 {
     constexpr int nw = 1024 / wave::size;
     __shared__ unsigned int lds[2 * nw];
-    unsigned int* counts = state.sort_counts;
+    if (ntiles == 0)
+    {
+        // Nothing to split: every key's range is empty
+        for (unsigned int k = threadIdx.x; k <= num_keys; k += 1024)
+            refs.offsets[k] = 0;
+        return;
+    }
+    unsigned int* counts = refs.counts;
     unsigned int const total = num_keys * ntiles;
     unsigned int const per = (total + 1023) / 1024;
     unsigned int const begin = min(threadIdx.x * per, total);
@@ -113,26 +149,27 @@ __global__ void __launch_bounds__(1024)
         unsigned int const c = counts[i];
         counts[i] = running;
         if (i % ntiles == 0)
-            state.action_offsets[i / ntiles] = running;
+            refs.offsets[i / ntiles] = running;
         running += c;
     }
     if (threadIdx.x == 0)
-        state.action_offsets[num_keys] = state.size;
+        refs.offsets[num_keys] = refs.n;
 }
 
 /*!
- * Stable scatter of slot indices to their sorted thread positions.
+ * Stable scatter of indices to their sorted thread positions.
  *
- * A tile is processed in rounds of one slot per thread, in slot order. In a
- * round, each wave splits its lanes by key: the lowest remaining lane's key
- * is broadcast, the lanes holding it are found with one ballot and ranked
- * with mbcnt, and they leave the loop (one iteration per distinct key in
- * the wave). Wave totals per key go through LDS to order the waves; a
- * per-key running base carries the order across rounds and tiles.
+ * A tile is processed in rounds of one index per thread, in index order. In
+ * a round, each wave splits its lanes by key: the lowest remaining lane's
+ * key is broadcast, the lanes holding it are found with one ballot and
+ * ranked with mbcnt, and they leave the loop (one iteration per distinct
+ * key in the wave). Wave totals per key go through LDS to order the waves;
+ * a per-key running base carries the order across rounds and tiles.
  */
+template<class KeyFn>
 __global__ void __launch_bounds__(block_size)
-    sort_scatter_kernel(CoreStateRef const state,
-                        SortKey const kind,
+    sort_scatter_kernel(KeyFn const key_of,
+                        SplitRefs const refs,
                         unsigned int const num_keys,
                         unsigned int const ntiles)
 {
@@ -141,7 +178,7 @@ __global__ void __launch_bounds__(block_size)
     int const lane = wave::lane_id();
     int const w = threadIdx.x / wave::size;
     for (unsigned int k = threadIdx.x; k < num_keys; k += block_size)
-        base[k] = state.sort_counts[k * ntiles + blockIdx.x];
+        base[k] = refs.counts[k * ntiles + blockIdx.x];
     size_type const tile_base = blockIdx.x * tile_size;
     for (int r = 0; r < items_per_thread; ++r)
     {
@@ -149,9 +186,9 @@ __global__ void __launch_bounds__(block_size)
             (&wcount[0][0])[k] = 0;
         __syncthreads();
 
-        size_type const slot = tile_base + r * block_size + threadIdx.x;
-        bool const valid = slot < state.size;
-        unsigned int const key = valid ? sort_key(state, kind, slot, num_keys - 1) : 0u;
+        size_type const idx = tile_base + r * block_size + threadIdx.x;
+        bool const valid = idx < refs.n;
+        unsigned int const key = valid ? key_of(idx, num_keys - 1) : 0u;
         unsigned int rank = 0;
         std::uint64_t remaining = wave::ballot(valid);
         while (remaining)
@@ -173,7 +210,7 @@ __global__ void __launch_bounds__(block_size)
             unsigned int pos = base[key] + rank;
             for (int v = 0; v < w; ++v)
                 pos += wcount[v][key];
-            state.track_slots[pos] = slot;
+            refs.sorted[pos] = idx;
         }
         __syncthreads();
         for (unsigned int k = threadIdx.x; k < num_keys; k += block_size)
@@ -187,6 +224,23 @@ __global__ void __launch_bounds__(block_size)
     }
 }
 
+template<class KeyFn>
+hipError_t launch_split(KeyFn const& key_of,
+                        SplitRefs const& refs,
+                        unsigned int num_keys,
+                        hipStream_t stream)
+{
+    if (num_keys < 1 || num_keys > max_sort_keys)
+        return hipErrorInvalidValue;
+    unsigned int const nt = num_tiles(refs.n);
+    if (nt > 0)
+        sort_count_kernel<<<nt, block_size, 0, stream>>>(key_of, refs, num_keys, nt);
+    sort_scan_kernel<<<1, 1024, 0, stream>>>(refs, num_keys, nt);
+    if (nt > 0)
+        sort_scatter_kernel<<<nt, block_size, 0, stream>>>(key_of, refs, num_keys, nt);
+    return hipGetLastError();
+}
+
 //---------------------------------------------------------------------------//
 }  // namespace
 
@@ -196,13 +250,22 @@ hipError_t launch_sort_tracks(CoreStateRef const& state,
                               unsigned int num_keys,
                               hipStream_t stream)
 {
-    if (num_keys < 1 || num_keys > max_sort_keys)
-        return hipErrorInvalidValue;
-    unsigned int const nt = num_tiles(state.size);
-    sort_count_kernel<<<nt, block_size, 0, stream>>>(state, kind, num_keys, nt);
-    sort_scan_kernel<<<1, 1024, 0, stream>>>(state, num_keys, nt);
-    sort_scatter_kernel<<<nt, block_size, 0, stream>>>(state, kind, num_keys, nt);
-    return hipGetLastError();
+    return launch_split(StateKey{state, kind},
+                        SplitRefs{state.size, state.sort_counts, state.track_slots,
+                                  state.action_offsets},
+                        num_keys, stream);
+}
+
+hipError_t launch_partition_keys(std::uint8_t const* keys,
+                                 size_type n,
+                                 unsigned int num_keys,
+                                 std::uint32_t* counts,
+                                 std::uint32_t* sorted,
+                                 std::uint32_t* offsets,
+                                 hipStream_t stream)
+{
+    return launch_split(ArrayKey{keys}, SplitRefs{n, counts, sorted, offsets}, num_keys,
+                        stream);
 }
 
 size_type sort_counts_size(size_type num_slots)

@@ -443,6 +443,20 @@ int celer_hip_sort_tracks(const void* state_ref,
                           uint32_t key_kind,
                           uint32_t num_keys,
                           void* stream);
+/* The same stable multi-split over a caller's keys (device memory):
+ * indices 0..n-1 ordered by keys[i] (values at or above num_keys - 1 take
+ * key num_keys - 1), equal keys in increasing index order, into sorted[n],
+ * and offsets[0..num_keys] (offsets[num_keys] = n), all on the device.
+ * Allocates its histogram scratch (4 * 64 * ceil(n / 2048) bytes) on the
+ * stream and frees it there. The track-slot form is celer_hip_sort_tracks
+ * (detail/TrackSortUtils.cu:145-206 sorts track_slots by per-slot action
+ * keys with thrust::sort_by_key). */
+int celer_hip_partition_keys(const uint8_t* keys,
+                             uint32_t n,
+                             uint32_t num_keys,
+                             uint32_t* sorted,
+                             uint32_t* offsets,
+                             void* stream);
 
 /*---------------------------------------------------------------------------*
  * SERVING: continuous batching of events that arrive over time. No

@@ -77,7 +77,7 @@ def test_cylinder_edep_statistical_parity(amd, physics, order):
 
     def oracle_batch(b):
         prim = wl.cylinder_primaries(nprim, emin=energy, emax=energy * 1.0001, seed=500 + b)
-        st = ob.State(oprob, 1 << 12)
+        st = ob.State(oprob, 1 << 12, stream_id=b + 1)  # independent RNG streams per batch
         ob.run(st, prim)
         return st.get_f64("calo_edep")[:ndet]
 

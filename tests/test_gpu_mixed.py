@@ -56,7 +56,7 @@ def test_testem3_mixed_vs_fp64_oracle(amd):
 
     def oracle_batch(b):
         prim = wl.testem3_primaries(nprim, emin=energy, emax=energy * 1.0001, seed=500 + b)
-        st = ob.State(oprob, 1 << 12)
+        st = ob.State(oprob, 1 << 12, stream_id=b + 1)  # independent RNG streams per batch
         ob.run(st, prim)
         return st.get_f64("calo_edep")
 
@@ -87,7 +87,7 @@ def test_cylinder_mixed_vs_fp64_oracle(amd):
 
     def oracle_batch(b):
         prim = wl.cylinder_primaries(nprim, emin=energy, emax=energy * 1.0001, seed=500 + b)
-        st = ob.State(oprob, 1 << 12)
+        st = ob.State(oprob, 1 << 12, stream_id=b + 1)  # independent RNG streams per batch
         ob.run(st, prim)
         return st.get_f64("calo_edep")[:ndet]
 

@@ -118,7 +118,7 @@ def test_mixed_served_events_vs_fp64_oracle(amd, mixed_params):
                                           max_events=16))
     oracle = np.zeros(len(events))
     for i, ev in enumerate(events):
-        st = ob.State(oprob, 1 << 12)
+        st = ob.State(oprob, 1 << 12, stream_id=i + 1)  # independent RNG streams per event
         prim = ev.copy()
         prim["event_id"] = 0
         ob.run(st, prim)

@@ -82,7 +82,7 @@ def test_testem3_edep_statistical_parity(amd, physics):
     def oracle_batch(b):
         prim = wl.testem3_primaries(nprim, emin=energy, emax=energy * 1.0001, seed=500 + b,
                                     gamma_fraction=0.5)
-        st = ob.State(oprob, 1 << 12)
+        st = ob.State(oprob, 1 << 12, stream_id=b + 1)  # independent RNG streams per batch
         ob.run(st, prim)
         return st.get_f64("calo_edep")
 

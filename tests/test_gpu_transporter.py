@@ -108,7 +108,7 @@ def test_flushed_edep_vs_oracle(amd, params):
 
     def oracle(b):
         prim = wl.testem3_primaries(nprim, emin=energy, emax=energy * 1.0001, seed=900 + b)
-        st = ob.State(oprob, 1 << 12)
+        st = ob.State(oprob, 1 << 12, stream_id=b + 1)  # independent RNG streams per batch
         ob.run(st, prim)
         return st.get_f64("calo_edep")
 
