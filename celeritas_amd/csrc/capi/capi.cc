@@ -880,6 +880,36 @@ int celer_hip_partition_keys(const uint8_t* keys,
     return err;
 }
 
+int celer_hip_partition_keys_noalloc(const uint8_t* keys,
+                                     uint32_t n,
+                                     uint32_t num_keys,
+                                     uint32_t* sorted,
+                                     uint32_t* offsets,
+                                     void* stream)
+{
+    static_assert(CELER_PARTITION_NOALLOC_MAX == max_partition_noalloc,
+                  "allocation-free partition limit");
+    g_last_error.clear();
+    if (n > max_partition_noalloc)
+    {
+        g_last_error = "celer_hip_partition_keys_noalloc: " + std::to_string(n)
+                       + " keys exceed the allocation-free limit of "
+                       + std::to_string(max_partition_noalloc);
+        return hipErrorInvalidValue;
+    }
+    if (num_keys < 1 || num_keys > max_sort_keys || (n > 0 && (!keys || !sorted)) || !offsets)
+    {
+        g_last_error = "celer_hip_partition_keys_noalloc: invalid arguments";
+        return hipErrorInvalidValue;
+    }
+    std::uint32_t* counts = nullptr;
+    hipError_t err = partition_noalloc_scratch(&counts);
+    if (err != hipSuccess)
+        return err;
+    return launch_partition_keys(keys, n, num_keys, counts, sorted, offsets,
+                                 static_cast<hipStream_t>(stream));
+}
+
 int celer_read_seltzer_berger(const char* dir, int z, char* buf, size_t len, size_t* needed)
 {
     return guarded([&] {

@@ -40,5 +40,15 @@ hipError_t launch_partition_keys(std::uint8_t const* keys,
 //! Words of histogram scratch needed to split n indices
 size_type sort_counts_size(size_type num_slots);
 
+//! Largest n of the allocation-free partition (its scratch is static)
+inline constexpr size_type max_partition_noalloc = size_type(1) << 22;
+
+/*!
+ * Device address, on the current device, of the static scratch of the
+ * allocation-free partition (looked up once per device). Calls that
+ * overlap on one device share it and race.
+ */
+hipError_t partition_noalloc_scratch(std::uint32_t** ptr);
+
 //---------------------------------------------------------------------------//
 }  // namespace celeritas_amd

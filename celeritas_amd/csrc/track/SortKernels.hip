@@ -20,6 +20,8 @@
 //---------------------------------------------------------------------------//
 #include "SortKernels.hh"
 
+#include <mutex>
+
 #include "corecel/WaveOps.hh"
 
 namespace celeritas_amd
@@ -266,6 +268,47 @@ hipError_t launch_partition_keys(std::uint8_t const* keys,
 {
     return launch_split(ArrayKey{keys}, SplitRefs{n, counts, sorted, offsets}, num_keys,
                         stream);
+}
+
+//---------------------------------------------------------------------------//
+// ALLOCATION-FREE PARTITION SCRATCH
+//---------------------------------------------------------------------------//
+namespace
+{
+//! Histogram table of the allocation-free partition: max_sort_keys words
+//! per tile for up to max_partition_noalloc indices (512 KiB)
+constexpr size_type noalloc_scratch_words
+    = max_sort_keys * ((max_partition_noalloc + tile_size - 1) / tile_size);
+
+__device__ __align__(256) std::uint32_t scratch[noalloc_scratch_words];
+
+constexpr int max_devices = 64;
+std::mutex scratch_lock;
+std::uint32_t* scratch_ptr[max_devices] = {};
+}  // namespace
+
+hipError_t partition_noalloc_scratch(std::uint32_t** ptr)
+{
+    int device = 0;
+    hipError_t err = hipGetDevice(&device);
+    if (err != hipSuccess)
+        return err;
+    if (device < 0 || device >= max_devices)
+        return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> guard(scratch_lock);
+    if (!scratch_ptr[device])
+    {
+        // The array's name is only a host-side stand-in: its address on
+        // this device comes from the runtime, once (each device holds its
+        // own copy of the array)
+        void* addr = nullptr;
+        err = hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch));
+        if (err != hipSuccess)
+            return err;
+        scratch_ptr[device] = static_cast<std::uint32_t*>(addr);
+    }
+    *ptr = scratch_ptr[device];
+    return hipSuccess;
 }
 
 size_type sort_counts_size(size_type num_slots)

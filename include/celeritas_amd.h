@@ -457,6 +457,22 @@ int celer_hip_partition_keys(const uint8_t* keys,
                              uint32_t* sorted,
                              uint32_t* offsets,
                              void* stream);
+/* Allocation-free celer_hip_partition_keys. No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Its histogram scratch is a static device array of 512 KiB per GPU, held
+ * until the process ends, so n is limited to CELER_PARTITION_NOALLOC_MAX
+ * (larger n fails with hipErrorInvalidValue and a message from
+ * celer_amd_last_error()). The array keeps the last call's contents, every
+ * call rewrites what it reads, and calls that overlap on one GPU race.
+ * Results are identical to celer_hip_partition_keys. */
+#define CELER_PARTITION_NOALLOC_MAX (1u << 22)
+int celer_hip_partition_keys_noalloc(const uint8_t* keys,
+                                     uint32_t n,
+                                     uint32_t num_keys,
+                                     uint32_t* sorted,
+                                     uint32_t* offsets,
+                                     void* stream);
 
 /*---------------------------------------------------------------------------*
  * SERVING: continuous batching of events that arrive over time. No
