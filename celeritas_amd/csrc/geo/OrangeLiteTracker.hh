@@ -514,14 +514,24 @@ CELER_FUNCTION void initialize(GeoParamsRef const& p,
 }
 
 /*!
- * Copy the geometry of a parent track with a new direction.
+ * State of a point known to lie inside a volume, away from its faces: what
+ * initialize() gives for it, without the search.
  */
-CELER_FORCEINLINE_FUNCTION void
-initialize_from_parent(GeoTrack& g, GeoTrack const& parent, double const* dir)
+CELER_FORCEINLINE_FUNCTION void initialize_interior(GeoTrack& g,
+                                                    std::uint32_t volume,
+                                                    double const* pos,
+                                                    double const* dir)
 {
-    g = parent;
     for (int k = 0; k < 3; ++k)
+    {
+        g.pos[k] = pos[k];
         g.dir[k] = dir[k];
+    }
+    g.failed = 0;
+    g.volume = volume;
+    g.surface = invalid_id;
+    g.sense = 0;
+    g.boundary = static_cast<std::uint8_t>(BoundaryResult::exiting);
     g.clear_next();
 }
 

@@ -71,21 +71,36 @@ CELER_FUNCTION void apply_sampling_failure(CoreParamsRef const& params,
  * Initialize a track in a vacant slot from an initializer.
  *
  * Follows InitTracksExecutor (celeritas/track/detail/InitTracksExecutor.hh:
- * 71-175) for the case where no parent geometry is available (the reference
- * clears the parent slots at every generate step, ExtendFromPrimariesAction
- * .cc:184-200, so geometry is always rebuilt from the position).
+ * 71-175). A secondary made in the previous step starts at its parent's
+ * position, which lies inside the parent's volume (interactions happen
+ * away from boundaries), so its geometry state is the parent's volume with
+ * the state a point location would give there, read from the parent slot
+ * (\c parent_slot) instead of searched for; other initializers, and
+ * parents on a surface, are located from the position. (The reference
+ * copies parent geometry the same way but clears the parent slots at every
+ * generate step, ExtendFromPrimariesAction.cc:184-200; here they are
+ * cleared only when primaries are inserted, the case that clearing is for.)
  */
 CELER_FUNCTION void init_track(CoreParamsRef const& params,
                                CoreStateRef const& state,
                                size_type slot,
-                               TrackInitializer const& ti)
+                               TrackInitializer const& ti,
+                               size_type parent_slot = invalid_id)
 {
     init_sim(state.sim, slot, ti);
     state.particle.particle_id[slot] = static_cast<std::uint8_t>(ti.particle_id);
     state.particle.energy[slot] = ti.energy;
 
     geo::GeoTrack g;
-    geo::initialize(params.geo, g, ti.pos, ti.dir);
+    if (parent_slot != invalid_id && state.geo.surface[parent_slot] == invalid_id
+        && !state.geo.failed[parent_slot])
+    {
+        geo::initialize_interior(g, state.geo.volume[parent_slot], ti.pos, ti.dir);
+    }
+    else
+    {
+        geo::initialize(params.geo, g, ti.pos, ti.dir);
+    }
     g.store(state.geo, slot);
     if (CELER_UNLIKELY(g.failed || geo::is_outside(params.geo, g)))
     {

@@ -39,9 +39,10 @@ __global__ void primaries_prepare_kernel(CoreCounters* c, size_type count)
     c->init_base = c->num_initializers;
     c->num_initializers += count;
     c->num_generated += count;
-    // Parent slots are cleared at every generate step in the reference, so
-    // no initializer can copy its geometry from a parent slot.
-    c->prev_secondaries = 0;
+    // New primaries go on top of the previous step's secondaries, whose
+    // parent slots are then no longer at the front of the stack
+    if (count > 0)
+        c->prev_secondaries = 0;
 }
 
 /*!
@@ -136,7 +137,16 @@ __global__ void __launch_bounds__(block_size)
         return;
     TrackInitializer const ti = state.init.initializers[c.init_base - tid - 1];
     size_type const slot = state.init.vacancies[c.vacancy_base - tid - 1];
-    init_track(params, state, slot, ti);
+    // The previous step's secondaries are the top of the stack: parents[j]
+    // holds the parent slot of its j-th new initializer
+    size_type parent = invalid_id;
+    if (tid < c.prev_secondaries)
+    {
+        size_type const j = c.prev_secondaries - 1 - tid;
+        if (j < state.size)
+            parent = state.init.parents[j];
+    }
+    init_track(params, state, slot, ti, parent);
 }
 
 //---------------------------------------------------------------------------//
@@ -438,6 +448,11 @@ __global__ void __launch_bounds__(block_size)
             {
                 if (next_idx < state.init.capacity)
                     state.init.initializers[next_idx] = ti;
+                // Parent slot of the j-th new initializer (the slot stays
+                // occupied: a dead parent was refilled in place above)
+                std::uint32_t const j = next_idx - c.init_base;
+                if (in_place && j < state.size)
+                    state.init.parents[j] = slot;
                 ++next_idx;
             }
         }
