@@ -141,18 +141,57 @@ __global__ void __launch_bounds__(1024)
     unsigned int const per = (total + 1023) / 1024;
     unsigned int const begin = min(threadIdx.x * per, total);
     unsigned int const end = min(begin + per, total);
+    // A chunk of up to max_held words is read with independent loads into
+    // registers (one memory latency instead of one per word) and kept
+    // there for the rewrite; longer chunks are walked word by word
+    constexpr unsigned int max_held = 32;
+    unsigned int held[max_held];
+    bool const in_registers = per <= max_held;
     unsigned int sum = 0;
-    for (unsigned int i = begin; i < end; ++i)
-        sum += counts[i];
+    if (in_registers)
+    {
+#pragma unroll
+        for (unsigned int u = 0; u < max_held; ++u)
+        {
+            unsigned int const i = begin + u;
+            held[u] = i < end ? counts[i] : 0u;
+        }
+#pragma unroll
+        for (unsigned int u = 0; u < max_held; ++u)
+            sum += held[u];
+    }
+    else
+    {
+        for (unsigned int i = begin; i < end; ++i)
+            sum += counts[i];
+    }
     wave::BlockScan2<nw> scan(sum, 0u, lds);
     unsigned int running = scan.excl_a;
-    for (unsigned int i = begin; i < end; ++i)
+    if (in_registers)
     {
-        unsigned int const c = counts[i];
-        counts[i] = running;
-        if (i % ntiles == 0)
-            refs.offsets[i / ntiles] = running;
-        running += c;
+#pragma unroll
+        for (unsigned int u = 0; u < max_held; ++u)
+        {
+            unsigned int const i = begin + u;
+            if (i < end)
+            {
+                counts[i] = running;
+                if (i % ntiles == 0)
+                    refs.offsets[i / ntiles] = running;
+                running += held[u];
+            }
+        }
+    }
+    else
+    {
+        for (unsigned int i = begin; i < end; ++i)
+        {
+            unsigned int const c = counts[i];
+            counts[i] = running;
+            if (i % ntiles == 0)
+                refs.offsets[i / ntiles] = running;
+            running += c;
+        }
     }
     if (threadIdx.x == 0)
         refs.offsets[num_keys] = refs.n;

@@ -320,28 +320,47 @@ __global__ void __launch_bounds__(1024)
     }
     __syncthreads();
     unsigned int* part = state.init.block_scratch;
-    for (unsigned int base = 0; base < nblocks; base += 1024)
+    // Each pass gives every thread a run of up to held consecutive pairs,
+    // read with independent loads, so a pass costs one memory latency
+    constexpr unsigned int held = 8;
+    for (unsigned int base = 0; base < nblocks; base += 1024 * held)
     {
-        unsigned int const i = base + threadIdx.x;
+        unsigned int const first = base + threadIdx.x * held;
+        unsigned int va[held], vb[held];
         unsigned int a = 0, b = 0;
-        if (i < nblocks)
+#pragma unroll
+        for (unsigned int u = 0; u < held; ++u)
         {
-            a = part[2 * i];
-            b = part[2 * i + 1];
+            unsigned int const i = first + u;
+            va[u] = i < nblocks ? part[2 * i] : 0u;
+            vb[u] = i < nblocks ? part[2 * i + 1] : 0u;
+        }
+#pragma unroll
+        for (unsigned int u = 0; u < held; ++u)
+        {
+            a += va[u];
+            b += vb[u];
         }
         wave::BlockScan2<nw> scan(a, b, lds);
-        unsigned int const ca = carry[0];
-        unsigned int const cb = carry[1];
-        if (i < nblocks)
+        unsigned int ra = carry[0] + scan.excl_a;
+        unsigned int rb = carry[1] + scan.excl_b;
+#pragma unroll
+        for (unsigned int u = 0; u < held; ++u)
         {
-            part[2 * i] = ca + scan.excl_a;
-            part[2 * i + 1] = cb + scan.excl_b;
+            unsigned int const i = first + u;
+            if (i < nblocks)
+            {
+                part[2 * i] = ra;
+                part[2 * i + 1] = rb;
+            }
+            ra += va[u];
+            rb += vb[u];
         }
         __syncthreads();
         if (threadIdx.x == 0)
         {
-            carry[0] = ca + scan.total_a;
-            carry[1] = cb + scan.total_b;
+            carry[0] += scan.total_a;
+            carry[1] += scan.total_b;
         }
         __syncthreads();
     }
