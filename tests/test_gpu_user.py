@@ -164,8 +164,9 @@ def test_hit_buffer_overflow_and_graph_replay(amd, testem3):
     assert eager["overflow_steps"] == 0 and len(eager["detector"]) > 0
     for c in ("detector", "energy_deposition", "post.energy"):
         np.testing.assert_array_equal(eager[c], replay[c])
-    # Secondary track IDs come from an atomic counter: the same set
-    np.testing.assert_array_equal(np.sort(eager["track_id"]), np.sort(replay["track_id"]))
+    # Secondary track IDs come from an atomic counter, so which secondary
+    # gets which ID varies: only the primaries' IDs (0) are fixed
+    np.testing.assert_array_equal(eager["track_id"] == 0, replay["track_id"] == 0)
     # A buffer smaller than the hits drops whole steps and counts them
     small = run(64, False)
     n = len(small["detector"])
