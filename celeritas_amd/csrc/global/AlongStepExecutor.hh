@@ -89,8 +89,8 @@ CELER_FUNCTION void along_step_charged_track(CoreParamsRef const& params,
             if (!done)
             {
                 displaced = true;
-                true_path = alongstep::safety_step_limit(mt, msc_range, g.is_on_boundary(),
-                                                         safety, step, rng);
+                true_path = alongstep::step_limit(mt, msc_range, g.is_on_boundary(), safety,
+                                                  step, rng);
                 ps.msc_range_init[slot] = msc_range.range_init;
                 ps.msc_range_factor[slot] = msc_range.range_factor;
                 ps.msc_limit_min[slot] = msc_range.limit_min;

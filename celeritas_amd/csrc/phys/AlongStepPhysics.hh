@@ -286,6 +286,8 @@ inline constexpr double msc_limit_min = 1e-8;  //!< 10 limit_min_fix [cm]
 inline constexpr double msc_min_step = 1e-7;  //!< 1 nm [cm]
 inline constexpr double msc_min_sampling_energy = 1e-6;  //!< [MeV]
 inline constexpr double msc_min_scaling_energy = 5e-3;  //!< [MeV]
+inline constexpr double msc_safety_plus_rho = 1e-3;  //!< min_range [cm]
+inline constexpr double msc_safety_plus_alpha = 0.35;  //!< max_step_over_range
 
 CELER_FORCEINLINE_FUNCTION double fastpow(double a, double b)
 {
@@ -405,51 +407,82 @@ struct MscRange
 };
 
 /*!
- * True-path limit for the "safety" algorithm (UrbanMscSafetyStepLimit);
+ * True-path limit of the "minimal" (UrbanMscMinimalStepLimit.hh:53-126) or
+ * "safety" / "safety_plus" (UrbanMscSafetyStepLimit.hh:102-213) algorithm;
  * updates the slot's MscRange when needed. Draws only when sampling.
  */
 template<class Engine>
-CELER_FUNCTION double safety_step_limit(MscTrack const& t,
-                                        MscRange& msc_range,
-                                        bool on_boundary,
-                                        double safety,
-                                        double phys_step,
-                                        Engine& rng)
+CELER_FUNCTION double step_limit(MscTrack const& t,
+                                 MscRange& msc_range,
+                                 bool on_boundary,
+                                 double safety,
+                                 double phys_step,
+                                 Engine& rng)
 {
     PhysicsParamsRef const& p = t.p;
     double const range = t.range;
-    if (!msc_range || on_boundary)
+    double max_step = phys_step;
+    double limit_min, limit;
+    if (p.msc.step_limit_algorithm == 0)
     {
-        MscRange r;
-        r.range_factor = p.msc.range_factor;
-        r.range_init = range > t.lambda ? range : t.lambda;
-        if (t.lambda > p.msc.lambda_limit)
+        if (!msc_range)
         {
-            double const c = 0.75;
-            r.range_factor *= c + (1 - c) * t.lambda / p.msc.lambda_limit;
+            msc_range.range_init = HUGE_VAL;
+            msc_range.range_factor = p.msc.range_factor;
+            msc_range.limit_min = 10 * msc_limit_min_fix;
         }
-        // calc_limit_min
-        UrbanMscMaterial const& md = p.msc_material[t.mat];
-        double const min_mfp
-            = std::fma(t.energy, std::fma(t.energy, md.stepmin_coeff[1], md.stepmin_coeff[0]),
-                       2.0);
-        double xm = t.lambda / min_mfp;
-        xm *= t.pmdata().scaled_zeff;
-        if (t.energy < msc_min_scaling_energy)
-            xm *= 0.5 + 0.5 * t.energy / msc_min_scaling_energy;
-        r.limit_min = xm > msc_limit_min_fix ? xm : msc_limit_min_fix;
-        msc_range = r;
+        limit_min = msc_range.limit_min;
+        if (on_boundary)
+        {
+            double const init
+                = msc_range.range_factor * (range > t.lambda ? range : t.lambda);
+            msc_range.range_init = init > limit_min ? init : limit_min;
+        }
+        limit = msc_range.range_init;
     }
-    double const limit_min = msc_range.limit_min;
-    double limit = range;
-    if (safety < range)
+    else
     {
-        double const a = msc_range.range_factor * msc_range.range_init;
-        double const b = p.msc.safety_factor * safety;
-        limit = a > b ? a : b;
+        bool const plus = p.msc.step_limit_algorithm == 2;
+        if (!msc_range || on_boundary)
+        {
+            MscRange r;
+            r.range_factor = p.msc.range_factor;
+            r.range_init = plus ? range : (range > t.lambda ? range : t.lambda);
+            if (t.lambda > p.msc.lambda_limit)
+            {
+                double const c = plus ? 0.84 : 0.75;
+                r.range_factor *= c + (1 - c) * t.lambda / p.msc.lambda_limit;
+            }
+            // calc_limit_min
+            UrbanMscMaterial const& md = p.msc_material[t.mat];
+            double const min_mfp = std::fma(
+                t.energy, std::fma(t.energy, md.stepmin_coeff[1], md.stepmin_coeff[0]), 2.0);
+            double xm = t.lambda / min_mfp;
+            xm *= t.pmdata().scaled_zeff;
+            if (t.energy < msc_min_scaling_energy)
+                xm *= 0.5 + 0.5 * t.energy / msc_min_scaling_energy;
+            r.limit_min = xm > msc_limit_min_fix ? xm : msc_limit_min_fix;
+            msc_range = r;
+        }
+        limit_min = msc_range.limit_min;
+        limit = range;
+        if (safety < range)
+        {
+            double const a = msc_range.range_factor * msc_range.range_init;
+            double const b = p.msc.safety_factor * safety;
+            limit = a > b ? a : b;
+        }
+        limit = limit > limit_min ? limit : limit_min;
+        if (plus && range > msc_safety_plus_rho)
+        {
+            // Scaled step range alpha r + rho (1 - alpha)(2 - rho / r)
+            double const limit_step
+                = msc_safety_plus_alpha * range
+                  + msc_safety_plus_rho * (1 - msc_safety_plus_alpha)
+                        * (2 - msc_safety_plus_rho / range);
+            max_step = max_step < limit_step ? max_step : limit_step;
+        }
     }
-    limit = limit > limit_min ? limit : limit_min;
-    double const max_step = phys_step;
     if (max_step <= limit)
         return max_step;
     if (limit == limit_min)

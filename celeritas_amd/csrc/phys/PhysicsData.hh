@@ -188,7 +188,9 @@ struct MscScalars
     double lambda_limit{0.1};  //!< [cm]
     std::uint32_t xs_grid{invalid_id};  //!< first XsGrid: [material][e-, e+]
     std::uint8_t enabled{0};
-    std::uint8_t pad_[3]{};
+    //! MscStepLimitAlgorithm: 0 minimal, 1 safety, 2 safety_plus
+    std::uint8_t step_limit_algorithm{1};
+    std::uint8_t pad_[2]{};
 };
 
 //! Component of a material

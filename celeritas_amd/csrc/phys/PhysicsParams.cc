@@ -745,11 +745,14 @@ PhysicsParams::PhysicsParams(JsonValue const& inp, ActionRegistry* reg)
         auto const& mj = inp.at("msc");
         CELER_VALIDATE(mj.get_str("model", "urban") == "urban",
                        << "only the Urban MSC model is available");
-        CELER_VALIDATE(mj.get_str("step_limit_algorithm", "safety") == "safety",
-                       << "only the 'safety' MSC step limit algorithm is available");
+        std::string const alg = mj.get_str("step_limit_algorithm", "safety");
+        CELER_VALIDATE(alg == "minimal" || alg == "safety" || alg == "safety_plus",
+                       << "unknown MSC step limit algorithm '" << alg
+                       << "' (minimal, safety or safety_plus)");
         CELER_VALIDATE(ref_.electron_id != invalid_id && ref_.positron_id != invalid_id,
                        << "MSC needs electrons and positrons");
         MscScalars& ms = ref_.msc;
+        ms.step_limit_algorithm = alg == "minimal" ? 0 : (alg == "safety" ? 1 : 2);
         ms.range_factor = mj.get("range_factor", 0.04);
         ms.safety_factor = mj.get("safety_factor", 0.6);
         ms.lambda_limit = mj.get("lambda_limit", 0.1);

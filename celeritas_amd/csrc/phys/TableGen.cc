@@ -1080,7 +1080,8 @@ JsonValue generate_physics(JsonValue const& spec)
     options.obj["lpm"] = JsonValue::boolean(enable_lpm);
     for (auto const& kv : opts.obj)
         if (kv.first != "bins_per_decade" && kv.first != "emin" && kv.first != "emax"
-            && kv.first != "msc")
+            && kv.first != "msc" && kv.first != "msc_step_limit_algorithm"
+            && kv.first != "msc_range_factor")
             options.obj[kv.first] = kv.second;
     out.obj["options"] = options;
 
@@ -1089,8 +1090,9 @@ JsonValue generate_physics(JsonValue const& spec)
     {
         JsonValue msc = JsonValue::object();
         msc.obj["model"] = JsonValue::string("urban");
-        msc.obj["step_limit_algorithm"] = JsonValue::string("safety");
-        msc.obj["range_factor"] = JsonValue::number(0.04);
+        msc.obj["step_limit_algorithm"]
+            = JsonValue::string(opts.get_str("msc_step_limit_algorithm", "safety"));
+        msc.obj["range_factor"] = JsonValue::number(opts.get("msc_range_factor", 0.04));
         msc.obj["safety_factor"] = JsonValue::number(0.6);
         msc.obj["lambda_limit"] = JsonValue::number(0.1);
         JsonValue grids = per_material([&](Material const& m, std::size_t, double e) {

@@ -40,27 +40,6 @@ constexpr double c_light = 29979245800.0;  // [cm/s]
 
 //---------------------------------------------------------------------------//
 // Distributions
-template<class U>
-struct NormalDist
-{
-    double mean, stddev;
-    double spare = 0;
-    bool has_spare = false;
-    double operator()(U& u)
-    {
-        if (has_spare)
-        {
-            has_spare = false;
-            return std::fma(spare, stddev, mean);
-        }
-        double const twopi = 2 * pi;
-        double const theta = twopi * u();
-        double const r = std::sqrt(-2 * std::log(u()));
-        spare = r * std::cos(theta);
-        has_spare = true;
-        return std::fma(r * std::sin(theta), stddev, mean);
-    }
-};
 
 template<class U>
 unsigned int poisson(double lambda, U& u)
@@ -245,44 +224,24 @@ void along_step_charged_impl(State& s)
             if (!have_path)
             {
                 displaced = true;
-                // UrbanMscSafetyStepLimit
-                bool const valid = t.msc_range_init > 0 && t.msc_range_factor > 0
-                                   && t.msc_limit_min > 0;
-                if (!valid || on_boundary)
-                {
-                    double factor = pd.msc.range_factor;
-                    double const init = std::max(ctx.range, ctx.lambda);
-                    if (ctx.lambda > pd.msc.lambda_limit)
-                        factor *= 0.75 + 0.25 * ctx.lambda / pd.msc.lambda_limit;
-                    MscDataO::Material const& md = pd.msc.mat[mat];
-                    double const e = t.energy;
-                    double xm = ctx.lambda / std::fma(e, std::fma(e, md.stepmin[1], md.stepmin[0]), 2.0);
-                    xm *= md.scaled_zeff[ctx.par];
-                    if (e < 5e-3)
-                        xm *= 0.5 + 0.5 * e / 5e-3;
-                    t.msc_range_init = init;
-                    t.msc_range_factor = factor;
-                    t.msc_limit_min = std::max(xm, msc_limit_min_fix);
-                }
-                double const lmin = t.msc_limit_min;
-                double limit = ctx.range;
-                if (safety < ctx.range)
-                    limit = std::max(t.msc_range_factor * t.msc_range_init,
-                                     pd.msc.safety_factor * safety);
-                limit = std::max(limit, lmin);
-                if (phys_step <= limit)
-                {
-                    true_path = phys_step;
-                }
-                else if (limit == lmin)
-                {
-                    true_path = lmin;
-                }
-                else
-                {
-                    NormalDist<XorwowUniform> n{limit, 0.1 * (limit - lmin)};
-                    true_path = std::clamp(n(u), lmin, phys_step);
-                }
+                MscLimitInput in;
+                in.alg = pd.msc.limit_alg;
+                in.energy = t.energy;
+                in.range = ctx.range;
+                in.lambda = ctx.lambda;
+                in.md = &pd.msc.mat[mat];
+                in.par = ctx.par;
+                in.range_factor = pd.msc.range_factor;
+                in.safety_factor = pd.msc.safety_factor;
+                in.lambda_limit = pd.msc.lambda_limit;
+                in.on_boundary = on_boundary;
+                in.safety = safety;
+                in.phys_step = phys_step;
+                MscRangeO r{t.msc_range_init, t.msc_range_factor, t.msc_limit_min};
+                true_path = msc_step_limit(in, r, u);
+                t.msc_range_init = r.range_init;
+                t.msc_range_factor = r.range_factor;
+                t.msc_limit_min = r.limit_min;
             }
             bool limited = true_path < phys_step;
             // MscStepToGeo

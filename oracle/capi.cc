@@ -916,6 +916,49 @@ void orc_msc_material_coeffs(double zeff, double* out)
     std::copy(std::begin(v), std::end(v), out);
 }
 
+/*!
+ * The step-limit sampling of UrbanMsc.test.cc:296-371 for an electron in a
+ * material of effective Z: per energy i (with its range and MSC MFP), a
+ * fresh MSC range, then nsamples limits of a physical step equal to the
+ * range, on a boundary or not, with zero safety; the engine (from
+ * orc_mt_create) continues across calls. out per energy: mean step,
+ * range_init, range_factor, limit_min.
+ */
+int orc_msc_step_limit_sample(void* mt, int alg, int on_boundary, double zeff,
+                              double range_factor, double const* energy,
+                              double const* range, double const* lambda, std::uint32_t n,
+                              std::uint32_t nsamples, double* out)
+{
+    return guard([&] {
+        if (alg < 0 || alg > 2)
+            throw std::runtime_error("unknown MSC step limit algorithm");
+        MscDataO::Material const md = msc_material_coeffs(zeff);
+        MtUniform u{*static_cast<std::mt19937*>(mt)};
+        for (std::uint32_t i = 0; i < n; ++i)
+        {
+            MscLimitInput in;
+            in.alg = static_cast<MscLimitAlg>(alg);
+            in.energy = energy[i];
+            in.range = range[i];
+            in.lambda = lambda[i];
+            in.md = &md;
+            in.par = 0;
+            in.range_factor = range_factor;
+            in.on_boundary = on_boundary != 0;
+            in.safety = 0;
+            in.phys_step = range[i];
+            MscRangeO r;
+            double sum = 0;
+            for (std::uint32_t k = 0; k < nsamples; ++k)
+                sum += msc_step_limit(in, r, u);
+            out[4 * i] = sum / nsamples;
+            out[4 * i + 1] = r.range_init;
+            out[4 * i + 2] = r.range_factor;
+            out[4 * i + 3] = r.limit_min;
+        }
+    });
+}
+
 double orc_urban_positron_correction(double zeff, double y)
 {
     return urban_positron_correction(zeff, y);

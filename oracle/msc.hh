@@ -25,6 +25,7 @@
 //---------------------------------------------------------------------------//
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 
 #include "physics.hh"
@@ -65,6 +66,133 @@ extern int g_msc_mutation;
 inline constexpr double msc_geom_limit = 5e-9;  // [cm]
 inline constexpr double msc_limit_min_fix = 1e-9;  // [cm]
 inline constexpr double msc_min_step = 1e-7;  // [cm]
+
+//! Box-Muller normal deviate keeping its spare (NormalDistribution.hh)
+template<class U>
+struct NormalDist
+{
+    double mean, stddev;
+    double spare = 0;
+    bool has_spare = false;
+    double operator()(U& u)
+    {
+        if (has_spare)
+        {
+            has_spare = false;
+            return std::fma(spare, stddev, mean);
+        }
+        double const twopi = 2 * 3.14159265358979323846;
+        double const theta = twopi * u();
+        double const r = std::sqrt(-2 * std::log(u()));
+        spare = r * std::cos(theta);
+        has_spare = true;
+        return std::fma(r * std::sin(theta), stddev, mean);
+    }
+};
+
+//! Persistent MSC range of a track (MscRange; invalid until first set)
+struct MscRangeO
+{
+    double range_init = 0;
+    double range_factor = 0;
+    double limit_min = 0;
+    bool valid() const { return range_init > 0 && range_factor > 0 && limit_min > 0; }
+};
+
+//! Inputs of one MSC true-path limit
+struct MscLimitInput
+{
+    MscLimitAlg alg = MscLimitAlg::safety;
+    double energy = 0;  //!< [MeV]
+    double range = 0;  //!< continuous-slowing-down range [cm]
+    double lambda = 0;  //!< MSC transport MFP [cm]
+    MscDataO::Material const* md = nullptr;
+    int par = 0;  //!< 0 e-, 1 e+
+    double range_factor = 0.04;
+    double safety_factor = 0.6;
+    double lambda_limit = 0.1;  //!< [cm]
+    bool on_boundary = false;
+    double safety = 0;  //!< [cm]
+    double phys_step = 0;  //!< [cm]
+};
+
+/*!
+ * True-path limit: UrbanMscMinimalStepLimit.hh:53-126 ("minimal") or
+ * UrbanMscSafetyStepLimit.hh:102-213 ("safety", "safety_plus"). Updates the
+ * track's MSC range; draws two uniforms when it samples the limit.
+ */
+template<class U>
+double msc_step_limit(MscLimitInput const& in, MscRangeO& r, U& u)
+{
+    double max_step = in.phys_step;
+    double limit_min, limit;
+    if (in.alg == MscLimitAlg::minimal)
+    {
+        if (!r.valid())
+        {
+            r.range_init = HUGE_VAL;
+            r.range_factor = in.range_factor;
+            r.limit_min = 10 * msc_limit_min_fix;
+        }
+        limit_min = r.limit_min;
+        if (in.on_boundary)
+        {
+            double const init = r.range_factor * std::max(in.range, in.lambda);
+            r.range_init = std::max(init, limit_min);
+        }
+        limit = r.range_init;
+    }
+    else
+    {
+        bool const plus = in.alg == MscLimitAlg::safety_plus;
+        if (!r.valid() || in.on_boundary)
+        {
+            double factor = in.range_factor;
+            double const init = plus && g_msc_mutation != 11
+                                    ? in.range
+                                    : std::max(in.range, in.lambda);
+            if (in.lambda > in.lambda_limit)
+            {
+                double const c = plus && g_msc_mutation != 12 ? 0.84 : 0.75;
+                factor *= c + (1 - c) * in.lambda / in.lambda_limit;
+            }
+            // calc_limit_min (G4UrbanMscModel::ComputeTlimitmin)
+            MscDataO::Material const& md = *in.md;
+            double const e = in.energy;
+            double xm = in.lambda / std::fma(e, std::fma(e, md.stepmin[1], md.stepmin[0]), 2.0);
+            xm *= md.scaled_zeff[in.par];
+            if (e < 5e-3)
+                xm *= 0.5 + 0.5 * e / 5e-3;
+            r.range_init = init;
+            r.range_factor = factor;
+            r.limit_min = std::max(xm, msc_limit_min_fix);
+        }
+        limit_min = r.limit_min;
+        limit = in.range;
+        if (in.safety < in.range)
+            limit = std::max(r.range_factor * r.range_init, in.safety_factor * in.safety);
+        limit = std::max(limit, limit_min);
+        if (plus && g_msc_mutation != 13)
+        {
+            // Scaled step range alpha r + rho (1 - alpha)(2 - rho / r)
+            double const rho = 1e-3;  // [cm]
+            double const alpha = 0.35;
+            if (in.range > rho)
+            {
+                double const limit_step
+                    = alpha * in.range + rho * (1 - alpha) * (2 - rho / in.range);
+                max_step = std::min(max_step, limit_step);
+            }
+        }
+    }
+    if (max_step <= limit)
+        return max_step;
+    if (limit == limit_min)
+        return limit_min;
+    NormalDist<U> n{limit, 0.1 * (limit - limit_min)};
+    return std::clamp(n(u), limit_min, max_step);
+}
+
 
 //! Material coefficients (UrbanMscParams::calc_material_data and the
 //! per-particle scaled Zeff and d_over_r)

@@ -338,6 +338,15 @@ void build_physics(Json const& inp, PhysicsData& out)
         msc.range_factor = mj->num("range_factor", 0.04);
         msc.safety_factor = mj->num("safety_factor", 0.6);
         msc.lambda_limit = mj->num("lambda_limit", 0.1);
+        std::string const alg = mj->str("step_limit_algorithm", "safety");
+        if (alg == "minimal")
+            msc.limit_alg = MscLimitAlg::minimal;
+        else if (alg == "safety_plus")
+            msc.limit_alg = MscLimitAlg::safety_plus;
+        else if (alg == "safety")
+            msc.limit_alg = MscLimitAlg::safety;
+        else
+            throw std::runtime_error("unknown MSC step limit algorithm '" + alg + "'");
         msc.xs[0] = grids_of(*mj, "electron", nullptr, nullptr, num_mats);
         msc.xs[1] = grids_of(*mj, "positron", nullptr, nullptr, num_mats);
         msc.low = std::exp(msc.xs[0].at(0).front);

@@ -110,6 +110,14 @@ struct MaterialDef
 };
 
 //! Urban MSC data (em/data/UrbanMscData.hh, params/UrbanMscParams.cc)
+//! Step-limit algorithms of Urban MSC (MscStepLimitAlgorithm)
+enum class MscLimitAlg
+{
+    minimal = 0,
+    safety = 1,
+    safety_plus = 2,
+};
+
 struct MscDataO
 {
     struct Material
@@ -124,6 +132,7 @@ struct MscDataO
     bool enabled = false;
     double low = 0, high = 0;  // energy limits [MeV]
     double range_factor = 0.04, safety_factor = 0.6, lambda_limit = 0.1;
+    MscLimitAlg limit_alg = MscLimitAlg::safety;
     std::vector<XsGridO> xs[2];  // [e-, e+][material]: sigma_tr * E^2
     std::vector<Material> mat;
 };

@@ -138,7 +138,7 @@ int main(int argc, char** argv)
             if (msc)
             {
                 mt.lambda = mt.calc_msc_mfp(energy);
-                true_path = alongstep::safety_step_limit(mt, mr, safety == 0, safety, step, rng);
+                true_path = alongstep::step_limit(mt, mr, safety == 0, safety, step, rng);
                 geom_path = alongstep::to_geo(mt, true_path, &alpha);
                 if (geom_path > mt.lambda) geom_path = mt.lambda;
                 if (U(gen) < 0.3)

@@ -529,3 +529,23 @@ def run_events_parallel(problem: "Problem", primaries: np.ndarray, num_slots: in
              calo.ctypes.data if num_calo else None, num_calo))
     return {"track_steps": int(out[0]), "iterations": int(out[1]), "seconds": out[2],
             "threads": int(out[3]), "calo_edep": calo[:num_calo]}
+
+
+def msc_step_limit_sample(mt, alg: str, on_boundary: bool, zeff: float, range_factor: float,
+                          energies, ranges, lambdas, nsamples: int) -> dict:
+    """UrbanMsc.test.cc's step-limit sampling through the oracle: per
+    energy, the mean limit of nsamples and the final MSC range."""
+    algs = {"minimal": 0, "safety": 1, "safety_plus": 2}
+    f = load().orc_msc_step_limit_sample
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                  ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                  ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]
+    f.restype = ctypes.c_int
+    e = np.ascontiguousarray(energies, np.float64)
+    r = np.ascontiguousarray(ranges, np.float64)
+    lam = np.ascontiguousarray(lambdas, np.float64)
+    out = np.zeros((len(e), 4))
+    _check(f(mt, algs[alg], int(on_boundary), zeff, range_factor, e.ctypes.data,
+             r.ctypes.data, lam.ctypes.data, len(e), nsamples, out.ctypes.data))
+    return {"mean_step": out[:, 0], "range_init": out[:, 1], "range_factor": out[:, 2],
+            "limit_min": out[:, 3]}

@@ -61,6 +61,37 @@ def test_testem3_early_steps_match_oracle(amd, physics):
         np.testing.assert_allclose(pos[live], opos[live], rtol=1e-9, atol=1e-9)
 
 
+@pytest.mark.parametrize("alg", ["minimal", "safety_plus"])
+def test_testem3_msc_step_limit_algorithms_match_oracle(amd, alg):
+    """The other Urban MSC step-limit algorithms (UrbanMscMinimalStepLimit,
+    UrbanMscSafetyStepLimit with safety_plus): early TestEm3 steps match
+    the oracle, whose algorithms reproduce the reference's step_limit
+    golden values (tests/test_oracle_msc_limit.py)."""
+    spec = wl.testem3_physics_spec()
+    spec["options"] = dict(spec["options"], msc_step_limit_algorithm=alg)
+    physics = amd_pkg.generate_physics(spec)
+    assert physics["msc"]["step_limit_algorithm"] == alg
+    prob = _problem(physics, 4242)
+    stepper = amd.Stepper(amd.CoreParams(prob), 4096)
+    ost = ob.State(ob.Problem(prob), 4096)
+    prim = wl.testem3_primaries(8, emin=1e3, emax=1e4, seed=7)
+    for step in range(25):
+        if step == 0:
+            r = stepper(prim)
+            ost.insert_primaries(prim)
+        else:
+            r = stepper()
+        o = ob.step_once(ost)
+        assert dict(r) == o, f"step {step}"
+        st = stepper.state
+        np.testing.assert_array_equal(st.get("status"), ost.get_u32("status"))
+        live = st.get("status") == 2
+        np.testing.assert_allclose(st.get("energy")[live], ost.get_f64("energy")[live],
+                                   rtol=1e-9, atol=1e-12)
+        np.testing.assert_allclose(st.get("step_length")[live],
+                                   ost.get_f64("step_length")[live], rtol=1e-9, atol=1e-12)
+
+
 def _batch_means(run_batch, nbatch):
     tallies = np.array([run_batch(b) for b in range(nbatch)])
     return tallies.mean(axis=0), tallies.std(axis=0, ddof=1) / np.sqrt(nbatch)
