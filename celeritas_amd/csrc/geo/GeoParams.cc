@@ -258,6 +258,9 @@ std::vector<std::int32_t> parse_logic_string(std::string const& s)
 //---------------------------------------------------------------------------//
 GeoInput geo_input_from_orange_json(JsonValue const& doc)
 {
+    CELER_VALIDATE(!doc.has("universes") || doc.at("universes").size() == 1,
+                   << "multi-universe geometry is not supported (the file has "
+                   << doc.at("universes").size() << " universes)");
     JsonValue const& univ = doc.has("universes") ? doc.at("universes")[0] : doc;
     CELER_VALIDATE(univ.get_str("_type", "simple unit") == "simple unit",
                    << "only a single simple unit is supported");

@@ -27,6 +27,8 @@ __global__ void __launch_bounds__(block_size)
                               ThreadSlots const ts)
 {
     extern __shared__ uint4 staged_tables[];
+    if (!workgroup_has_work(ts, state.size))
+        return;
     CoreParamsRef const params = stage_tables(params_in, staged_tables);
     size_type slot;
     if (!thread_to_slot(ts, blockIdx.x * blockDim.x + threadIdx.x, state.size, &slot))

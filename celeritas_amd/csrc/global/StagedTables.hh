@@ -35,6 +35,25 @@ inline unsigned int staged_table_bytes(CoreParamsRef const& p)
     return (n > 0 && n <= max_staged_table_bytes) ? n : 0;
 }
 
+/*!
+ * Whether any thread of this workgroup maps to a slot.
+ *
+ * A sorted launch covers all slots but its range is read on the device, so
+ * most of its workgroups can be empty: they return before staging (the
+ * answer is uniform over the workgroup, so no barrier is left waiting).
+ */
+__device__ __forceinline__ bool workgroup_has_work(ThreadSlots const& ts,
+                                                   size_type size)
+{
+    size_type const first = blockIdx.x * blockDim.x;
+    if (ts.range)
+    {
+        size_type const end = ts.range_end ? *ts.range_end : ts.range[1];
+        return first < end - ts.range[0];
+    }
+    return first < size;
+}
+
 namespace detail
 {
 template<class T>

@@ -43,10 +43,12 @@ __global__ void __launch_bounds__(block_size)
                     ThreadSlots const ts)
 {
     extern __shared__ uint4 staged_tables[];
-    CoreParamsRef const params = stage_tables(params_in, staged_tables);
     size_type const tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid == 0)
         state.counters->secondary_stack_size = 0;
+    if (!workgroup_has_work(ts, state.size))
+        return;
+    CoreParamsRef const params = stage_tables(params_in, staged_tables);
     size_type slot;
     if (!thread_to_slot(ts, tid, state.size, &slot))
         return;
@@ -90,6 +92,8 @@ __global__ void __launch_bounds__(block_size)
                            ThreadSlots const ts)
 {
     extern __shared__ uint4 staged_tables[];
+    if (!workgroup_has_work(ts, state.size))
+        return;
     CoreParamsRef const params = stage_tables(params_in, staged_tables);
     size_type slot;
     if (!thread_to_slot(ts, blockIdx.x * blockDim.x + threadIdx.x, state.size, &slot))
@@ -138,6 +142,8 @@ __global__ void __launch_bounds__(block_size)
                         ThreadSlots const ts)
 {
     extern __shared__ uint4 staged_tables[];
+    if (!workgroup_has_work(ts, state.size))
+        return;
     CoreParamsRef const params = stage_tables(params_in, staged_tables);
     size_type slot;
     if (!thread_to_slot(ts, blockIdx.x * blockDim.x + threadIdx.x, state.size, &slot))
