@@ -76,6 +76,9 @@ def parse_args():
                          "reindex_step_limit_action, reindex_both_action)")
     ap.add_argument("--steps-per-sync", type=int, default=4,
                     help="steps launched (graph replays) per counter readback")
+    ap.add_argument("--serial-actions", action="store_true",
+                    help="launch every action on the state's stream (no concurrent "
+                         "groups of track-disjoint actions)")
     ap.add_argument("--no-graph", action="store_true",
                     help="enqueue every step's kernels instead of replaying a graph")
     ap.add_argument("--no-autotune", action="store_true",
@@ -211,9 +214,11 @@ def main():
         ndet = 100
 
     def problem_of(precision="fp64"):
-        return make_problem(physics=physics, init_capacity=1 << 26,
+        prob = make_problem(physics=physics, init_capacity=1 << 26,
                             max_events=max(args.primaries, 1),
                             track_order=args.track_order, precision=precision)
+        prob["concurrent_actions"] = not args.serial_actions
+        return prob
 
     problem = problem_of()
     block_size, tune_trials = autotune_block(amd, problem, primaries, args.slots, args)
@@ -258,6 +263,7 @@ def main():
             "slots": args.slots,
             "track_order": args.track_order,
             "graph_replay": not args.no_graph,
+            "concurrent_actions": not args.serial_actions,
             "steps_per_readback": args.steps_per_sync,
             "workgroup_size": block_size,
             "autotune_trials": tune_trials,

@@ -54,6 +54,14 @@ class StepActionInterface : virtual public ActionInterface
   public:
     virtual StepActionOrder order() const = 0;
     virtual void step(CoreParams const&, CoreState&) const = 0;
+    //! Whether the action reads and writes only the tracks whose along- or
+    //! post-step action is its own (the reference's ConditionalTrackExecutor
+    //! filter), so that such actions of one order touch disjoint tracks and
+    //! may run concurrently
+    virtual bool track_disjoint() const { return false; }
+    //! Whether a step of this action enqueues any work (an action whose
+    //! work another action launches on its behalf returns false)
+    virtual bool enqueues_work(CoreParams const&) const { return true; }
 };
 
 using CoreStepActionInterface = StepActionInterface;

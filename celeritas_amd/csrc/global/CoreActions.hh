@@ -133,6 +133,7 @@ class AlongStepNeutralAction final : public StepActionInterface,
     {
     }
     void step(CoreParams const&, CoreState&) const final;
+    bool track_disjoint() const final { return true; }
     StepActionOrder order() const final { return StepActionOrder::along; }
 };
 
@@ -146,6 +147,7 @@ class AlongStepChargedAction final : public StepActionInterface,
     {
     }
     void step(CoreParams const&, CoreState&) const final;
+    bool track_disjoint() const final { return true; }
     StepActionOrder order() const final { return StepActionOrder::along; }
 };
 
@@ -158,6 +160,7 @@ class BoundaryAction final : public StepActionInterface, public ConcreteAction
     {
     }
     void step(CoreParams const&, CoreState&) const final;
+    bool track_disjoint() const final { return true; }
     StepActionOrder order() const final { return StepActionOrder::post; }
 };
 
@@ -170,6 +173,7 @@ class TrackingCutAction final : public StepActionInterface, public ConcreteActio
     {
     }
     void step(CoreParams const&, CoreState&) const final;
+    bool track_disjoint() const final { return true; }
     StepActionOrder order() const final { return StepActionOrder::post; }
 };
 

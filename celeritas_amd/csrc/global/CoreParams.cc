@@ -107,6 +107,7 @@ CoreParams::CoreParams(Input inp)
     sc.track_order = inp.track_order;
     sc.precision = inp.precision;
     fuse_interactions_ = inp.fuse_interactions;
+    concurrent_actions_ = inp.concurrent_actions;
 
     ref_.geo = geo_->device_ref();
     ref_.volume_to_material = d_vol_to_mat_.data();
@@ -235,6 +236,7 @@ std::shared_ptr<CoreParams> CoreParams::from_json(std::string const& text)
                    << "precision must be 'fp64' or 'mixed'");
     inp.precision = prec == "fp64" ? PrecisionMode::fp64 : PrecisionMode::mixed;
     inp.fuse_interactions = doc.get_bool("fuse_interactions", true);
+    inp.concurrent_actions = doc.get_bool("concurrent_actions", true);
     return std::make_shared<CoreParams>(std::move(inp));
 }
 

@@ -46,6 +46,10 @@ class CoreParams
         //! Sample every discrete model in one launch (the first model
         //! action launches it; the others launch nothing)
         bool fuse_interactions{true};
+        //! Run the track-disjoint actions of one step order (the along-step
+        //! actions; the interactions, boundary and tracking cut) on
+        //! concurrent streams
+        bool concurrent_actions{true};
         //! Along-step action for charged particles (label), created by caller
         std::string along_step_label;
         //! Uniform magnetic field (enabled for the uniform-field along-step)
@@ -67,6 +71,7 @@ class CoreParams
     TrackOrder track_order() const { return ref_.scalars.track_order; }
     PrecisionMode precision() const { return ref_.scalars.precision; }
     bool fuse_interactions() const { return fuse_interactions_; }
+    bool concurrent_actions() const { return concurrent_actions_; }
     std::vector<std::uint32_t> const& volume_to_material() const
     {
         return vol_to_mat_;
@@ -82,6 +87,7 @@ class CoreParams
     size_type max_events_;
     size_type num_detectors_{0};
     bool fuse_interactions_{true};
+    bool concurrent_actions_{true};
     std::vector<std::uint32_t> vol_to_mat_;
     DeviceVector<std::uint32_t> d_vol_to_mat_;
     DeviceVector<std::uint32_t> d_vol_to_det_;

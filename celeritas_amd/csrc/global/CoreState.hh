@@ -38,7 +38,13 @@ class CoreState
     CoreStateRef const& ref() const { return ref_; }
     size_type size() const { return ref_.size; }
     std::uint32_t stream_id() const { return ref_.stream_id; }
-    hipStream_t stream() const { return stream_; }
+    //! Stream the step kernels are launched on: the state's own, or a side
+    //! stream while the stepper runs an action concurrently with others
+    hipStream_t stream() const { return launch_stream_ ? launch_stream_ : stream_; }
+    //! The state's own stream
+    hipStream_t own_stream() const { return stream_; }
+    //! Launch on a side stream (null: back to the state's own)
+    void launch_stream(hipStream_t s) { launch_stream_ = s; }
 
     //! Copy device counters to the host mirror (synchronizes the stream)
     CoreCounters const& sync_counters();
@@ -77,6 +83,7 @@ class CoreState
     CoreStateRef ref_;
     DeviceArena arena_;
     hipStream_t stream_{nullptr};
+    hipStream_t launch_stream_{nullptr};
     CoreCounters* host_counters_{nullptr};
     DeviceVector<Primary> primaries_;
     DeviceVector<std::uint32_t> primary_ranks_;

@@ -42,6 +42,40 @@ Stepper::Stepper(StepperInput input)
     for (auto const& a : actions_)
         labels_.emplace_back(a->label());
 
+    // Groups of consecutive track-disjoint actions of one order run on
+    // concurrent streams; every other action is a group of its own
+    std::size_t num_concurrent = 0;
+    for (std::size_t i = 0; i < actions_.size();)
+    {
+        std::size_t j = i + 1;
+        if (params_->concurrent_actions() && actions_[i]->track_disjoint())
+        {
+            while (j < actions_.size() && actions_[j]->track_disjoint()
+                   && actions_[j]->order() == actions_[i]->order())
+                ++j;
+        }
+        std::size_t working = 0;
+        for (std::size_t k = i; k < j; ++k)
+            working += actions_[k]->enqueues_work(*params_) ? 1 : 0;
+        if (working > 1)
+            ++num_concurrent;
+        else
+            j = i + 1;
+        groups_.emplace_back(i, j);
+        i = j;
+    }
+    if (num_concurrent > 0)
+    {
+        side_streams_.resize(max_side_streams);
+        for (auto& s : side_streams_)
+            CELER_HIP_CALL(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        fork_events_.resize(num_concurrent);
+        join_events_.resize(num_concurrent * max_side_streams);
+        for (auto* ev : {&fork_events_, &join_events_})
+            for (auto& e : *ev)
+                CELER_HIP_CALL(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+
     ActionId pid = reg.find_action(ExtendFromPrimariesAction::label_str());
     CELER_VALIDATE(pid != invalid_id,
                    << "primary generator was not added to the stepping loop");
@@ -60,6 +94,11 @@ Stepper::~Stepper()
 {
     for (auto e : events_)
         (void)hipEventDestroy(e);
+    for (auto* ev : {&fork_events_, &join_events_})
+        for (auto e : *ev)
+            (void)hipEventDestroy(e);
+    for (auto s : side_streams_)
+        (void)hipStreamDestroy(s);
     if (graph_exec_)
         (void)hipGraphExecDestroy(graph_exec_);
 }
@@ -82,8 +121,7 @@ void Stepper::capture_step()
     hipStream_t const stream = state_->stream();
     hipGraph_t graph = nullptr;
     CELER_HIP_CALL(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    for (auto const& a : actions_)
-        a->step(*params_, *state_);
+    this->launch_actions();
     CELER_HIP_CALL(hipStreamEndCapture(stream, &graph));
     hipError_t const err = hipGraphInstantiate(&graph_exec_, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
@@ -121,11 +159,66 @@ void Stepper::enqueue_step()
     }
     else
     {
-        for (std::size_t i = 0; i < actions_.size(); ++i)
+        this->launch_actions();
+    }
+}
+
+/*!
+ * Enqueue every action once, forking each concurrent group onto the side
+ * streams: the group's first working action stays on the state's stream,
+ * the others go round-robin to the side streams, which wait for the work
+ * before the group and are waited for by the state's stream after it.
+ */
+void Stepper::launch_actions()
+{
+    // Launches go back to the state's own stream however the group ends
+    struct Restore
+    {
+        CoreState& s;
+        ~Restore() { s.launch_stream(nullptr); }
+    } restore{*state_};
+
+    hipStream_t const own = state_->own_stream();
+    std::size_t fork = 0;
+    for (auto const& [begin, end] : groups_)
+    {
+        if (end - begin == 1)
         {
+            ScopedProfiling prof(labels_[begin].c_str());
+            actions_[begin]->step(*params_, *state_);
+            continue;
+        }
+        CELER_HIP_CALL(hipEventRecord(fork_events_[fork], own));
+        std::vector<bool> used(side_streams_.size(), false);
+        std::size_t working = 0;
+        for (std::size_t i = begin; i < end; ++i)
+        {
+            hipStream_t launch = nullptr;
+            if (actions_[i]->enqueues_work(*params_) && working++ > 0)
+            {
+                std::size_t const s = (working - 2) % side_streams_.size();
+                if (!used[s])
+                {
+                    CELER_HIP_CALL(
+                        hipStreamWaitEvent(side_streams_[s], fork_events_[fork], 0));
+                    used[s] = true;
+                }
+                launch = side_streams_[s];
+            }
+            state_->launch_stream(launch);
             ScopedProfiling prof(labels_[i].c_str());
             actions_[i]->step(*params_, *state_);
         }
+        state_->launch_stream(nullptr);
+        for (std::size_t s = 0; s < side_streams_.size(); ++s)
+        {
+            if (!used[s])
+                continue;
+            hipEvent_t const join = join_events_[fork * side_streams_.size() + s];
+            CELER_HIP_CALL(hipEventRecord(join, side_streams_[s]));
+            CELER_HIP_CALL(hipStreamWaitEvent(own, join, 0));
+        }
+        ++fork;
     }
 }
 

@@ -49,6 +49,12 @@ struct StepperResult
  * (corecel/sys/ActionGroups.t.hh:23-54). Each action enqueues its kernels on
  * the state's stream; the only host synchronization per step is the counter
  * readback that produces the StepperResult.
+ *
+ * Consecutive actions of one order that touch disjoint tracks (the along-
+ * step actions; the interactions, boundary crossing and tracking cut) are
+ * forked onto side streams and joined back with events, so a latency-bound
+ * kernel with few waves shares the GPU with its siblings instead of running
+ * alone; a captured step graph holds the same fork and join.
  */
 class Stepper
 {
@@ -108,8 +114,17 @@ class Stepper
     bool events_pending_{false};
     bool use_graph_{false};
     hipGraphExec_t graph_exec_{nullptr};
+    //! Side streams of concurrent groups (with the state's own stream, one
+    //! per hardware queue a process gets by default)
+    static constexpr std::size_t max_side_streams = 3;
+    //! [begin, end) of each group of actions_ launched together
+    std::vector<std::pair<std::size_t, std::size_t>> groups_;
+    std::vector<hipStream_t> side_streams_;
+    std::vector<hipEvent_t> fork_events_;  //!< one per concurrent group
+    std::vector<hipEvent_t> join_events_;  //!< one per side launch
 
     StepperResult make_result();
+    void launch_actions();
     StepperResult step_impl(bool new_primaries);
     void launch_step(bool new_primaries);
 };

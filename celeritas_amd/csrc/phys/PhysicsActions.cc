@@ -31,6 +31,12 @@ void DiscreteSelectAction::step(CoreParams const& params, CoreState& state) cons
         state.stream()));
 }
 
+bool ModelAction::enqueues_work(CoreParams const& params) const
+{
+    return !params.fuse_interactions()
+           || this->action_id() == params.device_ref().phys.scalars.first_model_action;
+}
+
 void ModelAction::step(CoreParams const& params, CoreState& state) const
 {
     bool const mixed = params.precision() == PrecisionMode::mixed;

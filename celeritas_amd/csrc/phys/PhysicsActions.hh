@@ -45,6 +45,8 @@ class ModelAction final : public StepActionInterface, public ConcreteAction
     {
     }
     void step(CoreParams const&, CoreState&) const final;
+    bool track_disjoint() const final { return true; }
+    bool enqueues_work(CoreParams const&) const final;
     StepActionOrder order() const final { return StepActionOrder::post; }
     std::uint32_t model_class() const { return model_class_; }
 
