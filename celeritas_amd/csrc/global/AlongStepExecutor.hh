@@ -36,7 +36,7 @@ namespace exec
 {
 //---------------------------------------------------------------------------//
 template<bool UseField>
-CELER_FUNCTION void along_step_charged_track(CoreParamsRef const& params,
+CELER_FORCEINLINE_FUNCTION void along_step_charged_track(CoreParamsRef const& params,
                                              CoreStateRef const& state,
                                              size_type slot,
                                              TrackStatus status)

@@ -19,7 +19,7 @@ namespace
 {
 constexpr int block_size = 256;
 
-template<bool UseField>
+template<bool UseField, bool Staged>
 __global__ void __launch_bounds__(block_size)
     along_step_charged_kernel(CoreParamsRef const params_in,
                               CoreStateRef const state,
@@ -29,7 +29,7 @@ __global__ void __launch_bounds__(block_size)
     extern __shared__ uint4 staged_tables[];
     if (!workgroup_has_work(ts, state.size))
         return;
-    CoreParamsRef const params = stage_tables(params_in, staged_tables);
+    CoreParamsRef const params = stage_tables<Staged>(params_in, staged_tables);
     size_type slot;
     if (!thread_to_slot(ts, blockIdx.x * blockDim.x + threadIdx.x, state.size, &slot))
         return;
@@ -50,12 +50,12 @@ hipError_t CELER_PREC_FN(launch_along_step_charged)(CoreParamsRef const& p,
                                      ThreadSlots const& ts,
                                      hipStream_t st)
 {
-    if (p.field.enabled)
-        along_step_charged_kernel<true>
-            <<<grid_of(s), s.block_size, staged_table_bytes(p), st>>>(p, s, action, ts);
-    else
-        along_step_charged_kernel<false>
-            <<<grid_of(s), s.block_size, staged_table_bytes(p), st>>>(p, s, action, ts);
+    unsigned int const lds = staged_table_bytes(p);
+    auto* kernel = p.field.enabled ? (lds ? along_step_charged_kernel<true, true>
+                                          : along_step_charged_kernel<true, false>)
+                                   : (lds ? along_step_charged_kernel<false, true>
+                                          : along_step_charged_kernel<false, false>);
+    kernel<<<grid_of(s), s.block_size, lds, st>>>(p, s, action, ts);
     return hipGetLastError();
 }
 

@@ -218,21 +218,33 @@ inline unsigned int grid_of(CoreStateRef const& s)
     return (s.size + s.block_size - 1) / s.block_size;
 }
 
+/*!
+ * Largest number of track slots in a state.
+ *
+ * Slot indices are masked to this many bits where a kernel obtains them, so
+ * the compiler knows the byte offset of any per-slot element fits 32 bits:
+ * every SoA access of a kernel then shares one 32-bit offset register with
+ * the array base in scalar registers, instead of a 64-bit address in vector
+ * registers per array.
+ */
+inline constexpr size_type max_track_slots = size_type(1) << 28;
+
 CELER_FORCEINLINE_FUNCTION bool
 thread_to_slot(ThreadSlots const& ts, size_type tid, size_type size, size_type* slot)
 {
+    constexpr size_type mask = max_track_slots - 1;
     if (ts.range)
     {
         size_type const begin = ts.range[0];
         size_type const end = ts.range_end ? *ts.range_end : ts.range[1];
         if (tid >= end - begin)
             return false;
-        *slot = ts.slots[begin + tid];
+        *slot = ts.slots[begin + tid] & mask;
         return true;
     }
     if (tid >= size)
         return false;
-    *slot = ts.slots ? ts.slots[tid] : tid;
+    *slot = (ts.slots ? ts.slots[tid] : tid) & mask;
     return true;
 }
 

@@ -24,6 +24,9 @@ CoreState::CoreState(CoreParams const& params,
 {
     require_device();
     CELER_VALIDATE(num_slots > 0, << "number of track slots must be positive");
+    CELER_VALIDATE(num_slots <= max_track_slots,
+                   << "number of track slots " << num_slots << " exceeds the maximum "
+                   << max_track_slots);
     ref_.size = num_slots;
     ref_.stream_id = stream_id;
     num_detectors_ = params.num_detectors();

@@ -70,14 +70,20 @@ __device__ __forceinline__ T const* rebase(T const* p, char const* from, char* t
  *
  * Every thread of the workgroup must call this before any of them returns
  * (it contains a workgroup barrier).
+ *
+ * A kernel instantiated with \c Staged true always stages (its launcher
+ * checks \c staged_table_bytes first): its table pointers then derive from
+ * the LDS array alone, so the compiler reads the tables with LDS
+ * instructions and 32-bit addresses rather than generic 64-bit ones.
  */
+template<bool Staged>
 __device__ __forceinline__ CoreParamsRef stage_tables(CoreParamsRef const& params,
                                                       void* lds)
 {
+    if constexpr (!Staged)
+        return params;
     PhysicsParamsRef const& pp = params.phys;
     unsigned int const n = pp.hot_bytes;
-    if (n == 0 || n > max_staged_table_bytes)
-        return params;
     auto const* src = reinterpret_cast<uint4 const*>(pp.hot_base);
     auto* dst = static_cast<uint4*>(lds);
     for (unsigned int i = threadIdx.x; i < n / 16; i += blockDim.x)

@@ -37,6 +37,7 @@ using exec::status_of;
 //---------------------------------------------------------------------------//
 // PRE-STEP
 //---------------------------------------------------------------------------//
+template<bool Staged>
 __global__ void __launch_bounds__(block_size)
     pre_step_kernel(CoreParamsRef const params_in,
                     CoreStateRef const state,
@@ -48,7 +49,7 @@ __global__ void __launch_bounds__(block_size)
         state.counters->secondary_stack_size = 0;
     if (!workgroup_has_work(ts, state.size))
         return;
-    CoreParamsRef const params = stage_tables(params_in, staged_tables);
+    CoreParamsRef const params = stage_tables<Staged>(params_in, staged_tables);
     size_type slot;
     if (!thread_to_slot(ts, tid, state.size, &slot))
         return;
@@ -86,6 +87,7 @@ __global__ void __launch_bounds__(block_size)
 //---------------------------------------------------------------------------//
 // DISCRETE SELECT
 //---------------------------------------------------------------------------//
+template<bool Staged>
 __global__ void __launch_bounds__(block_size)
     discrete_select_kernel(CoreParamsRef const params_in,
                            CoreStateRef const state,
@@ -94,7 +96,7 @@ __global__ void __launch_bounds__(block_size)
     extern __shared__ uint4 staged_tables[];
     if (!workgroup_has_work(ts, state.size))
         return;
-    CoreParamsRef const params = stage_tables(params_in, staged_tables);
+    CoreParamsRef const params = stage_tables<Staged>(params_in, staged_tables);
     size_type slot;
     if (!thread_to_slot(ts, blockIdx.x * blockDim.x + threadIdx.x, state.size, &slot))
         return;
@@ -136,6 +138,7 @@ __global__ void __launch_bounds__(block_size)
  * bound runs of the separate per-model launches overlap instead of
  * following one another.
  */
+template<bool Staged>
 __global__ void __launch_bounds__(block_size)
     interactions_kernel(CoreParamsRef const params_in,
                         CoreStateRef const state,
@@ -144,7 +147,7 @@ __global__ void __launch_bounds__(block_size)
     extern __shared__ uint4 staged_tables[];
     if (!workgroup_has_work(ts, state.size))
         return;
-    CoreParamsRef const params = stage_tables(params_in, staged_tables);
+    CoreParamsRef const params = stage_tables<Staged>(params_in, staged_tables);
     size_type slot;
     if (!thread_to_slot(ts, blockIdx.x * blockDim.x + threadIdx.x, state.size, &slot))
         return;
@@ -326,7 +329,9 @@ hipError_t CELER_PREC_FN(launch_pre_step)(CoreParamsRef const& p,
                            ThreadSlots const& ts,
                            hipStream_t st)
 {
-    pre_step_kernel<<<grid_of(s), s.block_size, staged_table_bytes(p), st>>>(p, s, ts);
+    unsigned int const lds = staged_table_bytes(p);
+    (lds ? pre_step_kernel<true> : pre_step_kernel<false>)<<<grid_of(s), s.block_size, lds, st>>>(
+        p, s, ts);
     return hipGetLastError();
 }
 
@@ -348,7 +353,9 @@ hipError_t CELER_PREC_FN(launch_discrete_select)(CoreParamsRef const& p,
                                   ThreadSlots const& ts,
                                   hipStream_t st)
 {
-    discrete_select_kernel<<<grid_of(s), s.block_size, staged_table_bytes(p), st>>>(p, s, ts);
+    unsigned int const lds = staged_table_bytes(p);
+    (lds ? discrete_select_kernel<true> : discrete_select_kernel<false>)<<<grid_of(s), s.block_size,
+                                                                         lds, st>>>(p, s, ts);
     return hipGetLastError();
 }
 
@@ -385,7 +392,9 @@ hipError_t CELER_PREC_FN(launch_interactions)(CoreParamsRef const& p,
                                ThreadSlots const& ts,
                                hipStream_t st)
 {
-    interactions_kernel<<<grid_of(s), s.block_size, staged_table_bytes(p), st>>>(p, s, ts);
+    unsigned int const lds = staged_table_bytes(p);
+    (lds ? interactions_kernel<true> : interactions_kernel<false>)<<<grid_of(s), s.block_size, lds,
+                                                                   st>>>(p, s, ts);
     return hipGetLastError();
 }
 
