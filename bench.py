@@ -40,6 +40,15 @@ HBM_PEAK_GBS = 8000.0
 
 #: Algorithmic bytes per processed track and per slot scanned, from the SoA
 #: layout in FP64 (fields each kernel must read and write once per track).
+#: Action whose byte model applies to a kernel (kernel registry names)
+KERNEL_ACTION = {
+    "along-step-linear": "along-step-general-linear",
+    "along-step-field": "along-step-uniform-msc",
+    "pre-step": "pre-step",
+    "along-step-neutral": "along-step-neutral",
+    "boundary": "geo-boundary",
+}
+
 BYTES_PER_UNIT = {
     # status, particle, energy, volume, mfp read; deposition, secondary
     # span, element, pre-volume, step, actions, per-process and total XS,
@@ -283,7 +292,7 @@ def main():
         line["serve"] = serve_run(amd, wl, physics, args)
     if not args.no_roofline:
         line["roofline"] = roofline(amd, problem, primaries, args.slots, args.track_order,
-                                    block_size, value)
+                                    block_size, value, max(args.steps_per_sync, 1))
     if args.cpu_primaries > 0 or args.cpu_config1 > 0:
         line["cpu_baseline"] = cpu_baseline(amd, wl, problem, physics, args)
     print(json.dumps(line), flush=True)
@@ -481,22 +490,39 @@ def step_bytes(new_tracks_per_step):
     return persistent + BYTES_PER_NEW_TRACK * new_tracks_per_step, persistent
 
 
-def roofline(amd, problem, primaries, slots, track_order, block_size, headline):
-    """Dominant kernel's algorithmic bandwidth from an instrumented run
-    (HIP events around each action's launches on the state's stream), and
-    the whole step's algorithmic bandwidth at the headline rate."""
-    st = amd.Stepper(amd.CoreParams(problem), slots, action_times=True)
+def roofline(amd, problem, primaries, slots, track_order, block_size, headline,
+             steps_per_sync):
+    """Dominant kernel's algorithmic bandwidth from an instrumented run, and
+    the whole step's algorithmic bandwidth at the headline rate.
+
+    The instrumented run is the same transport with eager launches, the
+    actions one after another on the state's stream (as the rocprofv3
+    kernel trace in profiles/ runs it), and every kernel launch carrying
+    HIP start and stop events that the runtime sets at the dispatch's own
+    start and end (hipExtLaunchKernel), so a kernel's time is its
+    dispatch's duration."""
+    st = amd.Stepper(amd.CoreParams(dict(problem, concurrent_actions=False)), slots)
     st.state.set_block_size(block_size)
     st.diagnostics(enable=True)
-    r = st.run(primaries)
-    times = st.action_times()
+    amd.kernel_times(reset=True)
+    amd.kernel_timing(True)
+    try:
+        r = st.run(primaries, steps_per_sync=steps_per_sync)
+        kt = amd.kernel_times(reset=True)
+    finally:
+        amd.kernel_timing(False)
+    times = {}
+    for t in kt:
+        secs, n = times.get(t["name"], (0.0, 0))
+        times[t["name"]] = (secs + t["seconds"], n + int(t["launches"]))
     diag = st.diagnostics()
     new_tracks = int(st.state.get("track_counters", len(primaries)).sum())
     ranked = sorted(times.items(), key=lambda kv: -kv[1][0])
-    log("action device times [s]: " + ", ".join(f"{k}={v[0]:.4f}" for k, v in ranked))
+    log("kernel device times [s]: " + ", ".join(f"{k}={v[0]:.4f}" for k, v in ranked))
     log(f"work counters: {diag}")
-    label, (secs, launches) = next((kv for kv in ranked if kv[0] in BYTES_PER_UNIT),
+    kname, (secs, launches) = next((kv for kv in ranked if kv[0] in KERNEL_ACTION),
                                    ranked[0])
+    label = KERNEL_ACTION[kname]
     per_unit, per_slot, counter = BYTES_PER_UNIT[label]
     units = diag[counter]
     if sorted_launch(label, track_order):
@@ -517,9 +543,15 @@ def roofline(amd, problem, primaries, slots, track_order, block_size, headline):
     per_step_new = (new_tracks - len(primaries)) / max(r["track_steps"], 1)
     b_step, persistent = step_bytes(per_step_new)
     whole = headline * b_step / 1e9
+    # Registers and occupancy of the kernel as the runtime reports them
+    attrs = next((a for a in amd.kernel_attributes()
+                  if a["name"] == kname and a["block_size"] == block_size), None)
+    total = sum(v[0] for v in times.values())
     return {
         "bound": "hbm",
-        "kernel": label,
+        "kernel": kname,
+        "action": label,
+        "kernel_attributes": attrs,
         "achieved": achieved,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
@@ -531,7 +563,10 @@ def roofline(amd, problem, primaries, slots, track_order, block_size, headline):
         "algorithmic_bytes_per_launch": per_launch,
         "avg_launch_us": 1e6 * avg_launch,
         "launches": launches,
-        "share_of_step_time": secs / sum(v[0] for v in times.values()),
+        "timing": "HIP start/stop events set at each dispatch's start and end "
+                  "(hipExtLaunchKernel), eager launches, actions in sequence",
+        "share_of_step_time": secs / total,
+        "kernel_avg_us": {k: round(1e6 * v[0] / max(v[1], 1), 2) for k, v in ranked[:12]},
         "work_counters": diag,
         "step": {
             "b_step": b_step,

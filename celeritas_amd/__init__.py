@@ -647,6 +647,34 @@ def _read_json_text(fn, *args) -> dict:
     return json.loads(buf.value.decode())
 
 
+def kernel_attributes() -> list:
+    """Attributes of the step kernels launched so far in this process (one
+    entry per kernel and launch shape): registers, scratch, LDS, resident
+    workgroups per CU, waves per SIMD and occupancy, as the runtime reports
+    them (the reference's KernelRegistry, corecel/sys/KernelAttributes.hh)."""
+    f = _load().celer_kernel_attributes_json
+    f.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+    return _read_json_text(f)
+
+
+def kernel_timing(enable: bool) -> None:
+    """Time every following step-kernel launch with events attached to the
+    dispatch itself (hipExtLaunchKernel), or stop timing."""
+    f = _load().celer_kernel_timing
+    f.argtypes = [ctypes.c_int]
+    f.restype = ctypes.c_int
+    _check(f(int(bool(enable))))
+
+
+def kernel_times(reset: bool = False) -> list:
+    """Device seconds and launch counts per kernel and launch shape of the
+    timed launches (waits for them to end)."""
+    f = _load().celer_kernel_times_json
+    f.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t,
+                  ctypes.POINTER(ctypes.c_size_t)]
+    return _read_json_text(f, int(bool(reset)))
+
+
 def read_seltzer_berger(z: int, directory: str | None = None) -> dict:
     """Seltzer-Berger table of element ``z`` from a G4EMLOW ``brem_SB``
     folder (``$G4LEDATA/brem_SB`` when ``directory`` is None), as the

@@ -11,6 +11,7 @@
 
 #include "accel/LocalTransporter.hh"
 #include "accel/ServingLoop.hh"
+#include "corecel/KernelRegistry.hh"
 #include "global/CoreActions.hh"
 #include "global/CoreParams.hh"
 #include "global/CoreState.hh"
@@ -908,6 +909,27 @@ int celer_hip_partition_keys_noalloc(const uint8_t* keys,
         return err;
     return launch_partition_keys(keys, n, num_keys, counts, sorted, offsets,
                                  static_cast<hipStream_t>(stream));
+}
+
+int celer_kernel_attributes_json(char* buf, size_t len, size_t* needed)
+{
+    return guarded(
+        [&] { copy_text(KernelRegistry::instance().to_json(), buf, len, needed); });
+}
+
+int celer_kernel_timing(int enable)
+{
+    return guarded([&] { KernelRegistry::instance().timing(enable != 0); });
+}
+
+int celer_kernel_times_json(int reset, char* buf, size_t len, size_t* needed)
+{
+    // A size query (no buffer) leaves the times in place for the call that
+    // reads them
+    return guarded([&] {
+        copy_text(KernelRegistry::instance().times_json(reset != 0 && buf != nullptr), buf,
+                  len, needed);
+    });
 }
 
 int celer_read_seltzer_berger(const char* dir, int z, char* buf, size_t len, size_t* needed)

@@ -49,6 +49,14 @@ __device__ __forceinline__ void count_active_lanes(unsigned long long* counter)
         atomicAdd(counter, static_cast<unsigned long long>(__popcll(mask)));
 }
 
+//! Add the wave's active lanes to one of \c Spread copies of a counter,
+//! chosen by the workgroup index (the copies are summed when read)
+template<unsigned int Spread>
+__device__ __forceinline__ void count_active_lanes_spread(unsigned long long* copies)
+{
+    count_active_lanes(copies + (blockIdx.x % Spread));
+}
+
 //! Inclusive scan of a 32-bit value over the wave
 __device__ __forceinline__ unsigned int inclusive_scan(unsigned int v)
 {

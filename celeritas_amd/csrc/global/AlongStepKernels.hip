@@ -9,6 +9,7 @@
 //---------------------------------------------------------------------------//
 #include "corecel/Precision.hh"
 #include "CoreActions.hh"
+#include "corecel/KernelLaunch.hh"
 #include "corecel/WaveOps.hh"
 #include "AlongStepExecutor.hh"
 #include "StagedTables.hh"
@@ -37,7 +38,7 @@ __global__ void __launch_bounds__(block_size)
     if (status == TrackStatus::inactive || state.sim.along_step_action[slot] != action)
         return;
     if (state.diag)
-        wave::count_active_lanes(state.diag + diag_along_charged);
+        wave::count_active_lanes_spread<diag_spread>(state.diag + diag_along_charged * diag_spread);
     exec::along_step_charged_track<UseField>(params, state, slot, status);
 }
 
@@ -55,7 +56,12 @@ hipError_t CELER_PREC_FN(launch_along_step_charged)(CoreParamsRef const& p,
                                           : along_step_charged_kernel<true, false>)
                                    : (lds ? along_step_charged_kernel<false, true>
                                           : along_step_charged_kernel<false, false>);
-    kernel<<<grid_of(s), s.block_size, lds, st>>>(p, s, action, ts);
+#if CELER_PREC_IS_MIXED
+    char const* name = p.field.enabled ? "along-step-field-mixed" : "along-step-linear-mixed";
+#else
+    char const* name = p.field.enabled ? "along-step-field" : "along-step-linear";
+#endif
+    launch_kernel(name, kernel, grid_of(s), s.block_size, lds, st, p, s, action, ts);
     return hipGetLastError();
 }
 

@@ -260,6 +260,11 @@ enum DiagCounter : int
     num_diag_counters = 8
 };
 
+//! Copies of each work counter, one per workgroup index modulo this: the
+//! waves of a launch add to different addresses instead of contending for
+//! one (row c of the counters holds counter c's copies)
+inline constexpr unsigned int diag_spread = 256;
+
 //---------------------------------------------------------------------------//
 //! Action IDs of the core actions, known to the kernels
 struct CoreScalars

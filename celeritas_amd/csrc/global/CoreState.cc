@@ -230,10 +230,9 @@ void CoreState::enable_diagnostics(bool enable)
     if (enable)
     {
         if (diag_.size() == 0)
-            diag_.resize(num_diag_counters);
+            diag_.resize(num_diag_counters * diag_spread);
         CELER_HIP_CALL(hipMemsetAsync(diag_.data(), 0,
-                                      sizeof(unsigned long long) * num_diag_counters,
-                                      stream_));
+                                      sizeof(unsigned long long) * diag_.size(), stream_));
         ref_.diag = diag_.data();
     }
     else
@@ -247,14 +246,19 @@ std::vector<std::uint64_t> CoreState::read_diagnostics(bool reset)
     std::vector<std::uint64_t> result;
     if (!ref_.diag)
         return result;
-    std::vector<unsigned long long> tmp(num_diag_counters);
+    std::vector<unsigned long long> tmp(num_diag_counters * diag_spread);
     CELER_HIP_CALL(hipMemcpyAsync(tmp.data(), ref_.diag, sizeof(unsigned long long) * tmp.size(),
                                   hipMemcpyDeviceToHost, stream_));
     CELER_HIP_CALL(hipStreamSynchronize(stream_));
     if (reset)
         CELER_HIP_CALL(hipMemsetAsync(ref_.diag, 0, sizeof(unsigned long long) * tmp.size(),
                                       stream_));
-    result.assign(tmp.begin(), tmp.end());
+    result.assign(num_diag_counters, 0);
+    for (unsigned int c = 0; c < num_diag_counters; ++c)
+    {
+        for (unsigned int i = 0; i < diag_spread; ++i)
+            result[c] += tmp[c * diag_spread + i];
+    }
     return result;
 }
 

@@ -14,6 +14,7 @@
 #include "corecel/Precision.hh"
 #include "CoreActions.hh"
 
+#include "corecel/KernelLaunch.hh"
 #include "corecel/WaveOps.hh"
 #include "StagedTables.hh"
 #include "StepExecutors.hh"
@@ -60,7 +61,7 @@ __global__ void __launch_bounds__(block_size)
         return;
     }
     if (state.diag)
-        wave::count_active_lanes(state.diag + diag_pre_step);
+        wave::count_active_lanes_spread<diag_spread>(state.diag + diag_pre_step * diag_spread);
     exec::pre_step_track(params, state, slot, status);
 }
 
@@ -80,7 +81,7 @@ __global__ void __launch_bounds__(block_size)
     if (status == TrackStatus::inactive || state.sim.along_step_action[slot] != action)
         return;
     if (state.diag)
-        wave::count_active_lanes(state.diag + diag_along_neutral);
+        wave::count_active_lanes_spread<diag_spread>(state.diag + diag_along_neutral * diag_spread);
     exec::along_step_neutral_track(params, state, slot, status);
 }
 
@@ -104,7 +105,7 @@ __global__ void __launch_bounds__(block_size)
         || status_of(state, slot) == TrackStatus::inactive)
         return;
     if (state.diag)
-        wave::count_active_lanes(state.diag + diag_discrete);
+        wave::count_active_lanes_spread<diag_spread>(state.diag + diag_discrete * diag_spread);
     exec::discrete_select_track(params, state, slot);
 }
 
@@ -127,7 +128,7 @@ __global__ void __launch_bounds__(block_size)
         || status_of(state, slot) == TrackStatus::inactive)
         return;
     if (state.diag)
-        wave::count_active_lanes(state.diag + diag_interaction);
+        wave::count_active_lanes_spread<diag_spread>(state.diag + diag_interaction * diag_spread);
     exec::interaction_track<M>(params, state, slot);
 }
 
@@ -159,7 +160,7 @@ __global__ void __launch_bounds__(block_size)
     if (model >= params.phys.scalars.num_models)
         return;
     if (state.diag)
-        wave::count_active_lanes(state.diag + diag_interaction);
+        wave::count_active_lanes_spread<diag_spread>(state.diag + diag_interaction * diag_spread);
     switch (static_cast<ModelClass>(params.phys.model_class[model]))
     {
 #define CELER_MODEL_CASE(M)                                  \
@@ -195,7 +196,7 @@ __global__ void __launch_bounds__(block_size)
         || status_of(state, slot) != TrackStatus::alive)
         return;
     if (state.diag)
-        wave::count_active_lanes(state.diag + diag_boundary);
+        wave::count_active_lanes_spread<diag_spread>(state.diag + diag_boundary * diag_spread);
     exec::boundary_track(params, state, slot);
 }
 
@@ -324,14 +325,20 @@ __global__ void __launch_bounds__(block_size)
 //---------------------------------------------------------------------------//
 // LAUNCHERS
 //---------------------------------------------------------------------------//
+#if CELER_PREC_IS_MIXED
+#    define CELER_KERNEL_NAME(S) S "-mixed"
+#else
+#    define CELER_KERNEL_NAME(S) S
+#endif
+
 hipError_t CELER_PREC_FN(launch_pre_step)(CoreParamsRef const& p,
                            CoreStateRef const& s,
                            ThreadSlots const& ts,
                            hipStream_t st)
 {
     unsigned int const lds = staged_table_bytes(p);
-    (lds ? pre_step_kernel<true> : pre_step_kernel<false>)<<<grid_of(s), s.block_size, lds, st>>>(
-        p, s, ts);
+    launch_kernel(CELER_KERNEL_NAME("pre-step"), lds ? pre_step_kernel<true> : pre_step_kernel<false>,
+           grid_of(s), s.block_size, lds, st, p, s, ts);
     return hipGetLastError();
 }
 
@@ -342,7 +349,8 @@ hipError_t launch_along_step_neutral(CoreParamsRef const& p,
                                      ThreadSlots const& ts,
                                      hipStream_t st)
 {
-    along_step_neutral_kernel<<<grid_of(s), s.block_size, 0, st>>>(p, s, action, ts);
+    launch_kernel("along-step-neutral", along_step_neutral_kernel, grid_of(s), s.block_size, 0, st, p, s,
+           action, ts);
     return hipGetLastError();
 }
 
@@ -354,8 +362,9 @@ hipError_t CELER_PREC_FN(launch_discrete_select)(CoreParamsRef const& p,
                                   hipStream_t st)
 {
     unsigned int const lds = staged_table_bytes(p);
-    (lds ? discrete_select_kernel<true> : discrete_select_kernel<false>)<<<grid_of(s), s.block_size,
-                                                                         lds, st>>>(p, s, ts);
+    launch_kernel(CELER_KERNEL_NAME("discrete-select"),
+           lds ? discrete_select_kernel<true> : discrete_select_kernel<false>, grid_of(s),
+           s.block_size, lds, st, p, s, ts);
     return hipGetLastError();
 }
 
@@ -368,9 +377,10 @@ hipError_t CELER_PREC_FN(launch_interaction)(CoreParamsRef const& p,
 {
     switch (static_cast<ModelClass>(model))
     {
-#define CELER_LAUNCH_MODEL(M)                                              \
-    case ModelClass::M:                                                    \
-        interaction_kernel<ModelClass::M><<<grid_of(s), s.block_size, 0, st>>>(p, s, action, ts); \
+#define CELER_LAUNCH_MODEL(M)                                                               \
+    case ModelClass::M:                                                                     \
+        launch_kernel(CELER_KERNEL_NAME("interaction-" #M), interaction_kernel<ModelClass::M>,    \
+               grid_of(s), s.block_size, 0, st, p, s, action, ts);                          \
         break;
         CELER_LAUNCH_MODEL(klein_nishina)
         CELER_LAUNCH_MODEL(livermore_pe)
@@ -393,8 +403,9 @@ hipError_t CELER_PREC_FN(launch_interactions)(CoreParamsRef const& p,
                                hipStream_t st)
 {
     unsigned int const lds = staged_table_bytes(p);
-    (lds ? interactions_kernel<true> : interactions_kernel<false>)<<<grid_of(s), s.block_size, lds,
-                                                                   st>>>(p, s, ts);
+    launch_kernel(CELER_KERNEL_NAME("interactions"),
+           lds ? interactions_kernel<true> : interactions_kernel<false>, grid_of(s), s.block_size,
+           lds, st, p, s, ts);
     return hipGetLastError();
 }
 
@@ -404,7 +415,7 @@ hipError_t launch_boundary(CoreParamsRef const& p,
                            ThreadSlots const& ts,
                            hipStream_t st)
 {
-    boundary_kernel<<<grid_of(s), s.block_size, 0, st>>>(p, s, ts);
+    launch_kernel("boundary", boundary_kernel, grid_of(s), s.block_size, 0, st, p, s, ts);
     return hipGetLastError();
 }
 
@@ -413,7 +424,7 @@ hipError_t launch_tracking_cut(CoreParamsRef const& p,
                                ThreadSlots const& ts,
                                hipStream_t st)
 {
-    tracking_cut_kernel<<<grid_of(s), s.block_size, 0, st>>>(p, s, ts);
+    launch_kernel("tracking-cut", tracking_cut_kernel, grid_of(s), s.block_size, 0, st, p, s, ts);
     return hipGetLastError();
 }
 
@@ -433,12 +444,12 @@ hipError_t launch_calo_tally(CoreParamsRef const& p,
         return hipSuccess;
     if (num_det > max_lds_detectors)
     {
-        calo_atomic_kernel<<<num_blocks(s.size), block_size, 0, st>>>(p, s, ts);
+        launch_kernel("calo-atomic", calo_atomic_kernel, num_blocks(s.size), block_size, 0, st, p, s, ts);
         return hipGetLastError();
     }
     unsigned int const rows = std::min<unsigned int>(num_blocks(s.size), calo_tally_blocks);
-    calo_partial_kernel<<<rows, block_size, 0, st>>>(p, s, num_det, ts);
-    calo_reduce_kernel<<<num_det, block_size, 0, st>>>(s, num_det, rows);
+    launch_kernel("calo-partial", calo_partial_kernel, rows, block_size, 0, st, p, s, num_det, ts);
+    launch_kernel("calo-reduce", calo_reduce_kernel, num_det, block_size, 0, st, s, num_det, rows);
     return hipGetLastError();
 }
 

@@ -131,6 +131,7 @@ void Stepper::capture_step()
 
 void Stepper::reset_action_times()
 {
+    timed_steps_pending_ = 0;
     std::fill(times_.begin(), times_.end(), 0.0);
     std::fill(counts_.begin(), counts_.end(), 0);
 }
@@ -141,21 +142,26 @@ void Stepper::enqueue_step()
     if (action_times_ && !state_->warming_up())
     {
         // Device time per action from events bracketing its launches on the
-        // state's stream; read back after the step's counter sync
+        // state's stream; read back at the next counter sync (or when the
+        // event ring is full), so the host can run ahead of the device and
+        // the brackets hold no host gaps
         if (events_.empty())
         {
-            events_.resize(2 * actions_.size());
+            events_.resize(2 * actions_.size() * max_timed_steps);
             for (auto& e : events_)
                 CELER_HIP_CALL(hipEventCreate(&e));
         }
+        if (timed_steps_pending_ == max_timed_steps)
+            this->read_action_times();
+        std::size_t const base = 2 * actions_.size() * timed_steps_pending_;
         for (std::size_t i = 0; i < actions_.size(); ++i)
         {
             ScopedProfiling prof(labels_[i].c_str());
-            CELER_HIP_CALL(hipEventRecord(events_[2 * i], state_->stream()));
+            CELER_HIP_CALL(hipEventRecord(events_[base + 2 * i], state_->stream()));
             actions_[i]->step(*params_, *state_);
-            CELER_HIP_CALL(hipEventRecord(events_[2 * i + 1], state_->stream()));
+            CELER_HIP_CALL(hipEventRecord(events_[base + 2 * i + 1], state_->stream()));
         }
-        events_pending_ = true;
+        ++timed_steps_pending_;
     }
     else
     {
@@ -222,20 +228,30 @@ void Stepper::launch_actions()
     }
 }
 
-StepperResult Stepper::make_result()
+void Stepper::read_action_times()
 {
-    CoreCounters const& c = state_->sync_counters();
-    if (events_pending_)
+    if (timed_steps_pending_ == 0)
+        return;
+    std::size_t const per_step = 2 * actions_.size();
+    CELER_HIP_CALL(hipEventSynchronize(events_[per_step * timed_steps_pending_ - 1]));
+    for (std::size_t p = 0; p < timed_steps_pending_; ++p)
     {
         for (std::size_t i = 0; i < actions_.size(); ++i)
         {
             float ms = 0;
-            CELER_HIP_CALL(hipEventElapsedTime(&ms, events_[2 * i], events_[2 * i + 1]));
+            CELER_HIP_CALL(hipEventElapsedTime(
+                &ms, events_[p * per_step + 2 * i], events_[p * per_step + 2 * i + 1]));
             times_[i] += 1e-3 * ms;
             counts_[i] += 1;
         }
-        events_pending_ = false;
     }
+    timed_steps_pending_ = 0;
+}
+
+StepperResult Stepper::make_result()
+{
+    CoreCounters const& c = state_->sync_counters();
+    this->read_action_times();
     CELER_VALIDATE(!(c.error_flags & err_initializer_overflow),
                    << "insufficient capacity (" << params_->init_capacity()
                    << ") for track initializers (created " << c.num_secondaries
@@ -285,14 +301,6 @@ StepperResult Stepper::step_impl(bool new_primaries)
 StepperResult Stepper::run_steps(size_type count)
 {
     CELER_VALIDATE(count > 0, << "no steps requested");
-    if (action_times_)
-    {
-        // Per-action timing reads its events back after every step
-        StepperResult r;
-        for (size_type i = 0; i < count; ++i)
-            r = (*this)();
-        return r;
-    }
     CELER_HIP_CALL(hipMemsetAsync(&state_->ref().counters->num_generated, 0,
                                   sizeof(std::uint32_t), state_->stream()));
     for (size_type i = 0; i < count; ++i)

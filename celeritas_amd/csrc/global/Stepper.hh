@@ -110,8 +110,10 @@ class Stepper
     bool action_times_{false};
     std::vector<double> times_;
     std::vector<std::uint64_t> counts_;
-    std::vector<hipEvent_t> events_;
-    bool events_pending_{false};
+    //! Timed steps whose events are recorded before a readback
+    static constexpr std::size_t max_timed_steps = 16;
+    std::vector<hipEvent_t> events_;  //!< [step][action][begin, end]
+    std::size_t timed_steps_pending_{0};
     bool use_graph_{false};
     hipGraphExec_t graph_exec_{nullptr};
     //! Side streams of concurrent groups (with the state's own stream, one
@@ -125,6 +127,7 @@ class Stepper
 
     StepperResult make_result();
     void launch_actions();
+    void read_action_times();
     StepperResult step_impl(bool new_primaries);
     void launch_step(bool new_primaries);
 };

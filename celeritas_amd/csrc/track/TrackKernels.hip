@@ -15,6 +15,7 @@
 //---------------------------------------------------------------------------//
 #include "TrackKernels.hh"
 
+#include "corecel/KernelLaunch.hh"
 #include "corecel/WaveOps.hh"
 #include "TrackInitImpl.hh"
 
@@ -556,11 +557,12 @@ hipError_t launch_initialize_tracks(CoreParamsRef const& params,
         partition_count_kernel<<<nb, block_size, 0, stream>>>(params, state);
         partition_scan_kernel<<<1, 1024, 0, stream>>>(state, nb);
         partition_scatter_kernel<<<nb, block_size, 0, stream>>>(params, state);
-        init_tracks_charge_kernel<<<nb, block_size, 0, stream>>>(params, state);
+        launch_kernel("init-tracks-charge", init_tracks_charge_kernel, nb, block_size, 0, stream,
+                      params, state);
     }
     else
     {
-        init_tracks_kernel<<<nb, block_size, 0, stream>>>(params, state);
+        launch_kernel("init-tracks", init_tracks_kernel, nb, block_size, 0, stream, params, state);
     }
     return hipGetLastError();
 }
@@ -572,9 +574,10 @@ hipError_t launch_extend_from_secondaries(CoreParamsRef const& params,
     unsigned int const nb = num_blocks(state.size);
     std::uint8_t const in_place = params.scalars.track_order
                                   != TrackOrder::init_charge;
-    locate_alive_kernel<<<nb, block_size, 0, stream>>>(state, in_place);
-    scan_partials_kernel<<<1, 1024, 0, stream>>>(state, nb);
-    process_secondaries_kernel<<<nb, block_size, 0, stream>>>(state, in_place);
+    launch_kernel("locate-alive", locate_alive_kernel, nb, block_size, 0, stream, state, in_place);
+    launch_kernel("scan-partials", scan_partials_kernel, 1, 1024, 0, stream, state, nb);
+    launch_kernel("process-secondaries", process_secondaries_kernel, nb, block_size, 0, stream,
+                  state, in_place);
     return hipGetLastError();
 }
 

@@ -206,6 +206,21 @@ int celer_read_livermore_pe(const char* dir, int z, char* buf, size_t len,
 /* Serialize host-side tables (physics, geometry) as JSON for checkers */
 int celer_params_export_json(const CelerParams* params, char* buf, size_t len,
                              size_t* needed);
+/* Attributes of every step kernel launched so far in this process, one
+ * entry per kernel and launch shape, as a JSON array: registers, scratch,
+ * static and dynamic LDS, resident workgroups per CU, waves per SIMD and
+ * occupancy. Replaces the reference's KernelRegistry / KernelAttributes
+ * (corecel/sys/KernelAttributes.hh:33-121), written to its output JSON. */
+int celer_kernel_attributes_json(char* buf, size_t len, size_t* needed);
+/* Time every following launch of the step kernels (enable = 1) with events
+ * the runtime sets at the dispatch's own start and end, or stop (0). Not
+ * for launches inside a stream capture. The reference times actions with
+ * stream synchronizations (ActionSequence.cc:78-138); this times kernels. */
+int celer_kernel_timing(int enable);
+/* Device time and launch count per kernel and launch shape since the last
+ * reset, as a JSON array (waits for the timed launches to end); a call with
+ * no buffer (a size query) does not reset */
+int celer_kernel_times_json(int reset, char* buf, size_t len, size_t* needed);
 
 /*---------------------------------------------------------------------------*
  * STATE: one CoreState<device> with its own HIP stream
