@@ -102,6 +102,9 @@ def parse_args():
                     help="skip the mixed-precision measurement")
     ap.add_argument("--cylinder-primaries", type=int, default=10000,
                     help="primaries of the barrel-detector line (0: skip)")
+    ap.add_argument("--cylinder-seeds", type=int, default=10,
+                    help="transports (seeds) of the barrel detector in each precision, "
+                         "for the per-volume deposition comparison")
     ap.add_argument("--serve-events", type=int, default=64,
                     help="events of the served TestEm3 workload (0: skip serving)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -302,8 +305,6 @@ def mixed_run(amd, problem, primaries, args, block_size, ndet, fp64_value, fp64_
               fp64_tally):
     """Same workload with the physics kernels in the mixed-precision mode,
     and its per-detector deposition against the FP64 loop's."""
-    from scipy import stats
-
     stepper = make_stepper(amd, problem, args.slots, block_size, args)
     totals, dt, tallies = timed_transports(stepper, primaries, args, ndet, "mixed")
     steps = sum(t["track_steps"] for t in totals)
@@ -318,25 +319,40 @@ def mixed_run(amd, problem, primaries, args, block_size, ndet, fp64_value, fp64_
            "dtype": "f32 transcendentals and draws in sampling, interpolation and MSC; "
                     "f64 positions, safety, field, energies and tallies"}
     # Per-detector deposition against the FP64 loop (itself checked against
-    # the FP64 oracle by the parity tests): each timed transport of the
-    # same primaries is an independent sample, so the spread over
-    # transports gives each detector's standard error
-    if len(tallies) >= 2 and len(fp64_tally) >= 2:
-        mm, ms = tallies.mean(0), tallies.std(0, ddof=1) / np.sqrt(len(tallies))
-        fm, fs = fp64_tally.mean(0), fp64_tally.std(0, ddof=1) / np.sqrt(len(fp64_tally))
-        sel = (ms > 0) & (fs > 0) & (mm + fm > 1e-5 * (mm + fm).max())
-        chi2 = float((((mm[sel] - fm[sel]) ** 2) / (ms[sel] ** 2 + fs[sel] ** 2)).sum())
-        ndf = int(sel.sum())
-        tot_err = float(np.sqrt((ms ** 2).sum() + (fs ** 2).sum()) / fm.sum())
-        rel = (mm[sel] - fm[sel]) / fm[sel]
-        out["edep_vs_fp64"] = {
-            "total_rel": float(mm.sum() / fm.sum() - 1), "total_rel_err": tot_err,
-            "chi2": chi2, "ndf": ndf, "p_value": float(stats.chi2.sf(chi2, ndf)),
-            "max_detector_rel": float(np.max(np.abs(rel))),
-            "samples": f"{len(tallies)} mixed and {len(fp64_tally)} FP64 transports of "
-                       "the same primaries (independent RNG streams)"}
-        log(f"mixed vs FP64 edep: total {out['edep_vs_fp64']['total_rel']:+.2e} +- "
-            f"{tot_err:.1e}, chi2 {chi2:.1f}/{ndf}")
+    # the FP64 oracle by the parity tests)
+    cmp = edep_vs_fp64(tallies, fp64_tally)
+    if cmp:
+        out["edep_vs_fp64"] = cmp
+    return out
+
+
+def edep_vs_fp64(tallies, fp64_tally):
+    """Per-volume deposition of mixed-precision transports against FP64
+    ones: each transport of the same primaries is an independent sample
+    (the RNG streams continue), so the spread over transports gives each
+    volume's standard error; chi^2 over the volumes with signal."""
+    from scipy import stats
+
+    if len(tallies) < 2 or len(fp64_tally) < 2:
+        return None
+    mm, ms = tallies.mean(0), tallies.std(0, ddof=1) / np.sqrt(len(tallies))
+    fm, fs = fp64_tally.mean(0), fp64_tally.std(0, ddof=1) / np.sqrt(len(fp64_tally))
+    sel = (ms > 0) & (fs > 0) & (mm + fm > 1e-5 * (mm + fm).max())
+    chi2 = float((((mm[sel] - fm[sel]) ** 2) / (ms[sel] ** 2 + fs[sel] ** 2)).sum())
+    ndf = int(sel.sum())
+    tot_err = float(np.sqrt((ms ** 2).sum() + (fs ** 2).sum()) / fm.sum())
+    rel = (mm[sel] - fm[sel]) / fm[sel]
+    out = {
+        "total_rel": float(mm.sum() / fm.sum() - 1), "total_rel_err": tot_err,
+        "chi2": chi2, "ndf": ndf, "p_value": float(stats.chi2.sf(chi2, ndf)),
+        "max_volume_rel": float(np.max(np.abs(rel))),
+        "per_volume_rel": [round(float(x), 5) for x in rel],
+        "per_volume_rel_err": [round(float(x), 5) for x in
+                               np.sqrt(ms[sel] ** 2 + fs[sel] ** 2) / fm[sel]],
+        "samples": f"{len(tallies)} mixed and {len(fp64_tally)} FP64 transports of the "
+                   "same primaries (independent RNG streams)"}
+    log(f"mixed vs FP64 edep: total {out['total_rel']:+.2e} +- {tot_err:.1e}, "
+        f"chi2 {chi2:.1f}/{ndf}")
     return out
 
 
@@ -349,20 +365,39 @@ def cylinder_run(amd, wl, args):
     problem = wl.cylinder_problem(physics=physics, init_capacity=1 << 26,
                                   max_events=max(n, 1), track_order=args.track_order)
     sub = argparse.Namespace(**vars(args))
-    sub.steps, sub.warmup = max(1, min(args.steps, 2)), 1
+    # Each timed transport is one seed of the per-volume deposition sample
+    sub.steps, sub.warmup = max(2, args.cylinder_seeds), 1
     block, trials = autotune_block(amd, problem, primaries, args.slots, sub)
     stepper = make_stepper(amd, problem, args.slots, block, sub)
     ndet = max(wl.cylinder_geometry()[2]) + 1
-    totals, dt, _ = timed_transports(stepper, primaries, sub, ndet, "cylinder")
+    totals, dt, fp64_tally = timed_transports(stepper, primaries, sub, ndet, "cylinder")
+    del stepper
     steps = sum(t["track_steps"] for t in totals)
     log(f"cylinder: {steps / dt:.4g} track-steps/s, {sub.steps * n / dt:.4g} events/s")
-    return {"workload": f"Cylindrical detector (Si tracker, PbWO4 ECAL, brass/scintillator "
-                        f"HCAL) in a uniform 3.8 T field, {n} primaries e-/e+/gamma "
-                        "40/20/40, log-uniform 0.5-50 GeV, isotropic, 2^20 slots, FP64, "
-                        "Dormand-Prince, graph replay, autotuned workgroup",
-            "value": steps / dt, "unit": "track-steps/s",
-            "events_per_s": sub.steps * n / dt, "timed_transports": sub.steps,
-            "workgroup_size": block, "autotune_trials": trials}
+    out = {"workload": f"Cylindrical detector (Si tracker, PbWO4 ECAL, brass/scintillator "
+                       f"HCAL) in a uniform 3.8 T field, {n} primaries e-/e+/gamma "
+                       "40/20/40, log-uniform 0.5-50 GeV, isotropic, 2^20 slots, FP64, "
+                       "Dormand-Prince, graph replay, autotuned workgroup",
+           "value": steps / dt, "unit": "track-steps/s",
+           "events_per_s": sub.steps * n / dt, "timed_transports": sub.steps,
+           "workgroup_size": block, "autotune_trials": trials}
+    if args.no_mixed:
+        return out
+    # The same detector, field and source in mixed precision, one transport
+    # per seed, with each volume's deposition against the FP64 transports
+    mixed = dict(problem, precision="mixed")
+    stepper = make_stepper(amd, mixed, args.slots, block, sub)
+    totals, dt, tallies = timed_transports(stepper, primaries, sub, ndet, "cylinder mixed")
+    fails = stepper.state.counters()["num_sampling_failures"]
+    del stepper
+    steps = sum(t["track_steps"] for t in totals)
+    log(f"cylinder mixed: {steps / dt:.4g} track-steps/s, {sub.steps * n / dt:.4g} events/s")
+    out["mixed"] = {"value": steps / dt, "unit": "track-steps/s",
+                    "events_per_s": sub.steps * n / dt,
+                    "speedup_events_vs_fp64": (sub.steps * n / dt) / out["events_per_s"],
+                    "sampling_failures": int(fails),
+                    "edep_vs_fp64": edep_vs_fp64(tallies, fp64_tally)}
+    return out
 
 
 def serve_run(amd, wl, physics, args):
