@@ -9,6 +9,7 @@
 #include <memory>
 #include <vector>
 
+#include "corecel/DeviceRuntime.hh"
 #include "global/CoreData.hh"
 #include "global/Stepper.hh"
 
@@ -32,8 +33,9 @@ struct ServeOptions
     size_type max_queued{0};
     //! Abort after this many step iterations (0: no limit)
     size_type max_steps{0};
-    //! Step iterations launched per counter readback while no event is
-    //! waiting to be admitted (the device runs them back to back)
+    //! Step iterations launched per counter readback while no arrival is
+    //! due to be staged (the device runs them back to back, admitting
+    //! staged events at the start of each)
     size_type steps_per_sync{4};
 };
 
@@ -51,17 +53,22 @@ struct ServeResult
 /*!
  * Transport a stream of events through one track-slot state.
  *
- * Events are admitted in arrival order once they have arrived (wall clock)
- * and while the initializer queue has room, so requests of uneven size
- * share the device with those already in flight: their primaries join the
- * next step, and their tracks fill slots as others free them. Completion
- * is detected on the device: the track-management kernels keep a live
- * count per event (initializers plus tracks) and append an event to a
- * completion list when its count reaches zero. While no arrived event is
- * waiting, the loop launches several steps (graph replays) back to back
- * and reads only the counters and the new completions afterwards; an
- * event's completion time is that of the readback that first shows it.
- * Latency is completion minus arrival.
+ * Admission and completion happen on the device. When an event arrives
+ * (wall clock) the host copies its primaries into a device queue and
+ * raises the queue's arrival count; nothing else about admission is
+ * decided on the host. At the start of every step the device admits
+ * arrived events in order while the queued initializers stay within
+ * \c max_queued (the first waiting event regardless when nothing is queued
+ * or alive), so requests of uneven size share the device with those in
+ * flight: their tracks fill slots as others free them. The track-
+ * management kernels keep a live count per event (initializers plus
+ * tracks) and append an event to a completion list when its count reaches
+ * zero. Every step, admission included, is one replay of the captured step
+ * graph; the host launches several back to back (fewer while arrivals are
+ * pending, so that they are staged promptly) and reads the counters, the
+ * queue's admission count and the new completions afterwards. An event's
+ * admission and completion times are those of the readback that first
+ * shows them; latency is completion minus arrival.
  */
 class ServingLoop
 {
@@ -80,6 +87,15 @@ class ServingLoop
   private:
     std::shared_ptr<CoreParams const> params_;
     std::unique_ptr<Stepper> stepper_;
+    // Device queue (attached to the state; grown and re-attached on demand)
+    DeviceVector<Primary> primaries_;
+    DeviceVector<std::uint32_t> event_first_;
+    DeviceVector<std::uint32_t> event_count_;
+    DeviceVector<std::uint32_t> admit_step_;
+    DeviceVector<ServeCounters> counters_;
+    size_type max_queued_{0};
+
+    void attach_queue(size_type num_primaries, size_type num_events, size_type max_queued);
 };
 
 //---------------------------------------------------------------------------//

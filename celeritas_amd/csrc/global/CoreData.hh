@@ -149,6 +149,37 @@ struct SecondaryStackRef
 };
 
 //---------------------------------------------------------------------------//
+//! Admission counters of the serving queue
+struct ServeCounters
+{
+    std::uint32_t arrived;  //!< events staged so far (written by the host)
+    std::uint32_t admitted;  //!< events admitted so far (device)
+    std::uint32_t first;  //!< first primary admitted this step
+    std::uint32_t count;  //!< primaries admitted this step
+};
+
+/*!
+ * Events waiting for the device to admit them (the serving loop's queue).
+ *
+ * The host stages an arriving event's primaries at its place in
+ * \c primaries (events lie in arrival order, each contiguous, with the
+ * event index as event ID) and then raises \c arrived; at the start of
+ * every step the device admits arrived events in order while the queued
+ * initializers stay within \c max_queued (or the first of them when
+ * nothing is queued or alive) and turns their primaries into initializers.
+ */
+struct ServeQueueRef
+{
+    Primary const* primaries{nullptr};
+    std::uint32_t const* event_first{nullptr};  //!< per event: first primary
+    std::uint32_t const* event_count{nullptr};  //!< per event: primaries
+    std::uint32_t* admit_step{nullptr};  //!< per event: step it was admitted
+    ServeCounters* counters{nullptr};  //!< null: no queue attached
+    size_type num_events{0};  //!< capacity of the event tables
+    size_type max_queued{0};
+};
+
+//---------------------------------------------------------------------------//
 /*!
  * All per-slot mutable state.
  */
@@ -176,6 +207,8 @@ struct CoreStateRef
     //! Optional work counters (null unless diagnostics are enabled), see
     //! DiagCounter
     unsigned long long* diag{nullptr};
+    //! Serving queue admitted from on the device (empty unless attached)
+    ServeQueueRef serve;
 };
 
 //! Largest number of sort keys (actions + 1 for inactive slots)

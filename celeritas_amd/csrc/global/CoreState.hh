@@ -56,6 +56,9 @@ class CoreState
     //! Maintain the per-event live-track and deposition tallies (set before
     //! a step is captured in a graph: launches take the state by value)
     void event_tally(bool on) { ref_.init.event_tally = on ? 1u : 0u; }
+    //! Attach a serving queue the device admits events from at the start of
+    //! each step (set before a step is captured in a graph)
+    void serve_queue(ServeQueueRef const& q) { ref_.serve = q; }
     bool event_tally() const { return ref_.init.event_tally != 0; }
     //! Enqueue an async copy of the counters to the pinned mirror
     void enqueue_counter_readback();

@@ -129,6 +129,16 @@ void Stepper::capture_step()
     use_graph_ = true;
 }
 
+void Stepper::reset_graph()
+{
+    if (graph_exec_)
+    {
+        CELER_HIP_CALL(hipStreamSynchronize(state_->own_stream()));
+        CELER_HIP_CALL(hipGraphExecDestroy(graph_exec_));
+        graph_exec_ = nullptr;
+    }
+}
+
 void Stepper::reset_action_times()
 {
     timed_steps_pending_ = 0;

@@ -87,6 +87,9 @@ class Stepper
     void capture_step();
     //! Whether steps without primaries replay the captured graph
     bool uses_graph() const { return graph_exec_ != nullptr; }
+    //! Drop the captured graph (after the state's device view changed); the
+    //! next step without primaries captures it again
+    void reset_graph();
 
     std::vector<std::shared_ptr<StepActionInterface const>> const& actions() const
     {

@@ -46,6 +46,25 @@ __global__ void primaries_prepare_kernel(CoreCounters* c, size_type count)
         c->prev_secondaries = 0;
 }
 
+//! Initializer of a primary (ProcessPrimariesExecutor.hh:56-76)
+__device__ __forceinline__ TrackInitializer primary_initializer(Primary const& p,
+                                                                std::uint32_t track_id)
+{
+    TrackInitializer ti;
+    ti.track_id = track_id;
+    ti.parent_id = invalid_id;
+    ti.event_id = p.event_id;
+    ti.particle_id = p.particle_id;
+    ti.time = p.time;
+    for (int k = 0; k < 3; ++k)
+    {
+        ti.pos[k] = p.pos[k];
+        ti.dir[k] = p.dir[k];
+    }
+    ti.energy = p.energy;
+    return ti;
+}
+
 /*!
  * Create initializers from primaries (ProcessPrimariesExecutor.hh:56-76).
  *
@@ -66,32 +85,90 @@ __global__ void __launch_bounds__(block_size)
         return;
     Primary const p = primaries[tid];
     size_type const idx = state.counters->init_base + tid;
-    TrackInitializer ti;
+    std::uint32_t track_id = invalid_id;
     if (p.event_id < state.init.max_events)
     {
-        ti.track_id = ranks ? state.init.track_counters[p.event_id] + ranks[tid]
-                            : atomicAdd(&state.init.track_counters[p.event_id], 1u);
+        track_id = ranks ? state.init.track_counters[p.event_id] + ranks[tid]
+                         : atomicAdd(&state.init.track_counters[p.event_id], 1u);
         if (state.init.event_tally)
             atomicAdd(&state.init.event_live[p.event_id], 1);
     }
     else
     {
-        ti.track_id = invalid_id;
         atomicOr(&state.counters->error_flags, err_event_overflow);
     }
-    ti.parent_id = invalid_id;
-    ti.event_id = p.event_id;
-    ti.particle_id = p.particle_id;
-    ti.time = p.time;
-    for (int k = 0; k < 3; ++k)
-    {
-        ti.pos[k] = p.pos[k];
-        ti.dir[k] = p.dir[k];
-    }
-    ti.energy = p.energy;
     if (idx < state.init.capacity)
-        state.init.initializers[idx] = ti;
+        state.init.initializers[idx] = primary_initializer(p, track_id);
 }
+
+//---------------------------------------------------------------------------//
+// DEVICE-SIDE ADMISSION (serving queue)
+//---------------------------------------------------------------------------//
+/*!
+ * Admit arrived events in order while the queued initializers stay within
+ * the limit; the first waiting event is admitted regardless when nothing
+ * is queued or alive. Takes the place of primaries_prepare_kernel: the
+ * admitted primaries go on top of the initializer stack.
+ */
+__global__ void serve_admit_kernel(CoreStateRef const state)
+{
+    if (threadIdx.x != 0)
+        return;
+    ServeQueueRef const& q = state.serve;
+    ServeCounters& sc = *q.counters;
+    CoreCounters& c = *state.counters;
+    std::uint32_t const begin = sc.admitted;
+    std::uint32_t const end = min(sc.arrived, static_cast<std::uint32_t>(q.num_events));
+    std::uint32_t const queued = c.num_initializers;
+    bool const idle = queued == 0 && c.num_alive == 0;
+    std::uint32_t total = 0;
+    std::uint32_t e = begin;
+    for (; e < end; ++e)
+    {
+        std::uint32_t const n = q.event_count[e];
+        if (!(idle && e == begin) && queued + total + n > q.max_queued)
+            break;
+        total += n;
+        q.admit_step[e] = c.steps_taken;
+    }
+    sc.first = begin < end ? q.event_first[begin] : 0;
+    sc.count = total;
+    sc.admitted = e;
+    c.init_base = c.num_initializers;
+    c.num_initializers += total;
+    c.num_generated += total;
+    if (total > 0)
+        c.prev_secondaries = 0;
+}
+
+/*!
+ * Initializers of the primaries admitted this step. A new event's track
+ * counter is zero, so a primary's track ID is its rank within its event;
+ * the event's last primary sets the counter past them.
+ */
+__global__ void __launch_bounds__(block_size) serve_primaries_kernel(CoreStateRef const state)
+{
+    ServeQueueRef const& q = state.serve;
+    std::uint32_t const count = q.counters->count;
+    std::uint32_t const first = q.counters->first;
+    size_type const base = state.counters->init_base;
+    for (std::uint32_t i = blockIdx.x * block_size + threadIdx.x; i < count;
+         i += gridDim.x * block_size)
+    {
+        Primary const p = q.primaries[first + i];
+        std::uint32_t const ev = p.event_id;
+        std::uint32_t const rank = first + i - q.event_first[ev];
+        if (state.init.event_tally)
+            atomicAdd(&state.init.event_live[ev], 1);
+        if (rank + 1 == q.event_count[ev])
+            state.init.track_counters[ev] = q.event_count[ev];
+        if (base + i < state.init.capacity)
+            state.init.initializers[base + i] = primary_initializer(p, rank);
+    }
+}
+
+//! Workgroups of the admission copy (a grid-stride loop over the primaries)
+constexpr unsigned int serve_blocks = 512;
 
 //! Advance per-event counters after ranked ID assignment (last of each event)
 __global__ void __launch_bounds__(block_size)
@@ -532,6 +609,14 @@ hipError_t launch_extend_from_primaries(CoreStateRef const& state,
                                         size_type count,
                                         hipStream_t stream)
 {
+    if (state.serve.counters && count == 0)
+    {
+        // Admission from the serving queue, decided on the device
+        launch_kernel("serve-admit", serve_admit_kernel, 1, wave::size, 0, stream, state);
+        launch_kernel("serve-primaries", serve_primaries_kernel, serve_blocks, block_size, 0,
+                      stream, state);
+        return hipGetLastError();
+    }
     primaries_prepare_kernel<<<1, 1, 0, stream>>>(state.counters, count);
     if (count > 0)
     {

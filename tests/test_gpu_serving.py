@@ -69,6 +69,29 @@ def test_events_overlap(amd, params):
     assert np.sum(r["admit"] <= first_done) >= 2
 
 
+def test_device_admission_in_order_within_limit(amd, params):
+    """The device admits staged events in arrival order while the queued
+    initializers stay within the limit, and an event larger than the limit
+    once nothing is queued or alive."""
+    server = serving.Server(params, 1 << 14, steps_per_sync=1)
+    events = _small_events(12, seed=41)
+    big = wl.served_events(1, seed=42, nmin=60, nmax=60, emin=100.0, emax=200.0)[0]
+    events.insert(5, big)
+    limit = 40
+    assert len(big) > limit and sum(len(e) for e in events) > 2 * limit
+    r = server.run(events, np.zeros(len(events)), max_queued=limit, max_steps=200_000)
+    print("admit", np.round(r["admit"], 4), "done", np.round(r["done"], 4), flush=True)
+    assert np.all(r["done"] > 0)
+    # In order, and not all at once: the limit holds later events back
+    assert np.all(np.diff(r["admit"]) >= 0)
+    assert r["admit"][-1] > r["admit"][0]
+    # The large event waits for the device to empty: everything admitted
+    # before it has completed by the time it is admitted
+    assert np.all(r["done"][:5] <= r["admit"][5])
+    calo = server.state.get("calo_edep", 100)
+    assert r["edep"].sum() == pytest.approx(calo.sum(), rel=1e-12)
+
+
 def test_same_alone_and_in_mix(amd, params):
     events = _small_events(40, seed=11)
     e0 = np.array([e["energy"].sum() for e in events])
