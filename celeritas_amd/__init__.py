@@ -95,6 +95,8 @@ INITIALIZER_DTYPE = np.dtype(
         ("pos", "<f8", (3,)),
         ("dir", "<f8", (3,)),
         ("energy", "<f8"),
+        ("volume", "<u4"),
+        ("reserved", "<u4"),
     ]
 )
 SECONDARY_DTYPE = np.dtype(

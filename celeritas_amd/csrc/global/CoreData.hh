@@ -32,6 +32,10 @@ struct TrackInitializer
     double pos[3];
     double dir[3];
     double energy;
+    //! Volume of the parent's interior it was made in (invalid: locate the
+    //! track from its position)
+    std::uint32_t volume;
+    std::uint32_t reserved;
 };
 
 //! Secondary particle (celeritas/phys/Secondary.hh)
@@ -124,7 +128,6 @@ struct InitStateRef
 {
     std::uint32_t* vacancies{nullptr};
     std::uint32_t* secondary_counts{nullptr};  //!< size+1
-    std::uint32_t* parents{nullptr};
     std::uint32_t* indices{nullptr};
     std::uint32_t* track_counters{nullptr};  //!< per event
     //! Per event: initializers plus live tracks (0 once the event is done)

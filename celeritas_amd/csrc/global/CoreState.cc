@@ -90,7 +90,6 @@ CoreState::CoreState(CoreParams const& params,
     s.rng.size = num_slots;
     add(&s.init.vacancies, n);
     add(&s.init.secondary_counts, n + 1);
-    add(&s.init.parents, n);
     add(&s.init.indices, n);
     s.init.capacity = params.init_capacity();
     s.init.max_events = params.max_events();
@@ -180,7 +179,6 @@ void CoreState::reset()
         CELER_HIP_CALL(hipMemcpyAsync(s.track_slots, shuffled_slots_.data(), n * 4,
                                       hipMemcpyHostToDevice, stream_));
     }
-    CELER_HIP_CALL(launch_fill_u32(s.init.parents, s.size, invalid_id, stream_));
     CELER_HIP_CALL(launch_fill_u32(s.geo.surface, s.size, invalid_id, stream_));
     CELER_HIP_CALL(launch_fill_u32(s.geo.next_surface, s.size, invalid_id, stream_));
     CELER_HIP_CALL(launch_fill_u32(s.sim.track_id, s.size, invalid_id, stream_));

@@ -71,31 +71,29 @@ CELER_FUNCTION void apply_sampling_failure(CoreParamsRef const& params,
  * Initialize a track in a vacant slot from an initializer.
  *
  * Follows InitTracksExecutor (celeritas/track/detail/InitTracksExecutor.hh:
- * 71-175). A secondary made in the previous step starts at its parent's
- * position, which lies inside the parent's volume (interactions happen
- * away from boundaries), so its geometry state is the parent's volume with
- * the state a point location would give there, read from the parent slot
- * (\c parent_slot) instead of searched for; other initializers, and
- * parents on a surface, are located from the position. (The reference
- * copies parent geometry the same way but clears the parent slots at every
- * generate step, ExtendFromPrimariesAction.cc:184-200; here they are
- * cleared only when primaries are inserted, the case that clearing is for.)
+ * 71-175). A secondary starts at its parent's position, which lies inside
+ * the parent's volume when the parent was not on a surface (interactions
+ * happen away from boundaries), so its geometry state is that volume with
+ * the state a point location would give there, taken from the volume the
+ * initializer carries instead of searched for; primaries, and secondaries
+ * of parents on a surface, are located from the position. (The reference
+ * copies the parent's geometry from its slot, for the secondaries of the
+ * step just taken only; carrying the volume in the initializer extends the
+ * same shortcut to secondaries that waited in the queue.)
  */
 CELER_FUNCTION void init_track(CoreParamsRef const& params,
                                CoreStateRef const& state,
                                size_type slot,
-                               TrackInitializer const& ti,
-                               size_type parent_slot = invalid_id)
+                               TrackInitializer const& ti)
 {
     init_sim(state.sim, slot, ti);
     state.particle.particle_id[slot] = static_cast<std::uint8_t>(ti.particle_id);
     state.particle.energy[slot] = ti.energy;
 
     geo::GeoTrack g;
-    if (parent_slot != invalid_id && state.geo.surface[parent_slot] == invalid_id
-        && !state.geo.failed[parent_slot])
+    if (ti.volume != invalid_id)
     {
-        geo::initialize_interior(g, state.geo.volume[parent_slot], ti.pos, ti.dir);
+        geo::initialize_interior(g, ti.volume, ti.pos, ti.dir);
     }
     else
     {

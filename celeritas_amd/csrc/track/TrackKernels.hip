@@ -62,6 +62,8 @@ __device__ __forceinline__ TrackInitializer primary_initializer(Primary const& p
         ti.dir[k] = p.dir[k];
     }
     ti.energy = p.energy;
+    ti.volume = invalid_id;
+    ti.reserved = 0;
     return ti;
 }
 
@@ -215,16 +217,7 @@ __global__ void __launch_bounds__(block_size)
         return;
     TrackInitializer const ti = state.init.initializers[c.init_base - tid - 1];
     size_type const slot = state.init.vacancies[c.vacancy_base - tid - 1];
-    // The previous step's secondaries are the top of the stack: parents[j]
-    // holds the parent slot of its j-th new initializer
-    size_type parent = invalid_id;
-    if (tid < c.prev_secondaries)
-    {
-        size_type const j = c.prev_secondaries - 1 - tid;
-        if (j < state.size)
-            parent = state.init.parents[j];
-    }
-    init_track(params, state, slot, ti, parent);
+    init_track(params, state, slot, ti);
 }
 
 //---------------------------------------------------------------------------//
@@ -507,6 +500,12 @@ __global__ void __launch_bounds__(block_size)
         double pos[3];
         for (int k = 0; k < 3; ++k)
             pos[k] = state.geo.pos[k][slot];
+        // A parent in a volume's interior (not on a surface) hands its volume
+        // to the secondaries made at its position
+        std::uint32_t const volume
+            = (state.geo.surface[slot] == invalid_id && !state.geo.failed[slot])
+                  ? state.geo.volume[slot]
+                  : invalid_id;
         for (size_type i = 0; i < count; ++i)
         {
             Secondary const sec = state.secondaries.storage[start + i];
@@ -525,6 +524,8 @@ __global__ void __launch_bounds__(block_size)
                 ti.pos[k] = pos[k];
                 ti.dir[k] = sec.dir[k];
             }
+            ti.volume = volume;
+            ti.reserved = 0;
             if (!initialized && status != TrackStatus::alive && in_place)
             {
                 // Refill the killed parent's slot in place: keep geometry,
@@ -545,11 +546,6 @@ __global__ void __launch_bounds__(block_size)
             {
                 if (next_idx < state.init.capacity)
                     state.init.initializers[next_idx] = ti;
-                // Parent slot of the j-th new initializer (the slot stays
-                // occupied: a dead parent was refilled in place above)
-                std::uint32_t const j = next_idx - c.init_base;
-                if (in_place && j < state.size)
-                    state.init.parents[j] = slot;
                 ++next_idx;
             }
         }

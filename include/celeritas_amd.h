@@ -87,6 +87,8 @@ typedef struct CelerTrackInitializer
     double pos[3];
     double dir[3];
     double energy;
+    uint32_t volume;   /* parent's interior volume, UINT32_MAX: locate */
+    uint32_t reserved;
 } CelerTrackInitializer;
 
 /* Secondary: celeritas/phys/Secondary.hh (particle_id UINT32_MAX = cut) */

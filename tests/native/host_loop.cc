@@ -322,6 +322,7 @@ void run_worker(CoreParamsRef const& p,
             ti.dir[k] = pr.dir[k];
         }
         ti.energy = pr.energy;
+        ti.volume = invalid_id;  // located from the position
         inits.push_back(ti);
     }
     // The reference pushes primaries in order and pops from the back
@@ -487,6 +488,10 @@ void run_worker(CoreParamsRef const& p,
                     ti.dir[d] = sec.dir[d];
                 }
                 ti.energy = sec.energy;
+                // Made in the parent's volume unless the parent is on a surface
+                ti.volume = (s.geo.surface[slot] == invalid_id && !s.geo.failed[slot])
+                                ? s.geo.volume[slot]
+                                : invalid_id;
                 inits.push_back(ti);
             }
             if (st == TrackStatus::killed)

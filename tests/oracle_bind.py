@@ -281,11 +281,24 @@ def rng_initialize_batch(seed: int, subsequence, offset) -> np.ndarray:
     return out
 
 
-def state_initializers(state: "State") -> np.ndarray:
-    from celeritas_amd import INITIALIZER_DTYPE
+#: The oracle's initializer records (the reference's TrackInitializer fields)
+ORACLE_INITIALIZER_DTYPE = np.dtype(
+    [
+        ("track_id", "<u4"),
+        ("parent_id", "<u4"),
+        ("event_id", "<u4"),
+        ("particle_id", "<u4"),
+        ("time", "<f8"),
+        ("pos", "<f8", (3,)),
+        ("dir", "<f8", (3,)),
+        ("energy", "<f8"),
+    ]
+)
 
+
+def state_initializers(state: "State") -> np.ndarray:
     n = state.counters()["num_initializers"]
-    out = np.zeros(max(n, 1), dtype=INITIALIZER_DTYPE)
+    out = np.zeros(max(n, 1), dtype=ORACLE_INITIALIZER_DTYPE)
     f = load().orc_state_initializers
     f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]
     f.restype = ctypes.c_uint32
