@@ -88,6 +88,10 @@ struct GeoParamsRef
 {
     size_type num_surfaces{0};
     size_type num_volumes{0};
+    //! Contiguous region of the surface, face, logic, box and connectivity
+    //! tables (what a tracking kernel can stage in LDS)
+    char const* hot_base{nullptr};
+    std::uint32_t hot_bytes{0};
 
     // Surfaces
     std::uint8_t const* surf_type{nullptr};

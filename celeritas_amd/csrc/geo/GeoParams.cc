@@ -7,7 +7,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <sstream>
+#include <type_traits>
 
 namespace celeritas_amd
 {
@@ -440,42 +442,64 @@ GeoParams::GeoParams(GeoInput const& inp)
             h_conn_vols.push_back(v);
     }
 
-    // Upload
-    d_surf_type_ = to_device(h_surf_type);
-    d_surf_offset_ = to_device(h_surf_offset);
-    d_surf_data_ = to_device(h_surf_data);
-    d_face_begin_ = to_device(h_face_begin);
-    d_num_faces_ = to_device(h_num_faces);
-    d_max_isect_ = to_device(h_max_isect);
-    d_faces_ = to_device(h_faces);
-    d_logic_begin_ = to_device(h_logic_begin);
-    d_logic_size_ = to_device(h_logic_size);
-    d_logic_ = to_device(h_logic);
-    d_flags_ = to_device(h_flags);
-    d_bbox_ = to_device(h_bbox);
-    d_conn_begin_ = to_device(h_conn_begin);
-    d_conn_size_ = to_device(h_conn_size);
-    d_conn_vols_ = to_device(h_conn_vols);
+    // Upload. The tables that tracking reads per track (surfaces, faces,
+    // volume logic and boxes, surface connectivity) share one contiguous
+    // region that a kernel can copy into LDS and rebase into; the point-
+    // location tree stays apart.
+    std::vector<char> hot;
+    auto place = [&hot](auto const& v) {
+        std::size_t const off = (hot.size() + 15) / 16 * 16;
+        std::size_t const n = v.size() * sizeof(v[0]);
+        hot.resize(off + n);
+        if (n)
+            std::memcpy(hot.data() + off, v.data(), n);
+        return off;
+    };
+    std::size_t const o_surf_type = place(h_surf_type);
+    std::size_t const o_surf_offset = place(h_surf_offset);
+    std::size_t const o_surf_data = place(h_surf_data);
+    std::size_t const o_face_begin = place(h_face_begin);
+    std::size_t const o_num_faces = place(h_num_faces);
+    std::size_t const o_max_isect = place(h_max_isect);
+    std::size_t const o_faces = place(h_faces);
+    std::size_t const o_logic_begin = place(h_logic_begin);
+    std::size_t const o_logic_size = place(h_logic_size);
+    std::size_t const o_logic = place(h_logic);
+    std::size_t const o_flags = place(h_flags);
+    std::size_t const o_bbox = place(h_bbox);
+    std::size_t const o_conn_begin = place(h_conn_begin);
+    std::size_t const o_conn_size = place(h_conn_size);
+    std::size_t const o_conn_vols = place(h_conn_vols);
+    hot.resize((hot.size() + 15) / 16 * 16);
+    d_hot_ = to_device(hot);
+    char const* base = d_hot_.data();
+    auto at = [base](std::size_t off, auto const& v) {
+        using T = std::remove_const_t<std::remove_reference_t<decltype(v[0])>>;
+        return v.empty() ? static_cast<T const*>(nullptr)
+                         : reinterpret_cast<T const*>(base + off);
+    };
     d_bih_nodes_ = to_device(h_bih_nodes);
     d_bih_vols_ = to_device(h_bih_vols);
 
     ref_.num_surfaces = num_surfaces_;
     ref_.num_volumes = num_volumes_;
-    ref_.surf_type = d_surf_type_.data();
-    ref_.surf_offset = d_surf_offset_.data();
-    ref_.surf_data = d_surf_data_.data();
-    ref_.vol_face_begin = d_face_begin_.data();
-    ref_.vol_num_faces = d_num_faces_.data();
-    ref_.vol_max_isect = d_max_isect_.data();
-    ref_.faces = d_faces_.data();
-    ref_.vol_logic_begin = d_logic_begin_.data();
-    ref_.vol_logic_size = d_logic_size_.data();
-    ref_.logic = d_logic_.data();
-    ref_.vol_flags = d_flags_.data();
-    ref_.vol_bbox = d_bbox_.data();
-    ref_.conn_begin = d_conn_begin_.data();
-    ref_.conn_size = d_conn_size_.data();
-    ref_.conn_vols = d_conn_vols_.data();
+    ref_.hot_base = base;
+    ref_.hot_bytes = static_cast<std::uint32_t>(hot.size());
+    ref_.surf_type = at(o_surf_type, h_surf_type);
+    ref_.surf_offset = at(o_surf_offset, h_surf_offset);
+    ref_.surf_data = at(o_surf_data, h_surf_data);
+    ref_.vol_face_begin = at(o_face_begin, h_face_begin);
+    ref_.vol_num_faces = at(o_num_faces, h_num_faces);
+    ref_.vol_max_isect = at(o_max_isect, h_max_isect);
+    ref_.faces = at(o_faces, h_faces);
+    ref_.vol_logic_begin = at(o_logic_begin, h_logic_begin);
+    ref_.vol_logic_size = at(o_logic_size, h_logic_size);
+    ref_.logic = at(o_logic, h_logic);
+    ref_.vol_flags = at(o_flags, h_flags);
+    ref_.vol_bbox = at(o_bbox, h_bbox);
+    ref_.conn_begin = at(o_conn_begin, h_conn_begin);
+    ref_.conn_size = at(o_conn_size, h_conn_size);
+    ref_.conn_vols = at(o_conn_vols, h_conn_vols);
     ref_.bih_nodes = d_bih_nodes_.data();
     ref_.bih_vols = d_bih_vols_.data();
     ref_.num_bih_nodes = static_cast<std::uint32_t>(h_bih_nodes.size());

@@ -91,21 +91,7 @@ class GeoParams
     size_type num_surfaces_{0};
     std::vector<std::string> labels_;
 
-    DeviceVector<std::uint8_t> d_surf_type_;
-    DeviceVector<std::uint32_t> d_surf_offset_;
-    DeviceVector<double> d_surf_data_;
-    DeviceVector<std::uint32_t> d_face_begin_;
-    DeviceVector<std::uint8_t> d_num_faces_;
-    DeviceVector<std::uint8_t> d_max_isect_;
-    DeviceVector<std::uint32_t> d_faces_;
-    DeviceVector<std::uint32_t> d_logic_begin_;
-    DeviceVector<std::uint16_t> d_logic_size_;
-    DeviceVector<std::int32_t> d_logic_;
-    DeviceVector<std::uint8_t> d_flags_;
-    DeviceVector<double> d_bbox_;
-    DeviceVector<std::uint32_t> d_conn_begin_;
-    DeviceVector<std::uint32_t> d_conn_size_;
-    DeviceVector<std::uint32_t> d_conn_vols_;
+    DeviceVector<char> d_hot_;  //!< the tables tracking kernels stage in LDS
     DeviceVector<BihNode> d_bih_nodes_;
     DeviceVector<std::uint32_t> d_bih_vols_;
     GeoParamsRef ref_;

@@ -20,7 +20,7 @@ namespace
 {
 constexpr int block_size = 256;
 
-template<bool UseField, bool Staged>
+template<bool UseField, Staging S>
 __global__ void __launch_bounds__(block_size)
     along_step_charged_kernel(CoreParamsRef const params_in,
                               CoreStateRef const state,
@@ -30,7 +30,7 @@ __global__ void __launch_bounds__(block_size)
     extern __shared__ uint4 staged_tables[];
     if (!workgroup_has_work(ts, state.size))
         return;
-    CoreParamsRef const params = stage_tables<Staged>(params_in, staged_tables);
+    CoreParamsRef const params = stage<S>(params_in, staged_tables);
     size_type slot;
     if (!thread_to_slot(ts, blockIdx.x * blockDim.x + threadIdx.x, state.size, &slot))
         return;
@@ -51,11 +51,21 @@ hipError_t CELER_PREC_FN(launch_along_step_charged)(CoreParamsRef const& p,
                                      ThreadSlots const& ts,
                                      hipStream_t st)
 {
-    unsigned int const lds = staged_table_bytes(p);
-    auto* kernel = p.field.enabled ? (lds ? along_step_charged_kernel<true, true>
-                                          : along_step_charged_kernel<true, false>)
-                                   : (lds ? along_step_charged_kernel<false, true>
-                                          : along_step_charged_kernel<false, false>);
+    // Physics and geometry tables in LDS where both fit, else the physics
+    unsigned int lds = staged_both_bytes(p);
+    Staging const staging = lds ? Staging::both
+                                : (staged_table_bytes(p) ? Staging::physics : Staging::none);
+    if (staging == Staging::physics)
+        lds = staged_table_bytes(p);
+    auto pick = [staging](auto both, auto physics, auto none) {
+        return staging == Staging::both ? both : staging == Staging::physics ? physics : none;
+    };
+    auto* kernel = p.field.enabled ? pick(along_step_charged_kernel<true, Staging::both>,
+                                          along_step_charged_kernel<true, Staging::physics>,
+                                          along_step_charged_kernel<true, Staging::none>)
+                                   : pick(along_step_charged_kernel<false, Staging::both>,
+                                          along_step_charged_kernel<false, Staging::physics>,
+                                          along_step_charged_kernel<false, Staging::none>);
 #if CELER_PREC_IS_MIXED
     char const* name = p.field.enabled ? "along-step-field-mixed" : "along-step-linear-mixed";
 #else

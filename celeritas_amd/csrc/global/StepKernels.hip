@@ -68,12 +68,17 @@ __global__ void __launch_bounds__(block_size)
 //---------------------------------------------------------------------------//
 // ALONG-STEP (NEUTRAL)
 //---------------------------------------------------------------------------//
+template<Staging S>
 __global__ void __launch_bounds__(block_size)
-    along_step_neutral_kernel(CoreParamsRef const params,
+    along_step_neutral_kernel(CoreParamsRef const params_in,
                               CoreStateRef const state,
                               std::uint8_t const action,
                               ThreadSlots const ts)
 {
+    extern __shared__ uint4 staged_tables[];
+    if (!workgroup_has_work(ts, state.size))
+        return;
+    CoreParamsRef const params = stage<S>(params_in, staged_tables);
     size_type slot;
     if (!thread_to_slot(ts, blockIdx.x * blockDim.x + threadIdx.x, state.size, &slot))
         return;
@@ -184,11 +189,16 @@ __global__ void __launch_bounds__(block_size)
 //---------------------------------------------------------------------------//
 // BOUNDARY AND TRACKING CUT
 //---------------------------------------------------------------------------//
+template<Staging S>
 __global__ void __launch_bounds__(block_size)
-    boundary_kernel(CoreParamsRef const params,
+    boundary_kernel(CoreParamsRef const params_in,
                     CoreStateRef const state,
                     ThreadSlots const ts)
 {
+    extern __shared__ uint4 staged_tables[];
+    if (!workgroup_has_work(ts, state.size))
+        return;
+    CoreParamsRef const params = stage<S>(params_in, staged_tables);
     size_type slot;
     if (!thread_to_slot(ts, blockIdx.x * blockDim.x + threadIdx.x, state.size, &slot))
         return;
@@ -349,8 +359,11 @@ hipError_t launch_along_step_neutral(CoreParamsRef const& p,
                                      ThreadSlots const& ts,
                                      hipStream_t st)
 {
-    launch_kernel("along-step-neutral", along_step_neutral_kernel, grid_of(s), s.block_size, 0, st, p, s,
-           action, ts);
+    unsigned int const lds = staged_geo_bytes(p);
+    launch_kernel("along-step-neutral",
+                  lds ? along_step_neutral_kernel<Staging::geometry>
+                      : along_step_neutral_kernel<Staging::none>,
+                  grid_of(s), s.block_size, lds, st, p, s, action, ts);
     return hipGetLastError();
 }
 
@@ -415,7 +428,10 @@ hipError_t launch_boundary(CoreParamsRef const& p,
                            ThreadSlots const& ts,
                            hipStream_t st)
 {
-    launch_kernel("boundary", boundary_kernel, grid_of(s), s.block_size, 0, st, p, s, ts);
+    unsigned int const lds = staged_geo_bytes(p);
+    launch_kernel("boundary",
+                  lds ? boundary_kernel<Staging::geometry> : boundary_kernel<Staging::none>,
+                  grid_of(s), s.block_size, lds, st, p, s, ts);
     return hipGetLastError();
 }
 

@@ -22,14 +22,17 @@ def test_step_kernels_recorded(amd):
     # (workgroups of 256, the tables staged in LDS where a kernel stages)
     entries = [a for a in amd_pkg.kernel_attributes() if a["block_size"] == 256]
     print(sorted({a["name"] for a in entries}))
-    staged = ("pre-step", "along-step-linear", "discrete-select", "interactions")
+    # Physics tables (pre-step, discrete select, interactions), geometry
+    # tables (neutral along-step, boundary) or both (charged along-step) in
+    # dynamic LDS
+    staged = {"pre-step": 32, "discrete-select": 32, "interactions": 32,
+              "along-step-neutral": 32, "boundary": 32, "along-step-linear": 52}
     for name in ("pre-step", "along-step-linear", "along-step-neutral", "discrete-select",
                  "interactions", "boundary", "tracking-cut", "init-tracks",
                  "process-secondaries"):
         shapes = [a for a in entries if a["name"] == name]
         if name in staged:
-            # The staged kernels take the physics tables as dynamic LDS
-            shapes = [a for a in shapes if 0 < a["dynamic_lds"] <= 32 * 1024]
+            shapes = [a for a in shapes if 0 < a["dynamic_lds"] <= staged[name] * 1024]
         else:
             shapes = [a for a in shapes if a["dynamic_lds"] == 0]
         assert shapes, name
