@@ -841,6 +841,16 @@ int celer_hip_calo_tally_launch(const void* params_ref,
                              slots_of(ts), static_cast<hipStream_t>(stream));
 }
 
+int celer_hip_sort_tracks_along_before_pre(const void* params_ref,
+                                           const void* state_ref,
+                                           uint32_t num_keys,
+                                           void* stream)
+{
+    return launch_sort_tracks(state_of(state_ref), SortKey::along_action_of_particle,
+                              num_keys, static_cast<hipStream_t>(stream),
+                              params_of(params_ref).particle_along_action);
+}
+
 int celer_hip_sort_tracks(const void* state_ref,
                           uint32_t key_kind,
                           uint32_t num_keys,

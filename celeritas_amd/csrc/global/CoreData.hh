@@ -229,6 +229,11 @@ enum class SortKey : std::uint8_t
     particle,  //!< by particle type (reindex_particle_type)
     along_action,  //!< by along-step action (reindex_along_step_action)
     post_action,  //!< by post-step action (reindex_step_limit_action)
+    //! By the along-step action the pre-step will choose: the particle's
+    //! (a function of its charge) for an alive or initializing track, none
+    //! for an inactive or errored one. Sorting before the pre-step with it
+    //! gives the permutation that sorting by along_action after it gives.
+    along_action_of_particle,
 };
 
 /*!
@@ -331,6 +336,8 @@ struct CoreParamsRef
     FieldParamsRef field;  //!< uniform magnetic field (disabled: none)
     XorwowJumpTables const* rng_jump{nullptr};
     std::uint32_t rng_seed{0};
+    //! Along-step action the pre-step chooses for each particle type
+    std::uint8_t const* particle_along_action{nullptr};
 };
 
 //---------------------------------------------------------------------------//

@@ -116,6 +116,17 @@ CoreParams::CoreParams(Input inp)
     ref_.field = inp.field;
     ref_.rng_jump = d_jump_.data();
     ref_.rng_seed = rng_->seed();
+
+    // The pre-step's along-step action of each particle type (by charge)
+    std::vector<std::uint8_t> along(std::max<size_type>(phys_->num_particles(), 1),
+                                    invalid_id8);
+    for (size_type p = 0; p < phys_->num_particles(); ++p)
+    {
+        along[p] = phys_->charge(p) == 0 ? sc.along_step_neutral_action
+                                         : sc.along_step_user_action;
+    }
+    d_particle_along_ = to_device(along);
+    ref_.particle_along_action = d_particle_along_.data();
 }
 
 //---------------------------------------------------------------------------//

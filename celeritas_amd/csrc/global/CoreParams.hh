@@ -92,6 +92,7 @@ class CoreParams
     DeviceVector<std::uint32_t> d_vol_to_mat_;
     DeviceVector<std::uint32_t> d_vol_to_det_;
     DeviceVector<XorwowJumpTables> d_jump_;
+    DeviceVector<std::uint8_t> d_particle_along_;
     CoreParamsRef ref_;
 };
 

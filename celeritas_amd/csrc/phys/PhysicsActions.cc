@@ -15,12 +15,21 @@ namespace celeritas_amd
 void PreStepAction::step(CoreParams const& params, CoreState& state) const
 {
     // Pre-step visits every slot once and each slot's result depends only on
-    // that slot, so it reads the slots in place (coalesced) instead of
-    // through the sorted permutation: measured 3% faster per TestEm3
-    // transport than the permuted launch, with identical results
+    // that slot. When the along-step sort has run before it (keyed by the
+    // action the pre-step will choose, so by particle charge) it visits the
+    // slots in that order: a wave then holds tracks of one kind (photons or
+    // charged particles) and evaluates one set of processes; otherwise it
+    // reads the slots in place
+    ThreadSlots ts;
+    TrackOrder const order = params.track_order();
+    if (order == TrackOrder::reindex_along_step_action
+        || order == TrackOrder::reindex_both_action)
+    {
+        ts.slots = state.ref().track_slots;
+    }
     bool const mixed = params.precision() == PrecisionMode::mixed;
     CELER_HIP_CALL((mixed ? launch_pre_step_mixed : launch_pre_step)(
-        params.device_ref(), state.ref(), ThreadSlots{}, state.stream()));
+        params.device_ref(), state.ref(), ts, state.stream()));
 }
 
 void DiscreteSelectAction::step(CoreParams const& params, CoreState& state) const

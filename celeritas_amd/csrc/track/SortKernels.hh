@@ -21,7 +21,8 @@ namespace celeritas_amd
 hipError_t launch_sort_tracks(CoreStateRef const& state,
                               SortKey kind,
                               unsigned int num_keys,
-                              hipStream_t stream);
+                              hipStream_t stream,
+                              std::uint8_t const* along_of_particle = nullptr);
 
 /*!
  * Stable multi-split of the indices 0..n-1 by a byte key per index (keys

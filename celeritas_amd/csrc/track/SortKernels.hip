@@ -44,6 +44,8 @@ struct StateKey
 {
     CoreStateRef state;
     SortKey kind;
+    //! Along-step action per particle type (along_action_of_particle)
+    std::uint8_t const* along_of_particle{nullptr};
 
     __device__ __forceinline__ unsigned int operator()(size_type slot, unsigned int last) const
     {
@@ -60,6 +62,13 @@ struct StateKey
                 break;
             case SortKey::along_action:
                 k = state.sim.along_step_action[slot];
+                break;
+            case SortKey::along_action_of_particle:
+                // The pre-step leaves an errored track's action unset
+                // (invalid) and gives others their particle's
+                if (status == TrackStatus::errored || !along_of_particle)
+                    return last;
+                k = along_of_particle[state.particle.particle_id[slot]];
                 break;
             default:
                 k = state.sim.post_step_action[slot];
@@ -289,9 +298,10 @@ hipError_t launch_split(KeyFn const& key_of,
 hipError_t launch_sort_tracks(CoreStateRef const& state,
                               SortKey kind,
                               unsigned int num_keys,
-                              hipStream_t stream)
+                              hipStream_t stream,
+                              std::uint8_t const* along_of_particle)
 {
-    return launch_split(StateKey{state, kind},
+    return launch_split(StateKey{state, kind, along_of_particle},
                         SplitRefs{state.size, state.sort_counts, state.track_slots,
                                   state.action_offsets},
                         num_keys, stream);

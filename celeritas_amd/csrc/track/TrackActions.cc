@@ -150,8 +150,11 @@ SortTracksAction::SortTracksAction(ActionId id, TrackOrder order, size_type num_
             key_ = SortKey::particle;
             break;
         case TrackOrder::reindex_along_step_action:
-            action_order_ = StepActionOrder::sort_pre;
-            key_ = SortKey::along_action;
+            // Sorted before the pre-step by the action it will choose (the
+            // permutation the reference's sort after it gives), so that the
+            // pre-step too runs over waves of one along-step action
+            action_order_ = StepActionOrder::sort_start;
+            key_ = SortKey::along_action_of_particle;
             break;
         case TrackOrder::reindex_step_limit_action:
             action_order_ = StepActionOrder::sort_pre_post;
@@ -162,9 +165,10 @@ SortTracksAction::SortTracksAction(ActionId id, TrackOrder order, size_type num_
     }
 }
 
-void SortTracksAction::step(CoreParams const&, CoreState& state) const
+void SortTracksAction::step(CoreParams const& params, CoreState& state) const
 {
-    CELER_HIP_CALL(launch_sort_tracks(state.ref(), key_, num_keys_, state.stream()));
+    CELER_HIP_CALL(launch_sort_tracks(state.ref(), key_, num_keys_, state.stream(),
+                                      params.device_ref().particle_along_action));
 }
 
 //---------------------------------------------------------------------------//

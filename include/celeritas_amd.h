@@ -460,6 +460,16 @@ int celer_hip_sort_tracks(const void* state_ref,
                           uint32_t key_kind,
                           uint32_t num_keys,
                           void* stream);
+/* The along-step sort as the sort-tracks-along-step action runs it, before
+ * the pre-step: key = the along-step action the pre-step will choose for
+ * the slot (its particle's, by charge; inactive and errored slots take
+ * num_keys - 1), which orders the slots as the reference's sort by the
+ * along-step action after the pre-step does (SortTracksAction.cc:102-131)
+ * and lets the pre-step run over the same permutation */
+int celer_hip_sort_tracks_along_before_pre(const void* params_ref,
+                                           const void* state_ref,
+                                           uint32_t num_keys,
+                                           void* stream);
 /* The same stable multi-split over a caller's keys (device memory):
  * indices 0..n-1 ordered by keys[i] (values at or above num_keys - 1 take
  * key num_keys - 1), equal keys in increasing index order, into sorted[n],

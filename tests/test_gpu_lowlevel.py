@@ -29,8 +29,7 @@ MODEL_OF_LABEL = {
     "brems-combined": 7,
 }
 ALONG, POST = 7, 11  # StepActionOrder
-SORT_KEY = {"sort-tracks-along-step": 2, "sort-tracks-post-step": 3,
-            "sort-tracks-status": 0, "sort-tracks-particle": 1}
+SORT_KEY = {"sort-tracks-post-step": 3, "sort-tracks-status": 0, "sort-tracks-particle": 1}
 
 
 class ThreadSlots(ctypes.Structure):
@@ -55,6 +54,7 @@ def _lib():
         "celer_hip_tracking_cut_launch": (ctypes.c_int, [vp, vp, vp, vp]),
         "celer_hip_calo_tally_launch": (ctypes.c_int, [vp, vp, u32, vp, vp]),
         "celer_hip_sort_tracks": (ctypes.c_int, [vp, u32, u32, vp]),
+        "celer_hip_sort_tracks_along_before_pre": (ctypes.c_int, [vp, vp, u32, vp]),
         "celer_hip_initialize_tracks": (ctypes.c_int, [vp, vp, vp]),
         "celer_hip_extend_from_secondaries": (ctypes.c_int, [vp, vp, vp]),
     }
@@ -103,6 +103,8 @@ def _low_level_step(lib, params, state, labels, sorted_launch, num_detectors):
                                                  ctypes.byref(ThreadSlots(None, None)), st)
         elif label in SORT_KEY:
             rc = lib.celer_hip_sort_tracks(s, SORT_KEY[label], nact + 1, st)
+        elif label == "sort-tracks-along-step":
+            rc = lib.celer_hip_sort_tracks_along_before_pre(p, s, nact + 1, st)
         else:
             raise AssertionError(f"no low-level launch for action {label}")
         assert rc == 0, (label, rc)

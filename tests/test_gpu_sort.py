@@ -61,26 +61,39 @@ SORTS = {
     # order: [(sort action label, key field)]
     "reindex_status": [("sort-tracks-status", "status")],
     "reindex_particle_type": [("sort-tracks-start", "particle_id")],
-    "reindex_along_step_action": [("sort-tracks-along-step", "along_step_action")],
+    # The along-step sort runs before the pre-step, keyed by the action the
+    # pre-step will choose: the particle's (by charge)
+    "reindex_along_step_action": [("sort-tracks-along-step", "along_of_particle")],
     "reindex_step_limit_action": [("sort-tracks-post-step", "post_step_action")],
     "reindex_both_action": [("sort-tracks-post-step", "post_step_action"),
-                            ("sort-tracks-along-step", "along_step_action")],
+                            ("sort-tracks-along-step", "along_of_particle")],
 }
 
 
-def _host_keys(st, field, num_keys):
+def _along_of_particle(params, physics):
+    labels = params.action_labels()
+    neutral = labels.index("along-step-neutral")
+    charged = labels.index("along-step-general-linear")
+    return np.array([neutral if p["charge"] == 0 else charged
+                     for p in physics["particles"]], dtype=np.int64)
+
+
+def _host_keys(st, field, num_keys, along=None):
     status = st.get("status")
     if field == "status":
         keys = np.zeros(len(status), dtype=np.int64)
+    elif field == "along_of_particle":
+        keys = np.minimum(along[st.get("particle_id").astype(np.int64)], num_keys - 1)
+        keys[status == 3] = num_keys - 1  # errored: the pre-step chooses no action
     else:
         keys = np.minimum(st.get(field).astype(np.int64), num_keys - 1)
     keys[status == 0] = num_keys - 1
     return keys
 
 
-def _check_sorted(st, field, num_keys):
+def _check_sorted(st, field, num_keys, along=None):
     n = st.size
-    keys = _host_keys(st, field, num_keys)
+    keys = _host_keys(st, field, num_keys, along)
     slots = st.get("track_slots").astype(np.int64)
     np.testing.assert_array_equal(np.sort(slots), np.arange(n))
     k = keys[slots]
@@ -108,7 +121,8 @@ def test_sort_invariants(amd, physics, order):
             num_keys = {"status": 2, "particle_id": 4}.get(field, num_actions + 1)
             stepper.state.execute(label)
             print(f"{order} step {step}: {label}", flush=True)
-            _check_sorted(stepper.state, field, num_keys)
+            _check_sorted(stepper.state, field, num_keys,
+                          _along_of_particle(params, physics))
         r = stepper()
         if not r["active"] and not r["queued"]:
             break
