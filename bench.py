@@ -40,6 +40,12 @@ HBM_PEAK_GBS = 8000.0
 
 #: Algorithmic bytes per processed track and per slot scanned, from the SoA
 #: layout in FP64 (fields each kernel must read and write once per track).
+#: Bytes of one discrete selection made inside an along-step kernel: per-
+#: process cross sections (4 x 8), total cross section 8, RNG state 24 + 24,
+#: element 1, post-step action 1, interaction MFP 8 (the particle, energy
+#: and material are already in registers)
+DISCRETE_SELECT_BYTES = 98
+
 #: Action whose byte model applies to a kernel (kernel registry names)
 KERNEL_ACTION = {
     "along-step-linear": "along-step-general-linear",
@@ -566,6 +572,11 @@ def roofline(amd, problem, primaries, slots, track_order, block_size, headline,
         algorithmic = (per_unit + 4) * units
     else:
         algorithmic = per_unit * units + per_slot * slots * launches
+    if kname == "along-step-linear" or kname == "along-step-field":
+        # The discrete selections the kernel makes for the tracks whose step
+        # ends in one: per-process and total cross sections, MFP and RNG
+        # state read, element, action and MFP written
+        algorithmic += DISCRETE_SELECT_BYTES * diag.get("discrete_in_charged", 0)
     per_launch = algorithmic / launches
     avg_launch = secs / launches
     achieved = per_launch / avg_launch / 1e9

@@ -453,7 +453,7 @@ class Stepper:
         out = np.zeros(8, np.uint64)
         _check(f(self.state._h, flag, out.ctypes.data, 8, int(reset)))
         names = ("pre_step", "along_charged", "along_neutral", "boundary", "discrete",
-                 "interaction")
+                 "interaction", "discrete_in_charged")
         return {k: int(v) for k, v in zip(names, out)}
 
     def action_times(self, reset: bool = False) -> dict:

@@ -40,6 +40,7 @@ __global__ void __launch_bounds__(block_size)
     if (state.diag)
         wave::count_active_lanes_spread<diag_spread>(state.diag + diag_along_charged * diag_spread);
     exec::along_step_charged_track<UseField>(params, state, slot, status);
+    fused_discrete_select<true>(params, state, slot);
 }
 
 //---------------------------------------------------------------------------//

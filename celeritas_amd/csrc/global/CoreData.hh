@@ -298,6 +298,8 @@ enum DiagCounter : int
     diag_boundary,  //!< boundary crossings
     diag_discrete,  //!< discrete selections
     diag_interaction,  //!< interactor calls
+    //! discrete selections made inside the charged along-step kernel
+    diag_discrete_charged,
     num_diag_counters = 8
 };
 
@@ -316,7 +318,11 @@ struct CoreScalars
     std::uint8_t along_step_user_action{invalid_id8};
     std::uint8_t along_step_neutral_action{invalid_id8};
     std::uint8_t initialize_tracks_action{invalid_id8};
-    std::uint8_t pad_[2];
+    //! 1: the along-step kernels also run the discrete selection of the
+    //! tracks whose step ends in it (the discrete-select action then
+    //! launches nothing)
+    std::uint8_t fuse_discrete_select{0};
+    std::uint8_t pad_;
     std::uint32_t max_steps{invalid_id};
     std::uint32_t num_actions{0};
     TrackOrder track_order{TrackOrder::none};

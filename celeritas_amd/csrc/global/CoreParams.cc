@@ -108,6 +108,7 @@ CoreParams::CoreParams(Input inp)
     sc.precision = inp.precision;
     fuse_interactions_ = inp.fuse_interactions;
     concurrent_actions_ = inp.concurrent_actions;
+    sc.fuse_discrete_select = inp.fuse_discrete_select ? 1 : 0;
 
     ref_.geo = geo_->device_ref();
     ref_.volume_to_material = d_vol_to_mat_.data();
@@ -248,6 +249,7 @@ std::shared_ptr<CoreParams> CoreParams::from_json(std::string const& text)
     inp.precision = prec == "fp64" ? PrecisionMode::fp64 : PrecisionMode::mixed;
     inp.fuse_interactions = doc.get_bool("fuse_interactions", true);
     inp.concurrent_actions = doc.get_bool("concurrent_actions", true);
+    inp.fuse_discrete_select = doc.get_bool("fuse_discrete_select", true);
     return std::make_shared<CoreParams>(std::move(inp));
 }
 

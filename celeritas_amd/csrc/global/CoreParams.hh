@@ -50,6 +50,8 @@ class CoreParams
         //! actions; the interactions, boundary and tracking cut) on
         //! concurrent streams
         bool concurrent_actions{true};
+        //! Run the discrete selection inside the along-step kernels
+        bool fuse_discrete_select{true};
         //! Along-step action for charged particles (label), created by caller
         std::string along_step_label;
         //! Uniform magnetic field (enabled for the uniform-field along-step)

@@ -23,7 +23,9 @@
 
 #include <cstdint>
 
+#include "corecel/WaveOps.hh"
 #include "CoreData.hh"
+#include "StepExecutors.hh"
 
 namespace celeritas_amd
 {
@@ -183,6 +185,35 @@ __device__ __forceinline__ CoreParamsRef stage(CoreParamsRef const& params, void
         return r;
     }
 }
+
+/*!
+ * Discrete selection at the end of an along-step kernel (when the params
+ * fuse it): the track's step ended in the discrete-select action, so the
+ * selection that action would make next draws from the same RNG stream in
+ * the same order, and the results are those of a separate launch.
+ */
+inline namespace CELER_PREC_NS
+{
+template<bool Charged>
+__device__ __forceinline__ void fused_discrete_select(CoreParamsRef const& params,
+                                                      CoreStateRef const& state,
+                                                      size_type slot)
+{
+    if (!params.scalars.fuse_discrete_select
+        || state.sim.post_step_action[slot] != params.phys.scalars.discrete_action)
+        return;
+    if (state.diag)
+    {
+        wave::count_active_lanes_spread<diag_spread>(state.diag + diag_discrete * diag_spread);
+        if (Charged)
+        {
+            wave::count_active_lanes_spread<diag_spread>(
+                state.diag + diag_discrete_charged * diag_spread);
+        }
+    }
+    exec::discrete_select_track(params, state, slot);
+}
+}  // namespace CELER_PREC_NS
 
 //! Stage the physics tables when \c Staged (see stage)
 template<bool Staged>

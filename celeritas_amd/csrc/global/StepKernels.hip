@@ -88,6 +88,7 @@ __global__ void __launch_bounds__(block_size)
     if (state.diag)
         wave::count_active_lanes_spread<diag_spread>(state.diag + diag_along_neutral * diag_spread);
     exec::along_step_neutral_track(params, state, slot, status);
+    fused_discrete_select<false>(params, state, slot);
 }
 
 //---------------------------------------------------------------------------//

@@ -32,8 +32,16 @@ void PreStepAction::step(CoreParams const& params, CoreState& state) const
         params.device_ref(), state.ref(), ts, state.stream()));
 }
 
+bool DiscreteSelectAction::enqueues_work(CoreParams const& params) const
+{
+    return !params.device_ref().scalars.fuse_discrete_select;
+}
+
 void DiscreteSelectAction::step(CoreParams const& params, CoreState& state) const
 {
+    // Fused: the along-step kernels made the selection
+    if (!this->enqueues_work(params))
+        return;
     bool const mixed = params.precision() == PrecisionMode::mixed;
     CELER_HIP_CALL((mixed ? launch_discrete_select_mixed : launch_discrete_select)(
         params.device_ref(), state.ref(), make_thread_slots(params, state, *this),

@@ -32,6 +32,7 @@ class DiscreteSelectAction final : public StepActionInterface, public ConcreteAc
     {
     }
     void step(CoreParams const&, CoreState&) const final;
+    bool enqueues_work(CoreParams const&) const final;
     StepActionOrder order() const final { return StepActionOrder::pre_post; }
 };
 
