@@ -336,15 +336,24 @@ def edep_vs_fp64(tallies, fp64_tally):
     """Per-volume deposition of mixed-precision transports against FP64
     ones: each transport of the same primaries is an independent sample
     (the RNG streams continue), so the spread over transports gives each
-    volume's standard error; chi^2 over the volumes with signal."""
+    volume's standard error. With a few transports per side each volume's
+    difference is a Welch t statistic of few degrees of freedom (its square
+    averages about 2, not 1), so each t is mapped to the normal quantile of
+    the same probability before the squares are summed: chi^2 over the
+    volumes with signal, ndf = their number."""
     from scipy import stats
 
     if len(tallies) < 2 or len(fp64_tally) < 2:
         return None
-    mm, ms = tallies.mean(0), tallies.std(0, ddof=1) / np.sqrt(len(tallies))
-    fm, fs = fp64_tally.mean(0), fp64_tally.std(0, ddof=1) / np.sqrt(len(fp64_tally))
+    nm, nf = len(tallies), len(fp64_tally)
+    mm, ms = tallies.mean(0), tallies.std(0, ddof=1) / np.sqrt(nm)
+    fm, fs = fp64_tally.mean(0), fp64_tally.std(0, ddof=1) / np.sqrt(nf)
     sel = (ms > 0) & (fs > 0) & (mm + fm > 1e-5 * (mm + fm).max())
-    chi2 = float((((mm[sel] - fm[sel]) ** 2) / (ms[sel] ** 2 + fs[sel] ** 2)).sum())
+    var = ms[sel] ** 2 + fs[sel] ** 2
+    t = (mm[sel] - fm[sel]) / np.sqrt(var)
+    dof = var ** 2 / (ms[sel] ** 4 / (nm - 1) + fs[sel] ** 4 / (nf - 1))
+    z = stats.norm.isf(stats.t.sf(t, dof))
+    chi2 = float((z ** 2).sum())
     ndf = int(sel.sum())
     tot_err = float(np.sqrt((ms ** 2).sum() + (fs ** 2).sum()) / fm.sum())
     rel = (mm[sel] - fm[sel]) / fm[sel]
@@ -356,7 +365,8 @@ def edep_vs_fp64(tallies, fp64_tally):
         "per_volume_rel_err": [round(float(x), 5) for x in
                                np.sqrt(ms[sel] ** 2 + fs[sel] ** 2) / fm[sel]],
         "samples": f"{len(tallies)} mixed and {len(fp64_tally)} FP64 transports of the "
-                   "same primaries (independent RNG streams)"}
+                   "same primaries (independent RNG streams)",
+        "statistic": "per-volume Welch t mapped to normal quantiles, squared and summed"}
     log(f"mixed vs FP64 edep: total {out['total_rel']:+.2e} +- {tot_err:.1e}, "
         f"chi2 {chi2:.1f}/{ndf}")
     return out
