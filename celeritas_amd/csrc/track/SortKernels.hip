@@ -126,84 +126,65 @@ __global__ void __launch_bounds__(block_size)
         refs.counts[k * ntiles + blockIdx.x] = hist[k];
 }
 
-//! Exclusive scan of the histogram table (one workgroup); the first tile
-//! of each key gives the key's thread offset, and offsets[num_keys] = n.
-//! Each thread owns a contiguous chunk: it sums the chunk, one workgroup
-//! scan orders the chunk sums, and the thread rewrites its chunk from its
-//! base (one workgroup-wide scan instead of one per 1024 entries).
-__global__ void __launch_bounds__(1024)
+/*!
+ * Exclusive scan of one key's tile counts (workgroup blockIdx.x = key): the
+ * counts become the key's thread offsets within its range, and the key's
+ * total goes to the totals row after the table (counts[num_keys * ntiles
+ * + key]). The keys' ranges are placed by the scatter kernel, which sums
+ * the totals itself, so no single workgroup walks the whole table.
+ */
+__global__ void __launch_bounds__(block_size)
     sort_scan_kernel(SplitRefs const refs,
                      unsigned int const num_keys,
                      unsigned int const ntiles)
 {
-    constexpr int nw = 1024 / wave::size;
-    __shared__ unsigned int lds[2 * nw];
+    __shared__ unsigned int lds[2 * num_waves];
+    __shared__ unsigned int carry;
+    unsigned int const key = blockIdx.x;
     if (ntiles == 0)
     {
-        // Nothing to split: every key's range is empty
-        for (unsigned int k = threadIdx.x; k <= num_keys; k += 1024)
-            refs.offsets[k] = 0;
+        // Nothing to split (no scatter follows): every key's range is empty
+        if (threadIdx.x == 0)
+        {
+            refs.offsets[key] = 0;
+            if (key == 0)
+                refs.offsets[num_keys] = 0;
+        }
         return;
     }
-    unsigned int* counts = refs.counts;
-    unsigned int const total = num_keys * ntiles;
-    unsigned int const per = (total + 1023) / 1024;
-    unsigned int const begin = min(threadIdx.x * per, total);
-    unsigned int const end = min(begin + per, total);
-    // A chunk of up to max_held words is read with independent loads into
-    // registers (one memory latency instead of one per word) and kept
-    // there for the rewrite; longer chunks are walked word by word
-    constexpr unsigned int max_held = 32;
-    unsigned int held[max_held];
-    bool const in_registers = per <= max_held;
-    unsigned int sum = 0;
-    if (in_registers)
+    if (threadIdx.x == 0)
+        carry = 0;
+    __syncthreads();
+    unsigned int* row = refs.counts + std::size_t(key) * ntiles;
+    // Runs of four tiles per thread, read with independent loads
+    constexpr unsigned int held = 4;
+    for (unsigned int first = 0; first < ntiles; first += block_size * held)
     {
+        unsigned int const begin = first + threadIdx.x * held;
+        unsigned int v[held];
+        unsigned int sum = 0;
 #pragma unroll
-        for (unsigned int u = 0; u < max_held; ++u)
+        for (unsigned int u = 0; u < held; ++u)
         {
-            unsigned int const i = begin + u;
-            held[u] = i < end ? counts[i] : 0u;
+            v[u] = begin + u < ntiles ? row[begin + u] : 0u;
+            sum += v[u];
         }
+        wave::BlockScan2<num_waves> scan(sum, 0u, lds);
+        unsigned int running = carry + scan.excl_a;
 #pragma unroll
-        for (unsigned int u = 0; u < max_held; ++u)
-            sum += held[u];
-    }
-    else
-    {
-        for (unsigned int i = begin; i < end; ++i)
-            sum += counts[i];
-    }
-    wave::BlockScan2<nw> scan(sum, 0u, lds);
-    unsigned int running = scan.excl_a;
-    if (in_registers)
-    {
-#pragma unroll
-        for (unsigned int u = 0; u < max_held; ++u)
+        for (unsigned int u = 0; u < held; ++u)
         {
-            unsigned int const i = begin + u;
-            if (i < end)
-            {
-                counts[i] = running;
-                if (i % ntiles == 0)
-                    refs.offsets[i / ntiles] = running;
-                running += held[u];
-            }
+            if (begin + u < ntiles)
+                row[begin + u] = running;
+            running += v[u];
         }
-    }
-    else
-    {
-        for (unsigned int i = begin; i < end; ++i)
-        {
-            unsigned int const c = counts[i];
-            counts[i] = running;
-            if (i % ntiles == 0)
-                refs.offsets[i / ntiles] = running;
-            running += c;
-        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+            carry += scan.total_a;
+        __syncthreads();
     }
     if (threadIdx.x == 0)
-        refs.offsets[num_keys] = refs.n;
+        refs.counts[std::size_t(num_keys) * ntiles + key] = carry;
 }
 
 /*!
@@ -227,8 +208,24 @@ __global__ void __launch_bounds__(block_size)
     __shared__ unsigned int wcount[num_waves][max_sort_keys];
     int const lane = wave::lane_id();
     int const w = threadIdx.x / wave::size;
-    for (unsigned int k = threadIdx.x; k < num_keys; k += block_size)
-        base[k] = refs.counts[k * ntiles + blockIdx.x];
+    // Key ranges from the key totals (one wave scans all of them); the
+    // first tile also publishes them as the per-key offsets
+    static_assert(max_sort_keys <= wave::size, "one wave scans the key totals");
+    if (threadIdx.x < wave::size)
+    {
+        unsigned int const k = threadIdx.x;
+        unsigned int const t
+            = k < num_keys ? refs.counts[std::size_t(num_keys) * ntiles + k] : 0u;
+        unsigned int const excl = wave::inclusive_scan(t) - t;
+        if (k < num_keys)
+        {
+            base[k] = excl + refs.counts[k * ntiles + blockIdx.x];
+            if (blockIdx.x == 0)
+                refs.offsets[k] = excl;
+        }
+        if (blockIdx.x == 0 && k == 0)
+            refs.offsets[num_keys] = refs.n;
+    }
     size_type const tile_base = blockIdx.x * tile_size;
     for (int r = 0; r < items_per_thread; ++r)
     {
@@ -285,7 +282,7 @@ hipError_t launch_split(KeyFn const& key_of,
     unsigned int const nt = num_tiles(refs.n);
     if (nt > 0)
         sort_count_kernel<<<nt, block_size, 0, stream>>>(key_of, refs, num_keys, nt);
-    sort_scan_kernel<<<1, 1024, 0, stream>>>(refs, num_keys, nt);
+    sort_scan_kernel<<<num_keys, block_size, 0, stream>>>(refs, num_keys, nt);
     if (nt > 0)
         sort_scatter_kernel<<<nt, block_size, 0, stream>>>(key_of, refs, num_keys, nt);
     return hipGetLastError();
@@ -325,9 +322,10 @@ hipError_t launch_partition_keys(std::uint8_t const* keys,
 namespace
 {
 //! Histogram table of the allocation-free partition: max_sort_keys words
-//! per tile for up to max_partition_noalloc indices (512 KiB)
+//! per tile for up to max_partition_noalloc indices plus a row of key
+//! totals (512 KiB + 256 B)
 constexpr size_type noalloc_scratch_words
-    = max_sort_keys * ((max_partition_noalloc + tile_size - 1) / tile_size);
+    = max_sort_keys * ((max_partition_noalloc + tile_size - 1) / tile_size + 1);
 
 __device__ __align__(256) std::uint32_t scratch[noalloc_scratch_words];
 
@@ -362,7 +360,8 @@ hipError_t partition_noalloc_scratch(std::uint32_t** ptr)
 
 size_type sort_counts_size(size_type num_slots)
 {
-    return max_sort_keys * num_tiles(num_slots);
+    // Per-tile histograms plus the row of key totals
+    return max_sort_keys * (num_tiles(num_slots) + 1);
 }
 
 //---------------------------------------------------------------------------//
