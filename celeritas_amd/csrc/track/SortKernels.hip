@@ -11,9 +11,11 @@
 //! with thrust::sort_by_key / std::sort (unstable) and then copies the
 //! per-action offsets to the host. Here the keys are few (at most
 //! max_sort_keys) so the sort is a three-pass stable multi-split over the
-//! slot index: per-workgroup key histograms in LDS, one exclusive scan over
-//! the key-major histogram table (which also yields the offsets), and a
-//! scatter that ranks equal keys inside each wave with ballots. The result
+//! slot index: per-tile key histograms in LDS, an exclusive scan of each
+//! key's row of the key-major histogram table (one workgroup per key), and
+//! a scatter that places the key ranges from the key totals and ranks equal
+//! keys inside each wave with ballots. Tiles of 1,024 indices give a 2^20
+//! slot sort 1,024 workgroups, four waves per SIMD. The result
 //! lists, for each key, its slots in increasing order, so the gathers of a
 //! sorted launch stay monotonic; the offsets stay on the device and sorted
 //! kernels read their range from there, so no host round trip is needed.
@@ -31,7 +33,7 @@ namespace
 //---------------------------------------------------------------------------//
 constexpr int block_size = 256;
 constexpr int num_waves = block_size / wave::size;
-constexpr int items_per_thread = 8;
+constexpr int items_per_thread = 4;
 constexpr int tile_size = block_size * items_per_thread;
 
 inline unsigned int num_tiles(size_type n)
@@ -323,7 +325,7 @@ namespace
 {
 //! Histogram table of the allocation-free partition: max_sort_keys words
 //! per tile for up to max_partition_noalloc indices plus a row of key
-//! totals (512 KiB + 256 B)
+//! totals (1 MiB + 256 B)
 constexpr size_type noalloc_scratch_words
     = max_sort_keys * ((max_partition_noalloc + tile_size - 1) / tile_size + 1);
 

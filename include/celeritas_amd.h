@@ -474,7 +474,7 @@ int celer_hip_sort_tracks_along_before_pre(const void* params_ref,
  * indices 0..n-1 ordered by keys[i] (values at or above num_keys - 1 take
  * key num_keys - 1), equal keys in increasing index order, into sorted[n],
  * and offsets[0..num_keys] (offsets[num_keys] = n), all on the device.
- * Allocates its histogram scratch (4 * 64 * (ceil(n / 2048) + 1) bytes) on the
+ * Allocates its histogram scratch (4 * 64 * (ceil(n / 1024) + 1) bytes) on the
  * stream and frees it there. The track-slot form is celer_hip_sort_tracks
  * (detail/TrackSortUtils.cu:145-206 sorts track_slots by per-slot action
  * keys with thrust::sort_by_key). */
@@ -487,7 +487,7 @@ int celer_hip_partition_keys(const uint8_t* keys,
 /* Allocation-free celer_hip_partition_keys. No counterpart in the
  * reference: for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
- * Its histogram scratch is a static device array of 512.25 KiB per GPU, held
+ * Its histogram scratch is a static device array of 1 MiB + 256 B per GPU, held
  * until the process ends, so n is limited to CELER_PARTITION_NOALLOC_MAX
  * (larger n fails with hipErrorInvalidValue and a message from
  * celer_amd_last_error()). The array keeps the last call's contents, every

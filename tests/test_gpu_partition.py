@@ -5,7 +5,7 @@ clamped keys and the cumulative key counts).
 The reference sorts track slots by per-slot action keys with
 thrust::sort_by_key (track/detail/TrackSortUtils.cu:145-206); the split
 here is stable, so the permutation is fixed exactly. Cases: empty and
-single-element inputs, sizes around the 2048-index tile, a million
+single-element inputs, sizes around the 1024-index tile, a million
 indices, one key, 64 keys, keys beyond the last (clamped), constant,
 ascending and descending keys.
 """
@@ -45,8 +45,10 @@ CASES = [
     ("empty", 0, 5, "random"),
     ("single", 1, 3, "random"),
     ("one key", 3000, 1, "random"),
-    ("tile", 2048, 7, "random"),
-    ("tile+1", 2049, 7, "random"),
+    ("tile", 1024, 7, "random"),
+    ("tile+1", 1025, 7, "random"),
+    ("two tiles", 2048, 7, "random"),
+    ("two tiles+1", 2049, 7, "random"),
     ("clamped", 10_000, 17, "wide"),
     ("constant", 5000, 64, "constant"),
     ("ascending", 4097, 64, "ascending"),
